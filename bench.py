@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Benchmark: filtered candidates/s of the plain GPU form at CC18 depth (config 2:
+M = 31#, depth 18, 123 552 classes, filter stages 37..2^10 / 2^10..2^14 / 2^14..2^16,
+2^36 candidates per launch from a seeded start in [2^86, 2^89), Baillie-PSW prove of the
+survivors).  One step = one filter launch over the whole batch + the prove launch over its
+survivors, inputs resident on the GPU.  Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "cunningham-chain-search_amd")
+for p in (ROOT, PKG):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+# published campaign baseline (RTX 4090/5090, v13 3-stage filter, depth 18-20): 57-65 G/s
+PUBLISHED_GPU_CAND_PER_S = 65e9
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_threads():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(name, t0, seconds):
+    """The oracle's CPU restatement of the reference CPU path, all host threads, on a
+    bounded sample of the same workload (consecutive tiles from the same start)."""
+    from oracle import pyoracle
+    from ccs import workloads as WL
+    p = WL.shape_params(name)
+    o = pyoracle.Shape(p["crt"], p["wheel"], p["target"], p["depth"], p["filter_primes"], p["pos0_max_q"])
+    nt = cpu_threads()
+    tiles = 6 * nt * 4
+    t = time.perf_counter()
+    o.filter(t0, tiles, cap=1 << 20, nthreads=nt)
+    dt = time.perf_counter() - t
+    tiles = int(max(tiles, min(6 * 10 ** 6, tiles * seconds / max(dt, 1e-3))))
+    tiles -= tiles % 6
+    t = time.perf_counter()
+    surv, counts = o.filter(t0, tiles, cap=1 << 22, cuts=[], nthreads=nt)
+    dt = time.perf_counter() - t
+    cand = counts[0]
+    import platform
+    model = platform.processor() or "cpu"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": cand / dt, "unit": "candidates/s", "cores": nt, "kind": "port",
+            "sample": f"{tiles} tiles = {cand} candidates of the same shape and start, "
+                      f"{dt:.1f} s on {nt} threads ({model}); v33-structured per-candidate filter, sieve only"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", default="c2", choices=["c2", "reference", "c3"])
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this pipeline shards by disjoint tile ranges with no "
+              "collective, and multi-GPU runs are not part of this build (use --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import ccs
+    from ccs import workloads as WL
+
+    ctx = ccs.Context(0)
+    name = args.config
+    shape = WL.make_shape(ctx, name)
+    if name == "c2":
+        t0, tc, n_start = WL.c2_range()
+        workload = ("config 2: M = 31# (CRT 5..19 x wheel 23,29,31 x mod 6), depth 18, 123552 classes, "
+                    "filter primes 37..65521 in stages 37..1021 / 1031..16381 / 16411..65521, "
+                    "2^36 candidates per launch from a seeded start in [2^86, 2^89), BPSW prove")
+    elif name == "c3":
+        t0, tc, n_start = WL.c3_range()
+        workload = "config 3: M = 37#, depth 19, filter primes to 2^18, 2^38 candidates per launch"
+    else:
+        t0, tc = WL.reference_range(bits=89, n_tiles=3337446)
+        n_start = t0 * shape.WP * shape.M
+        workload = "v13 campaign shape (primes 37..863), 89-bit, 3337446 tiles"
+    bits = n_start.bit_length()
+    cand = shape.count_candidates(t0, tc)
+    cap, pcap = 1 << 20, 1 << 16
+    d_surv = ctx.dalloc(cap * 16)
+    d_count = ctx.dalloc(64)
+    d_res = ctx.dalloc(pcap * 32)
+
+    def step():
+        shape.filter_launch(t0, tc, d_surv, cap, d_count)
+        shape.prove_launch(t0, d_surv, d_count, pcap, d_res, bits=bits)
+
+    log(f"[bench] {name}: tiles={tc} candidates/step={cand} tuning={shape.tuning}")
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    ctx.profile_start()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    dt = time.perf_counter() - t
+    kt = ctx.profile_stop()
+    n_surv = np.zeros(1, np.uint32)
+    ctx.d2h(n_surv, d_count)
+    res = np.zeros(min(int(n_surv[0]), pcap), ccs.CHAIN_RESULT_DTYPE)
+    if len(res):
+        ctx.d2h(res, d_res)
+    value = cand * args.steps / dt
+    ms_sieve = kt["ms_sieve"] / max(kt["n_sieve"], 1)
+    # algorithmic work: E[residue tests] per candidate under ascending-prime early exit,
+    # 4 int32 VALU ops per test, against the int32 VALU peak
+    fs = []
+    for q in shape.filter_primes:
+        lo = 0 if q <= shape.pos0_max_q else 1
+        fs.append((q, len(WL.forbidden(q, lo, shape.depth))))
+    e_tests, surv_frac = WL.expected_tests(fs)
+    ops = cand * e_tests * WL.INT32_OPS_PER_TEST
+    achieved = ops / (ms_sieve / 1e3) / 1e12
+    out = {
+        "metric": "filtered candidates/s at CC18 depth (bit-exact)",
+        "value": value,
+        "unit": "candidates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / PUBLISHED_GPU_CAND_PER_S,
+        "dtype": "int32 (filter), u128 (prove)",
+        "data": "synthetic: seeded tile range (xorshift64), no dataset",
+        "config": {"workload": workload, "tile_start": str(t0), "tile_count": tc,
+                   "candidates_per_step": cand, "tuning": shape.tuning},
+        "roofline": {"bound": "valu_int32", "achieved": achieved, "peak": WL.INT32_PEAK_TOPS,
+                     "unit": "Tops/s (int32 VALU)", "frac": achieved / WL.INT32_PEAK_TOPS,
+                     "traffic": None, "kernel": "sieve_kernel",
+                     "kernel_ms": ms_sieve, "e_tests_per_candidate": e_tests,
+                     "ops_per_launch": ops},
+        "kernels_ms": {"sieve": ms_sieve, "deep": kt["ms_deep"] / max(kt["n_deep"], 1),
+                       "prove": kt["ms_prove"] / max(kt["n_prove"], 1)},
+        "survivors_per_step": int(n_surv[0]),
+        "primes_per_step": int(((res["flags"] & ccs.F_PRIME) != 0).sum()),
+        "analytic_survival": surv_frac,
+    }
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(name, t0, args.cpu_seconds)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

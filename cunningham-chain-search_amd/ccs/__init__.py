@@ -1,0 +1,249 @@
+"""Host-side mirror of the reference's filter/prove interface over libccs.so.
+
+The reference (nmicic/cunningham-chain-search) exposes its pipeline through one engine
+per file: tables + CRT bases (v13.cu:793-1059), the K1 filter launch over a tile batch
+(v13.cu:3764-3796), and the prover over the survivors (v13.cu:1794-2068).  This module
+gives the same operations as Python calls on top of the C ABI (include/ccs.h); all
+compute runs in the HIP kernels.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import (CHAIN_RESULT_DTYPE, JOB_SURVIVOR_DTYPE, SURVIVOR_DTYPE, CcsError,  # noqa: F401
+                   F_BITS_OK, F_NONROOT, F_OVERFLOW, F_PRIME, F_TRIAL_OK, F_UNSUPPORTED)
+
+__all__ = ["Context", "Shape", "CcsError", "device_count", "format_results", "u128_split"]
+
+
+def u128_split(v):
+    return v & ((1 << 64) - 1), v >> 64
+
+
+def device_count():
+    n = C.c_int(0)
+    L.check(L.load().ccs_device_count(C.byref(n)), "ccs_device_count")
+    return n.value
+
+
+class Context:
+    """One GPU: stream, scratch buffers and timing events (ccs_ctx)."""
+
+    def __init__(self, device=0):
+        self.lib = L.load()
+        self.h = C.c_void_p()
+        L.check(self.lib.ccs_ctx_create(device, C.byref(self.h)), "ccs_ctx_create")
+
+    def close(self):
+        if self.h:
+            self.lib.ccs_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        L.check(self.lib.ccs_synchronize(self.h), "ccs_synchronize")
+
+    def profile_start(self):
+        L.check(self.lib.ccs_profile_start(self.h), "ccs_profile_start")
+
+    def profile_stop(self):
+        t = L.KernelTimes()
+        L.check(self.lib.ccs_profile_stop(self.h, C.byref(t)), "ccs_profile_stop")
+        return t.as_dict()
+
+    # device buffers for the asynchronous entry points
+    def dalloc(self, nbytes):
+        p = C.c_void_p()
+        L.check(self.lib.ccs_dmalloc(self.h, C.byref(p), nbytes), "ccs_dmalloc")
+        return p
+
+    def dfree(self, p):
+        L.check(self.lib.ccs_dfree(self.h, p), "ccs_dfree")
+
+    def h2d(self, dptr, arr):
+        arr = np.ascontiguousarray(arr)
+        L.check(self.lib.ccs_memcpy_h2d(self.h, dptr, arr.ctypes.data, arr.nbytes), "h2d")
+
+    def d2h(self, arr, dptr):
+        L.check(self.lib.ccs_memcpy_d2h(self.h, arr.ctypes.data, dptr, arr.nbytes), "d2h")
+        return arr
+
+
+def _u32arr(xs):
+    a = np.ascontiguousarray(np.asarray(list(xs), dtype=np.uint32))
+    return a, a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+class Shape:
+    """CRT shape + filter profile resident on the GPU (ccs_shape).
+
+    n = (tile*WP + k)*M + base, M = prod(crt), WP = prod(wheel), n = 5 (mod 6);
+    filter primes tested at positions [0, depth) (q <= pos0_max_q) or [1, depth)."""
+
+    def __init__(self, ctx, crt, wheel, target, depth, filter_primes, stage_end=(),
+                 pos0_max_q=97, classes=True, tuning=None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self._keep = []
+        crt_a, crt_p = _u32arr(crt)
+        wh_a, wh_p = _u32arr(wheel)
+        fp_a, fp_p = _u32arr(filter_primes)
+        st_a, st_p = _u32arr(stage_end)
+        self._keep += [crt_a, wh_a, fp_a, st_a]
+        self.crt, self.wheel, self.filter_primes = list(crt), list(wheel), [int(x) for x in fp_a]
+        self.target, self.depth, self.pos0_max_q = target, depth, pos0_max_q
+        self.stage_end = list(stage_end)
+        p = L.ShapeParams(len(crt_a), crt_p, len(wh_a), wh_p, target, depth, len(fp_a), fp_p,
+                          len(st_a), st_p, pos0_max_q, 0 if classes else L.CCS_SHAPE_NO_CLASSES)
+        self.h = C.c_void_p()
+        tp = None
+        if tuning is not None:
+            tp = C.byref(tuning if isinstance(tuning, L.Tuning) else L.Tuning(**tuning))
+        L.check(self.lib.ccs_shape_build(ctx.h, C.byref(p), tp, C.byref(self.h)), "ccs_shape_build")
+        self._refresh_info()
+
+    def _refresh_info(self):
+        info = L.ShapeInfo()
+        L.check(self.lib.ccs_shape_query(self.h, C.byref(info)), "ccs_shape_query")
+        self.info = info
+        self.M, self.WP, self.W = int(info.M), int(info.WP), int(info.W)
+        self.n_bases, self.n_classes = int(info.n_bases), int(info.n_classes)
+
+    def close(self):
+        if self.h:
+            self.lib.ccs_shape_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def tuning(self):
+        t = L.Tuning()
+        L.check(self.lib.ccs_shape_get_tuning(self.h, C.byref(t)), "get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = L.Tuning(**{**self.tuning, **kw})
+        L.check(self.lib.ccs_shape_set_tuning(self.h, C.byref(t)), "ccs_shape_set_tuning")
+        self._refresh_info()
+
+    def bases(self):
+        out = np.zeros(self.n_bases, np.uint64)
+        L.check(self.lib.ccs_shape_bases(self.h, out.ctypes.data, self.n_bases), "bases")
+        return out
+
+    def classes(self):
+        n = self.n_classes
+        a = np.zeros(n, np.uint64)
+        b = np.zeros(n, np.uint32)
+        k = np.zeros(n, np.uint32)
+        r = np.zeros(n, np.uint32)
+        L.check(self.lib.ccs_shape_classes(self.h, a.ctypes.data, b.ctypes.data, k.ctypes.data,
+                                           r.ctypes.data, n), "classes")
+        return a, b, k, r
+
+    def count_candidates(self, tile_start, tile_count):
+        return int(self.lib.ccs_count_candidates(self.h, L.U128.of(tile_start), tile_count))
+
+    def filter(self, tile_start, tile_count, cap=1 << 20):
+        """Plain form: survivors of tiles [tile_start, tile_start + tile_count) as a
+        structured array of v13 Survivor records (unordered), and the stage counters."""
+        out = np.zeros(cap, SURVIVOR_DTYPE)
+        n = C.c_uint32(0)
+        st = L.FilterStats()
+        rc = self.lib.ccs_filter(self.ctx.h, self.h, L.U128.of(tile_start), tile_count,
+                                 out.ctypes.data, cap, C.byref(n), C.byref(st))
+        L.check(rc, "ccs_filter")
+        if rc == L.CCS_OVERFLOW:
+            raise CcsError(f"survivor overflow: {n.value} > cap {cap}")
+        return out[: n.value].copy(), st.as_dict()
+
+    def filter_launch(self, tile_start, tile_count, d_out, cap, d_count, stream=None):
+        """Asynchronous plain filter into device buffers (ccs_filter_launch)."""
+        L.check(self.lib.ccs_filter_launch(self.ctx.h, self.h, L.U128.of(tile_start), tile_count,
+                                           d_out, cap, d_count, stream), "ccs_filter_launch")
+
+    def prove_launch(self, tile_start, d_surv, d_count, cap, d_out, bits=0, test="bpsw", bases=(2,),
+                     stream=None):
+        """Asynchronous prove of device survivors (ccs_prove_launch)."""
+        pp = _prove_params(bits, test, bases)
+        L.check(self.lib.ccs_prove_launch(self.ctx.h, self.h, L.U128.of(tile_start), d_surv, d_count, cap,
+                                          C.byref(pp), d_out, stream), "ccs_prove_launch")
+
+    def prove(self, tile_start, surv, bits=0, test="bpsw", bases=(2,)):
+        surv = np.ascontiguousarray(surv, SURVIVOR_DTYPE)
+        out = np.zeros(len(surv), CHAIN_RESULT_DTYPE)
+        pp = _prove_params(bits, test, bases)
+        L.check(self.lib.ccs_prove(self.ctx.h, self.h, L.U128.of(tile_start), surv.ctypes.data,
+                                   len(surv), C.byref(pp), out.ctypes.data), "ccs_prove")
+        return out
+
+    def batch(self, jobs, cap=1 << 20):
+        """Batched form: one persistent launch over class-form jobs (a, u0, count)."""
+        jarr = jobs_array(jobs)
+        out = np.zeros(cap, JOB_SURVIVOR_DTYPE)
+        n = C.c_uint32(0)
+        st = L.FilterStats()
+        rc = self.lib.ccs_batch(self.ctx.h, self.h, jarr.ctypes.data, len(jarr), out.ctypes.data, cap,
+                                C.byref(n), C.byref(st))
+        L.check(rc, "ccs_batch")
+        if rc == L.CCS_OVERFLOW:
+            raise CcsError(f"survivor overflow: {n.value} > cap {cap}")
+        return out[: n.value].copy(), st.as_dict()
+
+    def prove_jobs(self, jobs, surv, bits=0, test="bpsw", bases=(2,)):
+        jarr = jobs_array(jobs)
+        surv = np.ascontiguousarray(surv, JOB_SURVIVOR_DTYPE)
+        out = np.zeros(len(surv), CHAIN_RESULT_DTYPE)
+        pp = _prove_params(bits, test, bases)
+        L.check(self.lib.ccs_prove_jobs(self.ctx.h, self.h, jarr.ctypes.data, len(jarr), surv.ctypes.data,
+                                        len(surv), C.byref(pp), out.ctypes.data), "ccs_prove_jobs")
+        return out
+
+    def n_of(self, tile_start, s):
+        """n of a plain survivor record (v13.cu:1819-1826)."""
+        return ((tile_start + int(s["tile_idx"])) * self.WP + int(s["k_offset"])) * self.M + \
+            int(self.bases()[int(s["base_idx"])])
+
+
+def jobs_array(jobs):
+    if isinstance(jobs, np.ndarray) and jobs.dtype == L.JOB_DTYPE:
+        return np.ascontiguousarray(jobs)
+    arr = np.zeros(len(jobs), L.JOB_DTYPE)
+    for i, (a, u0, count) in enumerate(jobs):
+        arr[i] = (a, u0 & ((1 << 64) - 1), u0 >> 64, count)
+    return arr
+
+
+def _prove_params(bits, test, bases):
+    pp = L.ProveParams()
+    pp.bits = bits
+    pp.test_kind = L.CCS_TEST_BPSW if test == "bpsw" else L.CCS_TEST_MR
+    bl = list(bases)
+    pp.n_bases = len(bl)
+    for i, b in enumerate(bl):
+        pp.bases[i] = b
+    return pp
+
+
+def format_results(res, log_threshold=8, target=18):
+    """Results-file lines in the v13 format (v13.cu:2030-2033)."""
+    lib = L.load()
+    res = np.ascontiguousarray(res, CHAIN_RESULT_DTYPE)
+    need = C.c_uint64(0)
+    L.check(lib.ccs_format_results(res.ctypes.data, len(res), log_threshold, target, None, 0,
+                                   C.byref(need)), "format")
+    buf = C.create_string_buffer(need.value + 1)
+    L.check(lib.ccs_format_results(res.ctypes.data, len(res), log_threshold, target, buf,
+                                   need.value + 1, C.byref(need)), "format")
+    return buf.value.decode()

@@ -1,0 +1,163 @@
+"""ctypes binding of libccs.so (the C ABI in include/ccs.h).
+
+The library must exist (built by cunningham-chain-search_amd/build.py); there is no
+fallback: every compute call goes to the HIP kernels.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libccs.so")
+
+CCS_OK, CCS_OVERFLOW = 0, 1
+CCS_TEST_BPSW, CCS_TEST_MR = 0, 1
+CCS_SHAPE_NO_CLASSES = 1
+F_BITS_OK, F_TRIAL_OK, F_PRIME, F_NONROOT, F_OVERFLOW, F_UNSUPPORTED = 1, 2, 4, 8, 16, 32
+
+EXPORTS = [
+    "ccs_ctx_create", "ccs_ctx_destroy", "ccs_device_count", "ccs_last_error",
+    "ccs_dmalloc", "ccs_dfree", "ccs_memcpy_h2d", "ccs_memcpy_d2h", "ccs_memset_d",
+    "ccs_synchronize", "ccs_shape_build", "ccs_shape_free", "ccs_shape_query",
+    "ccs_shape_bases", "ccs_shape_classes", "ccs_shape_set_tuning", "ccs_shape_get_tuning",
+    "ccs_count_candidates", "ccs_filter_launch", "ccs_filter", "ccs_last_stats",
+    "ccs_prove_launch", "ccs_prove", "ccs_format_results", "ccs_batch_launch", "ccs_batch",
+    "ccs_prove_jobs", "ccs_profile_start", "ccs_profile_stop",
+]
+
+
+class U128(C.Structure):
+    _fields_ = [("lo", C.c_uint64), ("hi", C.c_uint64)]
+
+    @staticmethod
+    def of(v):
+        return U128(v & ((1 << 64) - 1), v >> 64)
+
+
+class ShapeParams(C.Structure):
+    _fields_ = [
+        ("n_crt", C.c_uint32), ("crt_primes", C.POINTER(C.c_uint32)),
+        ("n_wheel", C.c_uint32), ("wheel_primes", C.POINTER(C.c_uint32)),
+        ("target", C.c_int32), ("depth", C.c_int32),
+        ("n_filter", C.c_uint32), ("filter_primes", C.POINTER(C.c_uint32)),
+        ("n_stages", C.c_uint32), ("stage_end", C.POINTER(C.c_uint32)),
+        ("pos0_max_q", C.c_uint32), ("flags", C.c_uint32),
+    ]
+
+
+class ShapeInfo(C.Structure):
+    _fields_ = [
+        ("M", C.c_uint64), ("WP", C.c_uint64), ("W", C.c_uint64),
+        ("n_bases", C.c_uint32), ("n_filter", C.c_uint32), ("n_classes", C.c_uint64),
+        ("n_dense", C.c_uint32), ("n_tierb", C.c_uint32), ("n_deep", C.c_uint32),
+        ("rows_per_group", C.c_uint32), ("unit_j", C.c_uint32),
+    ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("n_dense", C.c_uint32), ("n_tierb", C.c_uint32), ("rows_per_group", C.c_uint32),
+        ("unit_j", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
+class FilterStats(C.Structure):
+    _fields_ = [
+        ("candidates", C.c_uint64), ("alive_dense", C.c_uint64), ("alive_tierb", C.c_uint64),
+        ("survivors", C.c_uint64), ("dense_end", C.c_uint32), ("tierb_end", C.c_uint32),
+        ("ms_sieve", C.c_float), ("ms_deep", C.c_float), ("ms_prove", C.c_float),
+    ]
+
+    def as_dict(self):
+        return {f: (float(getattr(self, f)) if t is C.c_float else int(getattr(self, f)))
+                for f, t in self._fields_}
+
+
+class ProveParams(C.Structure):
+    _fields_ = [("bits", C.c_int32), ("test_kind", C.c_int32), ("n_bases", C.c_uint32),
+                ("bases", C.c_uint32 * 16)]
+
+
+class KernelTimes(C.Structure):
+    _fields_ = [("n_sieve", C.c_uint32), ("n_deep", C.c_uint32), ("n_prove", C.c_uint32),
+                ("ms_sieve", C.c_double), ("ms_deep", C.c_double), ("ms_prove", C.c_double)]
+
+    def as_dict(self):
+        return {f: (float(getattr(self, f)) if t is C.c_double else int(getattr(self, f)))
+                for f, t in self._fields_}
+
+
+class Job(C.Structure):
+    _fields_ = [("a", C.c_uint64), ("u0", U128), ("count", C.c_uint64)]
+
+
+SURVIVOR_DTYPE = np.dtype([("base_idx", "<u4"), ("k_offset", "<u4"), ("tile_idx", "<u4"), ("_pad", "<u4")])
+JOB_SURVIVOR_DTYPE = np.dtype([("job_idx", "<u4"), ("u_off", "<u4")])
+CHAIN_RESULT_DTYPE = np.dtype([("survivor_idx", "<u4"), ("chain_len", "<u4"), ("back_steps", "<u4"),
+                               ("flags", "<u4"), ("root_lo", "<u8"), ("root_hi", "<u8")])
+JOB_DTYPE = np.dtype([("a", "<u8"), ("u0_lo", "<u8"), ("u0_hi", "<u8"), ("count", "<u8")])
+
+_lib = None
+
+
+class CcsError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libccs.so; raise when it is missing (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CcsError(f"{LIB_PATH} is not built: run cunningham-chain-search_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.ccs_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.ccs_ctx_destroy.argtypes = [vp]
+    L.ccs_ctx_destroy.restype = None
+    L.ccs_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.ccs_last_error.restype = C.c_char_p
+    L.ccs_dmalloc.argtypes = [vp, C.POINTER(vp), C.c_uint64]
+    L.ccs_dfree.argtypes = [vp, vp]
+    L.ccs_memcpy_h2d.argtypes = [vp, vp, vp, C.c_uint64]
+    L.ccs_memcpy_d2h.argtypes = [vp, vp, vp, C.c_uint64]
+    L.ccs_memset_d.argtypes = [vp, vp, C.c_int, C.c_uint64]
+    L.ccs_synchronize.argtypes = [vp]
+    L.ccs_shape_build.argtypes = [vp, C.POINTER(ShapeParams), C.POINTER(Tuning), C.POINTER(vp)]
+    L.ccs_shape_free.argtypes = [vp]
+    L.ccs_shape_free.restype = None
+    L.ccs_shape_query.argtypes = [vp, C.POINTER(ShapeInfo)]
+    L.ccs_shape_bases.argtypes = [vp, vp, C.c_uint32]
+    L.ccs_shape_classes.argtypes = [vp, vp, vp, vp, vp, C.c_uint64]
+    L.ccs_shape_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.ccs_shape_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.ccs_count_candidates.argtypes = [vp, U128, C.c_uint32]
+    L.ccs_count_candidates.restype = C.c_uint64
+    L.ccs_filter_launch.argtypes = [vp, vp, U128, C.c_uint32, vp, C.c_uint32, vp, vp]
+    L.ccs_filter.argtypes = [vp, vp, U128, C.c_uint32, vp, C.c_uint32, C.POINTER(C.c_uint32),
+                             C.POINTER(FilterStats)]
+    L.ccs_last_stats.argtypes = [vp, C.POINTER(FilterStats)]
+    L.ccs_prove_launch.argtypes = [vp, vp, U128, vp, vp, C.c_uint32, C.POINTER(ProveParams), vp, vp]
+    L.ccs_prove.argtypes = [vp, vp, U128, vp, C.c_uint32, C.POINTER(ProveParams), vp]
+    L.ccs_format_results.argtypes = [vp, C.c_uint32, C.c_int32, C.c_int32, C.c_char_p, C.c_uint64,
+                                     C.POINTER(C.c_uint64)]
+    L.ccs_batch_launch.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp, vp]
+    L.ccs_batch.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_uint32, C.POINTER(C.c_uint32),
+                            C.POINTER(FilterStats)]
+    L.ccs_prove_jobs.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_uint32, C.POINTER(ProveParams), vp]
+    L.ccs_profile_start.argtypes = [vp]
+    L.ccs_profile_stop.argtypes = [vp, C.POINTER(KernelTimes)]
+    _lib = L
+    return L
+
+
+def check(rc, what=""):
+    if rc < 0:
+        msg = load().ccs_last_error().decode(errors="replace")
+        raise CcsError(f"{what} failed ({rc}): {msg}")
+    return rc

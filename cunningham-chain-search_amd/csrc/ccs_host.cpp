@@ -1,0 +1,1075 @@
+// Host runtime of the MI355X Cunningham-chain pipeline: contexts, shape/table building,
+// kernel launches and the C ABI declared in include/ccs.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ccs.h"
+#include "ccs_internal.hpp"
+
+namespace ccs {
+hipError_t upload_prove_constants(const PrimeMod* small18, const PrimeMod* trial62);
+}
+
+using namespace ccs;
+typedef unsigned __int128 u128;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(CCS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+extern "C" const char* ccs_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+
+struct ccs_ctx {
+    int device = 0;
+    int n_cu = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    // scratch
+    MidRec* mid = nullptr;
+    u32 mid_cap = 0;
+    u32* counters = nullptr;            // [0] mid_count [1] out_count [2..3] unit counter (u64)
+    unsigned long long* stats = nullptr;   // [0] dense alive [1] stage-B tested
+    void* out = nullptr;                // host-form output staging
+    u64 out_bytes = 0;
+    void* res = nullptr;                // prove result staging
+    u64 res_bytes = 0;
+    JobRec* jobs = nullptr;
+    u64* unit_prefix = nullptr;
+    u32 jobs_cap = 0;
+    ccs_filter_stats last = {};
+    bool timed = false;
+    // kernel-time collection (ccs_profile_start/stop): event pairs per launch
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, size_t>> marks;   // (kind, index of the begin event)
+};
+
+// begin/end events of one kernel launch while profiling (kind: 0 sieve, 1 deep, 2 prove)
+static hipEvent_t prof_event(ccs_ctx* c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+static void prof_begin(ccs_ctx* c, int kind, hipStream_t st) {
+    if (!c->profiling) return;
+    size_t idx = c->ev_used;
+    hipEvent_t a = prof_event(c);
+    if (!a) return;
+    (void)hipEventRecord(a, st);
+    c->marks.push_back({kind, idx});
+}
+
+static void prof_end(ccs_ctx* c, hipStream_t st) {
+    if (!c->profiling || c->marks.empty()) return;
+    hipEvent_t b = prof_event(c);
+    if (b) (void)hipEventRecord(b, st);
+}
+
+static int ctx_check(ccs_ctx* c) {
+    if (!c) return fail(CCS_ERR_INVALID, "null context");
+    HIPCHK(hipSetDevice(c->device));
+    return CCS_OK;
+}
+
+static u64 mod_pow(u64 b, u64 e, u64 m) {
+    u128 r = 1 % m, x = b % m;
+    while (e) {
+        if (e & 1) r = r * x % m;
+        x = x * x % m;
+        e >>= 1;
+    }
+    return (u64)r;
+}
+
+static PrimeMod make_primemod(u32 q, u64 W) {
+    PrimeMod pm;
+    pm.q = q;
+    pm.winv = W ? (u32)mod_pow(W % q, q - 2, q) : 0;
+    pm.c32 = (u32)(((u128)1 << 32) % q);
+    pm.c64 = (u32)(((u128)1 << 64) % q);
+    pm.c96 = (u32)(((u128)1 << 96) % q);
+    pm.pad = 0;
+    pm.rcp = 1.0 / (double)q;
+    return pm;
+}
+
+extern "C" int ccs_device_count(int* out) {
+    if (!out) return fail(CCS_ERR_INVALID, "null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *out = n;
+    return CCS_OK;
+}
+
+extern "C" int ccs_ctx_create(int device, ccs_ctx** out) {
+    if (!out) return fail(CCS_ERR_INVALID, "null out");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(CCS_ERR_HIP, "no HIP device");
+    if (device < 0 || device >= n) return fail(CCS_ERR_INVALID, "bad device index");
+    ccs_ctx* c = new ccs_ctx();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(CCS_ERR_HIP, hipGetErrorString(e));
+    }
+    c->n_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(CCS_ERR_HIP, "stream create");
+    }
+    for (auto& ev : c->ev) (void)hipEventCreate(&ev);
+    if (hipMalloc((void**)&c->counters, 64) != hipSuccess ||
+        hipMalloc((void**)&c->stats, 64) != hipSuccess) {
+        ccs_ctx_destroy(c);
+        return fail(CCS_ERR_NOMEM, "counters");
+    }
+    // prove constants: small primes 2..61, trial primes 67..409
+    static const u32 small[18] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61};
+    PrimeMod ps[18], pt[62];
+    for (int i = 0; i < 18; ++i) ps[i] = make_primemod(small[i], 0);
+    int k = 0;
+    for (u32 q = 67; k < 62; ++q) {
+        bool pr = true;
+        for (u32 d = 2; d * d <= q; ++d)
+            if (q % d == 0) { pr = false; break; }
+        if (pr) pt[k++] = make_primemod(q, 0);
+    }
+    if (upload_prove_constants(ps, pt) != hipSuccess) {
+        ccs_ctx_destroy(c);
+        return fail(CCS_ERR_HIP, "prove constants");
+    }
+    *out = c;
+    return CCS_OK;
+}
+
+extern "C" void ccs_ctx_destroy(ccs_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->mid);
+    (void)hipFree(c->counters);
+    (void)hipFree(c->stats);
+    (void)hipFree(c->out);
+    (void)hipFree(c->res);
+    (void)hipFree(c->jobs);
+    (void)hipFree(c->unit_prefix);
+    for (auto& ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : c->ev_pool) (void)hipEventDestroy(ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int ccs_dmalloc(ccs_ctx* c, void** p, uint64_t bytes) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!p) return fail(CCS_ERR_INVALID, "null");
+    HIPCHK(hipMalloc(p, bytes ? bytes : 1));
+    return CCS_OK;
+}
+extern "C" int ccs_dfree(ccs_ctx* c, void* p) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    HIPCHK(hipFree(p));
+    return CCS_OK;
+}
+extern "C" int ccs_memcpy_h2d(ccs_ctx* c, void* d, const void* h, uint64_t bytes) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CCS_OK;
+}
+extern "C" int ccs_memcpy_d2h(ccs_ctx* c, void* h, const void* d, uint64_t bytes) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CCS_OK;
+}
+extern "C" int ccs_memset_d(ccs_ctx* c, void* d, int v, uint64_t bytes) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(d, v, bytes, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CCS_OK;
+}
+extern "C" int ccs_synchronize(ccs_ctx* c) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipDeviceSynchronize());
+    return CCS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// shape
+// ---------------------------------------------------------------------------
+
+struct ccs_shape {
+    int device = 0;
+    std::vector<u32> crt, wheel, fp, stage_end;
+    int target = 0, depth = 0;
+    u32 pos0_max_q = 97;
+    u32 flags = 0;
+    u64 M = 1, WP = 1, W = 1;
+    std::vector<u64> bases;
+    std::vector<ClassRec> classes;
+    u64 per_rho[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<std::vector<u32>> forb;      // n-space forbidden residues per filter prime
+    ccs_tuning tune = {};
+    // device tables
+    ClassRec* d_cls = nullptr;
+    u64* d_bases = nullptr;
+    u32* d_lds_image = nullptr;
+    DensePrime* d_dense = nullptr;
+    TierBPrime* d_tierb = nullptr;
+    PrimeMod* d_pmod = nullptr;
+    DeepPrime* d_deep = nullptr;
+    u32* d_hash = nullptr;
+    u32 lds_words = 0, nd = 0, nb = 0, nc = 0;
+    double est_survival_b = 1.0;             // expected fraction alive after stage B
+};
+
+static bool is_prime_u32(u32 q) {
+    if (q < 2) return false;
+    for (u32 d = 2; (u64)d * d <= q; ++d)
+        if (q % d == 0) return false;
+    return true;
+}
+
+// F_q: residues (2^-i - 1) mod q for i in [lo, min(hi, q-1)) (v13.cu:771-787)
+static std::vector<u32> forbidden(u32 q, int lo, int hi) {
+    std::vector<u32> out;
+    if (hi > (int)q - 1) hi = (int)q - 1;
+    for (int i = lo; i < hi; ++i) {
+        u32 inv2i = (u32)mod_pow(mod_pow(2, i, q), q - 2, q);
+        u32 r = (inv2i + q - 1) % q;
+        if (std::find(out.begin(), out.end(), r) == out.end()) out.push_back(r);
+    }
+    return out;
+}
+
+static void free_device_tables(ccs_shape* s) {
+    (void)hipFree(s->d_cls);
+    (void)hipFree(s->d_bases);
+    (void)hipFree(s->d_lds_image);
+    (void)hipFree(s->d_dense);
+    (void)hipFree(s->d_tierb);
+    (void)hipFree(s->d_pmod);
+    (void)hipFree(s->d_deep);
+    (void)hipFree(s->d_hash);
+    s->d_cls = nullptr;
+    s->d_bases = nullptr;
+    s->d_lds_image = nullptr;
+    s->d_dense = nullptr;
+    s->d_tierb = nullptr;
+    s->d_pmod = nullptr;
+    s->d_deep = nullptr;
+    s->d_hash = nullptr;
+}
+
+static constexpr u32 kMaxLdsWords = (64u * 1024u) / 4u - 4u * kQueueCap;
+
+static u32 dense_ext_words(u32 q, u32 rg) { return ((q - 1) >> 5) + 64u * rg + 1u; }
+
+// default split: dense primes below 128 (at most 64), stage B up to 256 primes below 2^12
+static void default_tuning(const ccs_shape* s, ccs_tuning* t) {
+    u32 nd = 0;
+    while (nd < s->fp.size() && nd < (u32)kMaxDense && s->fp[nd] < 128) ++nd;
+    u32 nb = 0;
+    while (nd + nb < s->fp.size() && nb < 256 && s->fp[nd + nb] < 4096) ++nb;
+    t->n_dense = nd;
+    t->n_tierb = nb;
+    t->rows_per_group = 2;
+    t->unit_j = 65536;
+    t->blocks_per_cu = 4;
+}
+
+static int validate_tuning(const ccs_shape* s, ccs_tuning* t) {
+    if (t->rows_per_group != 1 && t->rows_per_group != 2 && t->rows_per_group != 4)
+        return fail(CCS_ERR_INVALID, "rows_per_group must be 1, 2 or 4");
+    if (t->n_dense > (u32)kMaxDense) return fail(CCS_ERR_INVALID, "n_dense > 64");
+    if (t->n_tierb > 256) return fail(CCS_ERR_INVALID, "n_tierb > 256");
+    if (t->n_dense + t->n_tierb > s->fp.size())
+        return fail(CCS_ERR_INVALID, "n_dense + n_tierb exceeds the filter primes");
+    if (t->unit_j == 0 || t->unit_j % kRowBits || t->unit_j > (1u << 20))
+        return fail(CCS_ERR_INVALID, "unit_j must be a multiple of 2048, <= 2^20");
+    for (u32 i = 0; i < t->n_dense + t->n_tierb; ++i)
+        if (s->fp[i] >= 4096) return fail(CCS_ERR_INVALID, "dense/stage-B primes must be < 4096");
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 16) return fail(CCS_ERR_INVALID, "blocks_per_cu");
+    u32 words = 0;
+    for (u32 i = 0; i < t->n_dense; ++i) words += dense_ext_words(s->fp[i], t->rows_per_group);
+    for (u32 i = 0; i < t->n_tierb; ++i) words += (s->fp[t->n_dense + i] + 31) / 32;
+    if (words > kMaxLdsWords) return fail(CCS_ERR_INVALID, "dense + stage-B tables exceed LDS");
+    return CCS_OK;
+}
+
+// Build the device tables for the current tuning.
+static int upload_tables(ccs_shape* s) {
+    const ccs_tuning& t = s->tune;
+    const u32 nd = t.n_dense, nb = t.n_tierb, nfp = (u32)s->fp.size(), nc = nfp - nd - nb;
+    const u32 rg = t.rows_per_group;
+    std::vector<u32> image;
+    std::vector<DensePrime> dense(nd ? nd : 1);
+    std::vector<TierBPrime> tierb(nb ? nb : 1);
+    std::vector<PrimeMod> pmod(nd + nb ? nd + nb : 1);
+    // u-space kill sets: u + (a * W^-1) = f * W^-1 (mod q)  for f in F_q
+    auto uset = [&](u32 i) {
+        const u32 q = s->fp[i];
+        const u64 winv = mod_pow(s->W % q, q - 2, q);
+        std::vector<u8> kill(q, 0);
+        for (u32 f : s->forb[i]) kill[(u64)f * winv % q] = 1;
+        return kill;
+    };
+    for (u32 i = 0; i < nd; ++i) {
+        const u32 q = s->fp[i];
+        std::vector<u8> kill = uset(i);
+        const u32 words = dense_ext_words(q, rg);
+        DensePrime d;
+        d.q = q;
+        d.lds_off = (u32)image.size();
+        d.step = (u32)(((u64)kRowBits * rg) % q);
+        d.pad = 0;
+        dense[i] = d;
+        for (u32 w = 0; w < words; ++w) {
+            u32 v = 0;
+            for (u32 b = 0; b < 32; ++b)
+                if (kill[((u64)w * 32 + b) % q]) v |= 1u << b;
+            image.push_back(v);
+        }
+        pmod[i] = make_primemod(q, s->W);
+    }
+    for (u32 i = 0; i < nb; ++i) {
+        const u32 q = s->fp[nd + i];
+        std::vector<u8> kill = uset(nd + i);
+        TierBPrime b;
+        b.q = q;
+        b.lds_off = (u32)image.size();
+        b.rcp = 1.0f / (float)q;
+        b.pad = 0;
+        tierb[i] = b;
+        for (u32 w = 0; w < (q + 31) / 32; ++w) {
+            u32 v = 0;
+            for (u32 bb = 0; bb < 32; ++bb) {
+                u32 x = w * 32 + bb;
+                if (x < q && kill[x]) v |= 1u << bb;
+            }
+            image.push_back(v);
+        }
+        pmod[nd + i] = make_primemod(q, s->W);
+    }
+    std::vector<DeepPrime> deep(nc ? nc : 1);
+    std::vector<u32> hash((size_t)(nc ? nc : 1) * kMaxHashSlots, 0xffffffffu);
+    for (u32 i = 0; i < nc; ++i) {
+        const u32 idx = nd + nb + i;
+        DeepPrime d;
+        d.pm = make_primemod(s->fp[idx], s->W);
+        d.hash_off = i * kMaxHashSlots;
+        d.pad[0] = d.pad[1] = d.pad[2] = 0;
+        deep[i] = d;
+        for (u32 f : s->forb[idx]) {
+            u32 slot = f & (kMaxHashSlots - 1);
+            while (hash[(size_t)i * kMaxHashSlots + slot] != 0xffffffffu) slot = (slot + 1) & (kMaxHashSlots - 1);
+            hash[(size_t)i * kMaxHashSlots + slot] = f;
+        }
+    }
+    if (image.empty()) image.push_back(0);
+    // expected survival after the GPU's stage B (sizes the stage-B survivor buffer)
+    double surv = 1.0;
+    for (u32 i = 0; i < nd + nb; ++i) surv *= 1.0 - (double)s->forb[i].size() / s->fp[i];
+    s->est_survival_b = surv;
+
+    free_device_tables(s);
+    HIPCHK(hipSetDevice(s->device));
+    auto up = [&](void** dst, const void* src, size_t bytes) -> hipError_t {
+        hipError_t e = hipMalloc(dst, bytes ? bytes : 16);
+        if (e != hipSuccess) return e;
+        if (bytes) e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+        return e;
+    };
+    HIPCHK(up((void**)&s->d_lds_image, image.data(), image.size() * 4));
+    HIPCHK(up((void**)&s->d_dense, dense.data(), dense.size() * sizeof(DensePrime)));
+    HIPCHK(up((void**)&s->d_tierb, tierb.data(), tierb.size() * sizeof(TierBPrime)));
+    HIPCHK(up((void**)&s->d_pmod, pmod.data(), pmod.size() * sizeof(PrimeMod)));
+    HIPCHK(up((void**)&s->d_deep, deep.data(), deep.size() * sizeof(DeepPrime)));
+    HIPCHK(up((void**)&s->d_hash, hash.data(), hash.size() * 4));
+    HIPCHK(up((void**)&s->d_bases, s->bases.data(), s->bases.size() * 8));
+    HIPCHK(up((void**)&s->d_cls, s->classes.data(), s->classes.size() * sizeof(ClassRec)));
+    s->lds_words = (u32)image.size();
+    s->nd = nd;
+    s->nb = nb;
+    s->nc = nc;
+    return CCS_OK;
+}
+
+extern "C" int ccs_shape_build(ccs_ctx* c, const ccs_shape_params* p, const ccs_tuning* tuning,
+                               ccs_shape** out) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!p || !out) return fail(CCS_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (p->depth < 1 || p->depth > 32) return fail(CCS_ERR_INVALID, "depth must be in 1..32");
+    if (p->target < 1 || p->target > 64) return fail(CCS_ERR_INVALID, "target must be in 1..64");
+    if (p->n_crt > 16 || p->n_wheel > 8) return fail(CCS_ERR_INVALID, "too many CRT/wheel primes");
+    ccs_shape* s = new ccs_shape();
+    s->device = c->device;
+    s->target = p->target;
+    s->depth = p->depth;
+    s->pos0_max_q = p->pos0_max_q;
+    s->flags = p->flags;
+    u32 last = 3;
+    for (u32 i = 0; i < p->n_crt; ++i) {
+        u32 q = p->crt_primes[i];
+        if (!is_prime_u32(q) || q <= last) { delete s; return fail(CCS_ERR_INVALID, "CRT primes must be ascending primes > 3"); }
+        last = q;
+        s->crt.push_back(q);
+        s->M *= q;
+    }
+    for (u32 i = 0; i < p->n_wheel; ++i) {
+        u32 q = p->wheel_primes[i];
+        if (!is_prime_u32(q) || q <= last) { delete s; return fail(CCS_ERR_INVALID, "wheel primes must be ascending primes above the CRT primes"); }
+        last = q;
+        s->wheel.push_back(q);
+        s->WP *= q;
+    }
+    if ((u128)6 * s->WP * s->M >= ((u128)1 << 63)) { delete s; return fail(CCS_ERR_INVALID, "6*WP*M must be < 2^63"); }
+    s->W = 6 * s->WP * s->M;
+    for (u32 i = 0; i < p->n_filter; ++i) {
+        u32 q = p->filter_primes[i];
+        if (!is_prime_u32(q) || q <= last || q >= (1u << 18)) {
+            delete s;
+            return fail(CCS_ERR_INVALID, "filter primes must be ascending primes above the wheel primes and < 2^18");
+        }
+        last = q;
+        s->fp.push_back(q);
+        s->forb.push_back(forbidden(q, q <= p->pos0_max_q ? 0 : 1, p->depth));
+    }
+    for (u32 i = 0; i < p->n_stages && p->stage_end; ++i) s->stage_end.push_back(p->stage_end[i]);
+
+    // CRT bases mod M admissible at target (v13.cu:907-935)
+    std::vector<std::vector<u8>> cforb;
+    for (u32 q : s->crt) {
+        std::vector<u8> f(q, 0);
+        for (u32 r : forbidden(q, 0, s->target)) f[r] = 1;
+        cforb.push_back(f);
+    }
+    for (u64 r = 0; r < s->M; ++r) {
+        bool ok = true;
+        for (size_t i = 0; i < s->crt.size() && ok; ++i) ok = !cforb[i][r % s->crt[i]];
+        if (ok) s->bases.push_back(r);
+    }
+    if (!(p->flags & CCS_SHAPE_NO_CLASSES)) {
+        // wheel offsets per base (v13.cu:977-1008) and the mod-6 tile residue (v13.cu:963-975)
+        std::vector<std::vector<u8>> wforb;
+        for (u32 q : s->wheel) {
+            std::vector<u8> f(q, 0);
+            for (u32 r : forbidden(q, 0, s->target)) f[r] = 1;
+            wforb.push_back(f);
+        }
+        const u64 WPM = s->WP * s->M;
+        for (size_t b = 0; b < s->bases.size(); ++b) {
+            const u64 base = s->bases[b];
+            std::vector<u32> bq(s->wheel.size()), mq(s->wheel.size());
+            for (size_t i = 0; i < s->wheel.size(); ++i) {
+                bq[i] = (u32)(base % s->wheel[i]);
+                mq[i] = (u32)(s->M % s->wheel[i]);
+            }
+            for (u64 k = 0; k < s->WP; ++k) {
+                bool ok = true;
+                for (size_t i = 0; i < s->wheel.size() && ok; ++i) {
+                    const u32 q = s->wheel[i];
+                    ok = !wforb[i][(bq[i] + (k % q) * mq[i]) % q];
+                }
+                if (!ok) continue;
+                u32 rho = 6;
+                for (u32 t = 0; t < 6; ++t)
+                    if ((t * (WPM % 6) + (k % 6) * (s->M % 6) + base % 6) % 6 == 5) { rho = t; break; }
+                if (rho == 6) continue;   // cannot happen: WP*M is a unit mod 6
+                ClassRec cr;
+                cr.a = rho * WPM + k * s->M + base;
+                cr.base_idx = (u32)b;
+                cr.k = (u32)k;
+                cr.rho = rho;
+                cr.pad = 0;
+                s->classes.push_back(cr);
+                s->per_rho[rho]++;
+            }
+        }
+    }
+    ccs_tuning t;
+    if (tuning) t = *tuning;
+    else default_tuning(s, &t);
+    rc = validate_tuning(s, &t);
+    if (rc) { delete s; return rc; }
+    s->tune = t;
+    rc = upload_tables(s);
+    if (rc) { free_device_tables(s); delete s; return rc; }
+    *out = s;
+    return CCS_OK;
+}
+
+extern "C" void ccs_shape_free(ccs_shape* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    free_device_tables(s);
+    delete s;
+}
+
+extern "C" int ccs_shape_set_tuning(ccs_shape* s, const ccs_tuning* t) {
+    if (!s || !t) return fail(CCS_ERR_INVALID, "null");
+    ccs_tuning tt = *t;
+    int rc = validate_tuning(s, &tt);
+    if (rc) return rc;
+    s->tune = tt;
+    return upload_tables(s);
+}
+
+extern "C" int ccs_shape_get_tuning(const ccs_shape* s, ccs_tuning* out) {
+    if (!s || !out) return fail(CCS_ERR_INVALID, "null");
+    *out = s->tune;
+    return CCS_OK;
+}
+
+extern "C" int ccs_shape_query(const ccs_shape* s, ccs_shape_info* o) {
+    if (!s || !o) return fail(CCS_ERR_INVALID, "null");
+    o->M = s->M;
+    o->WP = s->WP;
+    o->W = s->W;
+    o->n_bases = (u32)s->bases.size();
+    o->n_filter = (u32)s->fp.size();
+    o->n_classes = s->classes.size();
+    o->n_dense = s->nd;
+    o->n_tierb = s->nb;
+    o->n_deep = s->nc;
+    o->rows_per_group = s->tune.rows_per_group;
+    o->unit_j = s->tune.unit_j;
+    return CCS_OK;
+}
+
+extern "C" int ccs_shape_bases(const ccs_shape* s, uint64_t* out, uint32_t cap) {
+    if (!s || !out) return fail(CCS_ERR_INVALID, "null");
+    u32 n = std::min<u32>(cap, (u32)s->bases.size());
+    memcpy(out, s->bases.data(), (size_t)n * 8);
+    return n < s->bases.size() ? CCS_OVERFLOW : CCS_OK;
+}
+
+extern "C" int ccs_shape_classes(const ccs_shape* s, uint64_t* a, uint32_t* b, uint32_t* k, uint32_t* rho,
+                                 uint64_t cap) {
+    if (!s) return fail(CCS_ERR_INVALID, "null");
+    u64 n = std::min<u64>(cap, s->classes.size());
+    for (u64 i = 0; i < n; ++i) {
+        if (a) a[i] = s->classes[i].a;
+        if (b) b[i] = s->classes[i].base_idx;
+        if (k) k[i] = s->classes[i].k;
+        if (rho) rho[i] = s->classes[i].rho;
+    }
+    return n < s->classes.size() ? CCS_OVERFLOW : CCS_OK;
+}
+
+static u128 mk128(ccs_u128 v) { return ((u128)v.hi << 64) | v.lo; }
+
+// candidates of a tile range: the v33 counting convention (cc_gmp_v33_03.c:1900)
+extern "C" uint64_t ccs_count_candidates(const ccs_shape* s, ccs_u128 ts, uint32_t tile_count) {
+    if (!s) return 0;
+    const u128 T0 = mk128(ts);
+    const u32 r0 = (u32)(T0 % 6);
+    u64 total = 0;
+    for (u32 m = 0; m < 6; ++m) {
+        const u64 first = (m + 6 - r0) % 6;
+        const u64 cnt = tile_count > first ? ((u64)tile_count - first + 5) / 6 : 0;
+        total += cnt * s->per_rho[m];
+    }
+    return total;
+}
+
+// ---------------------------------------------------------------------------
+// launches
+// ---------------------------------------------------------------------------
+
+static int ensure_mid(ccs_ctx* c, u64 want) {
+    if (want < (1u << 16)) want = 1u << 16;
+    if (want > (1u << 28)) want = 1u << 28;
+    if (c->mid_cap >= want) return CCS_OK;
+    (void)hipFree(c->mid);
+    c->mid = nullptr;
+    c->mid_cap = 0;
+    HIPCHK(hipMalloc((void**)&c->mid, want * sizeof(MidRec)));
+    c->mid_cap = (u32)want;
+    return CCS_OK;
+}
+
+static void fill_sieve_common(const ccs_shape* s, ccs_ctx* c, SieveArgs& a) {
+    memset(&a, 0, sizeof(a));
+    a.W = s->W;
+    a.lds_image = s->d_lds_image;
+    a.lds_words = s->lds_words;
+    a.nd = s->nd;
+    a.nb = s->nb;
+    a.unit_j = s->tune.unit_j;
+    a.dense = s->d_dense;
+    a.tierb = s->d_tierb;
+    a.pmod = s->d_pmod;
+    a.unit_counter = (u64*)(c->counters + 2);
+    a.mid = c->mid;
+    a.mid_cap = c->mid_cap;
+    a.mid_count = c->counters + 0;
+    a.stats = c->stats;
+}
+
+static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, SieveArgs& sa, DeepArgs& da, u64 n_units,
+                              u32* d_count, hipStream_t st) {
+    // zero: mid_count, unit counter, stats, caller's count (fill kernels, graph-safe)
+    if (launch_zero(c->counters, 16, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
+    if (launch_zero((u32*)c->stats, 16, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
+    if (launch_zero(d_count, 1, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
+    sa.n_units = n_units;
+    const int nbv = s->nb ? (int)((s->nb + 63) / 64) : 1;
+    const int block = 256;
+    const size_t lds = ((size_t)s->lds_words + (size_t)(block / 64) * kQueueCap) * 4;
+    u64 want_grid = (u64)c->n_cu * s->tune.blocks_per_cu;
+    const u64 waves_needed = (n_units + 3) / 4;
+    if (want_grid > waves_needed) want_grid = waves_needed ? waves_needed : 1;
+    if (c->timed) HIPCHK(hipEventRecord(c->ev[0], st));
+    prof_begin(c, 0, st);
+    hipError_t e = launch_sieve(sa, (int)s->tune.rows_per_group, nbv, (int)want_grid, block, lds, st);
+    if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("sieve launch: ") + hipGetErrorString(e));
+    prof_end(c, st);
+    if (c->timed) HIPCHK(hipEventRecord(c->ev[1], st));
+    da.mid = c->mid;
+    da.mid_count = c->counters + 0;
+    da.mid_cap = c->mid_cap;
+    da.out_count = d_count;
+    prof_begin(c, 1, st);
+    e = launch_deep(da, c->mid_cap, st);
+    if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("deep launch: ") + hipGetErrorString(e));
+    prof_end(c, st);
+    if (c->timed) HIPCHK(hipEventRecord(c->ev[2], st));
+    return CCS_OK;
+}
+
+static int filter_launch_impl(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint32_t tile_count,
+                              ccs_survivor* d_out, uint32_t cap, uint32_t* d_count, hipStream_t st) {
+    if (!s || !d_out || !d_count) return fail(CCS_ERR_INVALID, "null argument");
+    if (s->classes.empty()) return fail(CCS_ERR_INVALID, "shape built without classes");
+    const u128 T0 = mk128(ts);
+    const u128 U0 = T0 / 6;
+    const u32 off = (u32)(T0 - U0 * 6);
+    const u64 jmax = ((u64)tile_count + off + 5) / 6;
+    const u64 cpc = (jmax + s->tune.unit_j - 1) / s->tune.unit_j;
+    const u64 n_units = cpc * s->classes.size();
+    const u64 cand = ccs_count_candidates(s, ts, tile_count);
+    int rc = ensure_mid(c, (u64)(cand * s->est_survival_b * 2.0) + (1u << 20));
+    if (rc) return rc;
+    SieveArgs sa;
+    fill_sieve_common(s, c, sa);
+    sa.mode = 0;
+    sa.plain.cls = s->d_cls;
+    sa.plain.nclasses = s->classes.size();
+    sa.plain.U0_lo = (u64)U0;
+    sa.plain.U0_hi = (u64)(U0 >> 64);
+    sa.plain.off = off;
+    sa.plain.tile_count = tile_count;
+    sa.plain.chunks_per_class = (u32)cpc;
+    DeepArgs da;
+    memset(&da, 0, sizeof(da));
+    da.W = s->W;
+    da.deep = s->d_deep;
+    da.hash = s->d_hash;
+    da.nc = s->nc;
+    da.mode = 0;
+    da.plain = sa.plain;
+    da.out = d_out;
+    da.out_cap = cap;
+    c->last = ccs_filter_stats();
+    c->last.candidates = cand;
+    c->last.dense_end = s->nd;
+    c->last.tierb_end = s->nd + s->nb;
+    return run_filter_kernels(c, s, sa, da, n_units, d_count, st);
+}
+
+extern "C" int ccs_filter_launch(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint32_t tile_count,
+                                 ccs_survivor* d_out, uint32_t cap, uint32_t* d_count, void* stream) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    c->timed = false;
+    return filter_launch_impl(c, s, ts, tile_count, d_out, cap, d_count,
+                              stream ? (hipStream_t)stream : c->stream);
+}
+
+static int ensure_buf(void** p, u64* have, u64 want) {
+    if (*have >= want) return CCS_OK;
+    (void)hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    HIPCHK(hipMalloc(p, want));
+    *have = want;
+    return CCS_OK;
+}
+
+static int collect_stats(ccs_ctx* c, u32 count) {
+    unsigned long long st[2] = {0, 0};
+    u32 midc = 0;
+    HIPCHK(hipMemcpy(st, c->stats, sizeof(st), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&midc, c->counters, 4, hipMemcpyDeviceToHost));
+    c->last.alive_dense = st[0];
+    c->last.alive_tierb = midc;
+    c->last.survivors = count;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->last.ms_sieve = ms;
+    if (hipEventElapsedTime(&ms, c->ev[1], c->ev[2]) == hipSuccess) c->last.ms_deep = ms;
+    if (midc > c->mid_cap) return fail(CCS_ERR_NOMEM, "stage-B survivor buffer overflow");
+    return CCS_OK;
+}
+
+extern "C" int ccs_filter(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint32_t tile_count,
+                          ccs_survivor* h_out, uint32_t cap, uint32_t* h_count, ccs_filter_stats* stats) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!h_count) return fail(CCS_ERR_INVALID, "null count");
+    rc = ensure_buf(&c->out, &c->out_bytes, (u64)(cap ? cap : 1) * sizeof(ccs_survivor) + 256);
+    if (rc) return rc;
+    u32* d_count = (u32*)((char*)c->out + (u64)(cap ? cap : 1) * sizeof(ccs_survivor));
+    c->timed = true;
+    rc = filter_launch_impl(c, s, ts, tile_count, (ccs_survivor*)c->out, cap, d_count, c->stream);
+    c->timed = false;
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    u32 n = 0;
+    HIPCHK(hipMemcpy(&n, d_count, 4, hipMemcpyDeviceToHost));
+    const u32 keep = std::min(n, cap);
+    if (keep && h_out) HIPCHK(hipMemcpy(h_out, c->out, (size_t)keep * sizeof(ccs_survivor), hipMemcpyDeviceToHost));
+    *h_count = n;
+    rc = collect_stats(c, n);
+    if (stats) *stats = c->last;
+    if (rc) return rc;
+    return n > cap ? CCS_OVERFLOW : CCS_OK;
+}
+
+extern "C" int ccs_last_stats(ccs_ctx* c, ccs_filter_stats* out) {
+    if (!c || !out) return fail(CCS_ERR_INVALID, "null");
+    *out = c->last;
+    return CCS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// prove
+// ---------------------------------------------------------------------------
+
+static int fill_prove(const ccs_shape* s, const ccs_prove_params* p, ProveArgs& a) {
+    memset(&a, 0, sizeof(a));
+    a.WP = s->WP;
+    a.M = s->M;
+    a.W = s->W;
+    a.bases = s->d_bases;
+    a.bits = p ? p->bits : 0;
+    a.test_kind = p ? p->test_kind : CCS_TEST_BPSW;
+    if (a.test_kind != CCS_TEST_BPSW && a.test_kind != CCS_TEST_MR) return fail(CCS_ERR_INVALID, "test_kind");
+    if (a.test_kind == CCS_TEST_MR) {
+        if (!p || p->n_bases < 1 || p->n_bases > 16) return fail(CCS_ERR_INVALID, "MR needs 1..16 bases");
+        a.n_bases = (int)p->n_bases;
+        for (u32 i = 0; i < p->n_bases; ++i) {
+            if (p->bases[i] < 2) return fail(CCS_ERR_INVALID, "MR base < 2");
+            a.bases_mr[i] = p->bases[i];
+        }
+    }
+    return CCS_OK;
+}
+
+extern "C" int ccs_prove_launch(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, const ccs_survivor* d_surv,
+                                const uint32_t* d_count, uint32_t cap, const ccs_prove_params* p,
+                                ccs_chain_result* d_out, void* stream) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!s || !d_surv || !d_count || !d_out) return fail(CCS_ERR_INVALID, "null argument");
+    ProveArgs a;
+    rc = fill_prove(s, p, a);
+    if (rc) return rc;
+    a.mode = 0;
+    a.T0_lo = ts.lo;
+    a.T0_hi = ts.hi;
+    a.surv = d_surv;
+    a.surv_count = d_count;
+    a.surv_cap = cap;
+    a.out = d_out;
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    if (c->timed) HIPCHK(hipEventRecord(c->ev[3], st));
+    prof_begin(c, 2, st);
+    hipError_t e = launch_prove(a, cap, st);
+    if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("prove launch: ") + hipGetErrorString(e));
+    prof_end(c, st);
+    if (c->timed) HIPCHK(hipEventRecord(c->ev[4], st));
+    return CCS_OK;
+}
+
+static int prove_host_common(ccs_ctx* c, ProveArgs& a, const void* h_surv, size_t rec_bytes, uint32_t n,
+                             ccs_chain_result* h_out) {
+    if (n == 0) return CCS_OK;
+    const u64 in_bytes = (u64)n * rec_bytes;
+    int rc = ensure_buf(&c->res, &c->res_bytes, in_bytes + 256 + (u64)n * sizeof(ccs_chain_result));
+    if (rc) return rc;
+    char* base = (char*)c->res;
+    u32* d_count = (u32*)(base + in_bytes);
+    ccs_chain_result* d_res = (ccs_chain_result*)(base + in_bytes + 256);
+    HIPCHK(hipMemcpyAsync(base, h_surv, in_bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_count, &n, 4, hipMemcpyHostToDevice, c->stream));
+    a.surv = base;
+    a.surv_count = d_count;
+    a.surv_cap = n;
+    a.out = d_res;
+    HIPCHK(hipEventRecord(c->ev[3], c->stream));
+    hipError_t e = launch_prove(a, n, c->stream);
+    if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("prove launch: ") + hipGetErrorString(e));
+    HIPCHK(hipEventRecord(c->ev[4], c->stream));
+    HIPCHK(hipMemcpyAsync(h_out, d_res, (size_t)n * sizeof(ccs_chain_result), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c->ev[3], c->ev[4]) == hipSuccess) c->last.ms_prove = ms;
+    return CCS_OK;
+}
+
+extern "C" int ccs_prove(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, const ccs_survivor* h_surv, uint32_t n,
+                         const ccs_prove_params* p, ccs_chain_result* h_out) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!s || (n && (!h_surv || !h_out))) return fail(CCS_ERR_INVALID, "null argument");
+    for (u32 i = 0; i < n; ++i)
+        if (h_surv[i].base_idx >= s->bases.size()) return fail(CCS_ERR_INVALID, "survivor base_idx out of range");
+    ProveArgs a;
+    rc = fill_prove(s, p, a);
+    if (rc) return rc;
+    a.mode = 0;
+    a.T0_lo = ts.lo;
+    a.T0_hi = ts.hi;
+    return prove_host_common(c, a, h_surv, sizeof(ccs_survivor), n, h_out);
+}
+
+// ---------------------------------------------------------------------------
+// result formatting (v13.cu:2016-2061)
+// ---------------------------------------------------------------------------
+
+static std::string hex128(u64 lo, u64 hi) {
+    char buf[40];
+    if (hi) snprintf(buf, sizeof(buf), "0x%llX%016llX", (unsigned long long)hi, (unsigned long long)lo);
+    else snprintf(buf, sizeof(buf), "0x%llX", (unsigned long long)lo);
+    return buf;
+}
+
+static int bitlen64x2(u64 lo, u64 hi) {
+    if (hi) return 128 - __builtin_clzll(hi);
+    return lo ? 64 - __builtin_clzll(lo) : 0;
+}
+
+extern "C" int ccs_format_results(const ccs_chain_result* res, uint32_t n, int32_t log_threshold, int32_t target,
+                                  char* buf, uint64_t buf_size, uint64_t* needed) {
+    if (n && !res) return fail(CCS_ERR_INVALID, "null");
+    std::string out;
+    for (u32 i = 0; i < n; ++i) {
+        const ccs_chain_result& r = res[i];
+        if (!(r.flags & CCS_F_PRIME)) continue;
+        const int len = (int)r.chain_len;
+        if (len < log_threshold && len < target) continue;
+        if (len >= target) out += "TARGET ";
+        if (r.flags & CCS_F_NONROOT) out += "NON-ROOT ";
+        out += "CC" + std::to_string(len) + " " + hex128(r.root_lo, r.root_hi) + " " +
+               std::to_string(bitlen64x2(r.root_lo, r.root_hi)) + "\n";
+    }
+    if (needed) *needed = out.size();
+    if (buf && buf_size) {
+        const size_t k = std::min<size_t>(out.size(), buf_size - 1);
+        memcpy(buf, out.data(), k);
+        buf[k] = 0;
+        if (k < out.size()) return CCS_OVERFLOW;
+    }
+    return CCS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched form
+// ---------------------------------------------------------------------------
+
+static int upload_jobs(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, u32 n_jobs, hipStream_t st,
+                       u64* n_units_out, u64* cand_out) {
+    if (!jobs || n_jobs == 0) return fail(CCS_ERR_INVALID, "no jobs");
+    if (c->jobs_cap < n_jobs) {
+        (void)hipFree(c->jobs);
+        (void)hipFree(c->unit_prefix);
+        c->jobs = nullptr;
+        c->unit_prefix = nullptr;
+        c->jobs_cap = 0;
+        HIPCHK(hipMalloc((void**)&c->jobs, (size_t)n_jobs * sizeof(JobRec)));
+        HIPCHK(hipMalloc((void**)&c->unit_prefix, ((size_t)n_jobs + 1) * 8));
+        c->jobs_cap = n_jobs;
+    }
+    std::vector<JobRec> jr(n_jobs);
+    std::vector<u64> pre(n_jobs + 1);
+    u64 units = 0, cand = 0;
+    for (u32 i = 0; i < n_jobs; ++i) {
+        if (jobs[i].a >= s->W) return fail(CCS_ERR_INVALID, "job residue a must be < W");
+        if (jobs[i].count > (1ull << 32)) return fail(CCS_ERR_INVALID, "job count must be <= 2^32");
+        jr[i].a = jobs[i].a;
+        jr[i].u0_lo = jobs[i].u0.lo;
+        jr[i].u0_hi = jobs[i].u0.hi;
+        jr[i].count = jobs[i].count;
+        pre[i] = units;
+        units += (jobs[i].count + s->tune.unit_j - 1) / s->tune.unit_j;
+        cand += jobs[i].count;
+    }
+    pre[n_jobs] = units;
+    HIPCHK(hipMemcpyAsync(c->jobs, jr.data(), jr.size() * sizeof(JobRec), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->unit_prefix, pre.data(), pre.size() * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *n_units_out = units;
+    *cand_out = cand;
+    return CCS_OK;
+}
+
+static int batch_launch_impl(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, uint32_t n_jobs,
+                             ccs_job_survivor* d_out, uint32_t cap, uint32_t* d_count, hipStream_t st) {
+    if (!s || !d_out || !d_count) return fail(CCS_ERR_INVALID, "null argument");
+    u64 n_units = 0, cand = 0;
+    int rc = upload_jobs(c, s, jobs, n_jobs, st, &n_units, &cand);
+    if (rc) return rc;
+    rc = ensure_mid(c, (u64)(cand * s->est_survival_b * 2.0) + (1u << 20));
+    if (rc) return rc;
+    SieveArgs sa;
+    fill_sieve_common(s, c, sa);
+    sa.mode = 1;
+    sa.jobsrc.jobs = c->jobs;
+    sa.jobsrc.unit_prefix = c->unit_prefix;
+    sa.jobsrc.n_jobs = n_jobs;
+    DeepArgs da;
+    memset(&da, 0, sizeof(da));
+    da.W = s->W;
+    da.deep = s->d_deep;
+    da.hash = s->d_hash;
+    da.nc = s->nc;
+    da.mode = 1;
+    da.jobsrc = sa.jobsrc;
+    da.out = d_out;
+    da.out_cap = cap;
+    c->last = ccs_filter_stats();
+    c->last.candidates = cand;
+    c->last.dense_end = s->nd;
+    c->last.tierb_end = s->nd + s->nb;
+    return run_filter_kernels(c, s, sa, da, n_units, d_count, st);
+}
+
+extern "C" int ccs_batch_launch(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, uint32_t n_jobs,
+                                ccs_job_survivor* d_out, uint32_t cap, uint32_t* d_count, void* stream) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    c->timed = false;
+    return batch_launch_impl(c, s, jobs, n_jobs, d_out, cap, d_count, stream ? (hipStream_t)stream : c->stream);
+}
+
+extern "C" int ccs_batch(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, uint32_t n_jobs,
+                         ccs_job_survivor* h_out, uint32_t cap, uint32_t* h_count, ccs_filter_stats* stats) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!h_count) return fail(CCS_ERR_INVALID, "null count");
+    rc = ensure_buf(&c->out, &c->out_bytes, (u64)(cap ? cap : 1) * sizeof(ccs_job_survivor) + 256);
+    if (rc) return rc;
+    u32* d_count = (u32*)((char*)c->out + (u64)(cap ? cap : 1) * sizeof(ccs_job_survivor));
+    c->timed = true;
+    rc = batch_launch_impl(c, s, jobs, n_jobs, (ccs_job_survivor*)c->out, cap, d_count, c->stream);
+    c->timed = false;
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    u32 n = 0;
+    HIPCHK(hipMemcpy(&n, d_count, 4, hipMemcpyDeviceToHost));
+    const u32 keep = std::min(n, cap);
+    if (keep && h_out) HIPCHK(hipMemcpy(h_out, c->out, (size_t)keep * sizeof(ccs_job_survivor), hipMemcpyDeviceToHost));
+    *h_count = n;
+    rc = collect_stats(c, n);
+    if (stats) *stats = c->last;
+    if (rc) return rc;
+    return n > cap ? CCS_OVERFLOW : CCS_OK;
+}
+
+extern "C" int ccs_prove_jobs(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, uint32_t n_jobs,
+                              const ccs_job_survivor* h_surv, uint32_t n, const ccs_prove_params* p,
+                              ccs_chain_result* h_out) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!s || !jobs || (n && (!h_surv || !h_out))) return fail(CCS_ERR_INVALID, "null argument");
+    for (u32 i = 0; i < n; ++i)
+        if (h_surv[i].job_idx >= n_jobs) return fail(CCS_ERR_INVALID, "survivor job_idx out of range");
+    u64 units = 0, cand = 0;
+    rc = upload_jobs(c, s, jobs, n_jobs, c->stream, &units, &cand);
+    if (rc) return rc;
+    ProveArgs a;
+    rc = fill_prove(s, p, a);
+    if (rc) return rc;
+    a.mode = 1;
+    a.jobs = c->jobs;
+    return prove_host_common(c, a, h_surv, sizeof(ccs_job_survivor), n, h_out);
+}
+
+// ---------------------------------------------------------------------------
+// kernel timing
+// ---------------------------------------------------------------------------
+
+extern "C" int ccs_profile_start(ccs_ctx* c) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->profiling = true;
+    c->ev_used = 0;
+    c->marks.clear();
+    return CCS_OK;
+}
+
+extern "C" int ccs_profile_stop(ccs_ctx* c, ccs_kernel_times* out) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!out) return fail(CCS_ERR_INVALID, "null");
+    HIPCHK(hipDeviceSynchronize());
+    memset(out, 0, sizeof(*out));
+    for (auto& m : c->marks) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev_pool[m.second], c->ev_pool[m.second + 1]));
+        if (m.first == 0) { out->n_sieve++; out->ms_sieve += ms; }
+        else if (m.first == 1) { out->n_deep++; out->ms_deep += ms; }
+        else { out->n_prove++; out->ms_prove += ms; }
+    }
+    c->profiling = false;
+    c->marks.clear();
+    c->ev_used = 0;
+    return CCS_OK;
+}

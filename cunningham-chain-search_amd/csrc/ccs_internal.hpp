@@ -1,0 +1,163 @@
+// Internal structures shared by the host runtime (ccs_host.cpp) and the gfx950
+// kernels (ccs_kernels.hip).  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ccs {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+typedef uint16_t u16;
+typedef uint8_t u8;
+
+// Candidates are organised as residue classes a (mod W) and a run of u: n = a + W*u.
+// For the reference lattice n = (tile*WP + k)*M + base (v13.cu:1479-1492) a class is
+// (base, k, rho) with tile = 6u + rho, W = 6*WP*M and a = rho*WP*M + k*M + base.
+
+constexpr int kRowBits = 2048;      // one row = 64 lanes x 32 candidates
+constexpr int kQueueCap = 128;      // per-wave stage-B candidate queue (LDS)
+constexpr int kMaxDense = 64;       // dense primes held one per lane
+constexpr int kMaxHashSlots = 64;   // forbidden-residue hash per deep prime
+
+// Per-prime constants for exact residues of u64/u128 values: y = sum limb_k*2^(32k)
+// folded with c_k = 2^(32k) mod q (< 2^52 for q < 2^18), then reduced with a double
+// reciprocal.
+struct PrimeMod {
+    u32 q;
+    u32 winv;       // W^-1 mod q
+    u32 c32, c64, c96;
+    u32 pad;
+    double rcp;     // 1.0 / q
+};
+
+// Dense stage: kill pattern over u, periodic with period q, stored as an extended bit
+// string in LDS so that a row (64 lanes x 32 bits) starting at any phase s reads the two
+// dwords (s>>5)+lane and (s>>5)+lane+1 with no wrap-around.
+struct DensePrime {
+    u32 q;
+    u32 lds_off;    // dword offset of the extended pattern in LDS
+    u32 step;       // (kRowBits * RG) mod q: phase advance per row group
+    u32 pad;
+};
+
+// Stage B: per-candidate test with a q-bit forbidden bitset in LDS.
+struct TierBPrime {
+    u32 q;
+    u32 lds_off;    // dword offset of the q-bit bitset in LDS
+    float rcp;      // 1.0f / q, for x mod q with x < 2^18
+    u32 pad;
+};
+
+// Deep stage (K1c): forbidden residues of n mod q in a 64-slot open-addressing hash,
+// key r, slot (r & 63), linear probing, empty slot = 0xFFFFFFFF.
+struct DeepPrime {
+    PrimeMod pm;
+    u32 hash_off;   // u32 offset into the hash table array
+    u32 pad[3];
+};
+
+struct ClassRec {   // plain lattice class (base_idx, k, rho) and its residue a mod W
+    u64 a;
+    u32 base_idx;
+    u32 k;
+    u32 rho;
+    u32 pad;
+};
+
+struct JobRec {     // class-form job: n = a + W*(u0 + j), j in [0, count)
+    u64 a;
+    u64 u0_lo, u0_hi;
+    u64 count;
+};
+
+struct MidRec {     // stage-B survivor: owner (class or job index) and j
+    u32 owner;
+    u32 j;
+};
+
+struct PlainSrc {
+    const ClassRec* cls;
+    u64 nclasses;
+    u64 U0_lo, U0_hi;   // floor(T0 / 6)
+    u32 off;            // T0 - 6*U0
+    u32 tile_count;
+    u32 chunks_per_class;
+    u32 pad;
+};
+
+struct JobSrc {
+    const JobRec* jobs;
+    const u64* unit_prefix;   // [n_jobs + 1] first unit index of each job
+    u32 n_jobs;
+    u32 pad;
+};
+
+struct SieveArgs {
+    u64 W;
+    // tables
+    const u32* lds_image;     // dense patterns + tier-B bitsets, copied to LDS at start
+    u32 lds_words;
+    u32 nd;                   // dense primes
+    u32 nb;                   // tier-B primes
+    u32 unit_j;               // candidates (u values) per work unit
+    const DensePrime* dense;
+    const TierBPrime* tierb;
+    const PrimeMod* pmod;     // [nd + nb] residue constants for dense then tier-B primes
+    // work
+    u64 n_units;
+    u64* unit_counter;        // work-stealing cursor (zeroed before launch)
+    int mode;                 // 0 = plain, 1 = jobs
+    PlainSrc plain;
+    JobSrc jobsrc;
+    // output
+    MidRec* mid;
+    u32 mid_cap;
+    u32* mid_count;
+    unsigned long long* stats; // [0] alive after dense stage, [1] tested in stage B (may be null)
+};
+
+struct DeepArgs {
+    u64 W;
+    const DeepPrime* deep;
+    const u32* hash;
+    u32 nc;
+    int mode;
+    PlainSrc plain;
+    JobSrc jobsrc;
+    const MidRec* mid;
+    const u32* mid_count;
+    u32 mid_cap;
+    void* out;                // plain: ccs_survivor[16 B]; jobs: ccs_job_survivor[8 B]
+    u32 out_cap;
+    u32* out_count;
+};
+
+// Prove: GMP confirmation engine semantics (v13.cu:1794-1924) per survivor.
+struct ProveArgs {
+    int mode;                  // 0 = plain survivors, 1 = job survivors
+    // plain reconstruction n = ((T0 + tile_idx)*WP + k)*M + base
+    u64 T0_lo, T0_hi;
+    u64 WP, M;
+    const u64* bases;
+    // jobs reconstruction n = a + W*(u0 + u_off)
+    u64 W;
+    const JobRec* jobs;
+    const void* surv;
+    const u32* surv_count;
+    u32 surv_cap;
+    int bits;                  // bit-size gate, 0 = off
+    int test_kind;             // 0 = Baillie-PSW (GMP equivalent), 1 = strong MR over bases
+    int n_bases;
+    u32 bases_mr[16];
+    void* out;                 // ccs_chain_result per survivor
+};
+
+// kernel launchers (ccs_kernels.hip)
+hipError_t launch_sieve(const SieveArgs& a, int rg, int nbv, int grid, int block, size_t lds_bytes,
+                        hipStream_t s);
+hipError_t launch_deep(const DeepArgs& a, u32 max_entries, hipStream_t s);
+hipError_t launch_prove(const ProveArgs& a, u32 max_entries, hipStream_t s);
+hipError_t launch_zero(u32* p, size_t n_words, hipStream_t s);
+
+}  // namespace ccs

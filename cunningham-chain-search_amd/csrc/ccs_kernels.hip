@@ -1,0 +1,809 @@
+// gfx950 kernels of the Cunningham-chain (first kind) candidate pipeline.
+//
+//   sieve_kernel  (K1)  enumerate candidates n = a + W*u in registers, dense bit-pattern
+//                        stage over the small filter primes (LDS patterns, 32 candidates per
+//                        lane per row), wave64 ballot compaction into an LDS queue, then
+//                        per-candidate stage-B tests against LDS bitsets.
+//                        Replaces cc18_filter_kernel (v13.cu:1551-1714).
+//   deep_kernel   (K1c) remaining (large) filter primes on the rare stage-B survivors and
+//                        emission in the reference Survivor format (v13.cu:207-212, :1688-1710).
+//   prove_kernel  (K2)  the GMP confirmation engine semantics of prover_worker
+//                        (v13.cu:1794-1924) with u128 Montgomery Baillie-PSW, replacing
+//                        cc18_prove_kernel (v13.cu:1462-1538).
+#include <hip/hip_runtime.h>
+#include "ccs_internal.hpp"
+#include "../../include/ccs.h"
+
+namespace ccs {
+
+// ---------------------------------------------------------------------------
+// wave64 helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ u32 lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// number of set bits of m below this lane
+__device__ __forceinline__ u32 lanes_below(u64 m) {
+    return __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+}
+
+__device__ __forceinline__ u64 ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ u32 bcast32(u32 v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ u64 bcast64(u64 v) {
+    u32 lo = __builtin_amdgcn_readfirstlane((u32)v);
+    u32 hi = __builtin_amdgcn_readfirstlane((u32)(v >> 32));
+    return ((u64)hi << 32) | lo;
+}
+
+// order this wave's LDS writes before its later LDS reads
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// exact residues
+// ---------------------------------------------------------------------------
+
+// y mod q for y < 2^52, q < 2^18: double reciprocal estimate, corrected by +-q
+__device__ __forceinline__ u32 mod52(u64 y, u32 q, double rcp) {
+    u64 qt = (u64)((double)y * rcp);
+    int64_t r = (int64_t)(y - qt * (u64)q);
+    if (r < 0) r += q;
+    if (r < 0) r += q;
+    if (r >= (int64_t)q) r -= q;
+    if (r >= (int64_t)q) r -= q;
+    return (u32)r;
+}
+
+// (hi:lo) mod q via limb folding with c_k = 2^(32k) mod q
+__device__ __forceinline__ u32 mod128(u64 lo, u64 hi, const PrimeMod& pm) {
+    u64 y = (lo & 0xffffffffull) + (lo >> 32) * (u64)pm.c32 + (hi & 0xffffffffull) * (u64)pm.c64 +
+            (hi >> 32) * (u64)pm.c96;
+    return mod52(y, pm.q, pm.rcp);
+}
+
+// phase of u_start for a prime: s = (u_start + a * W^-1) mod q, so that a candidate
+// j units after u_start is killed iff (s + j) mod q lies in the prime's pattern.
+__device__ __forceinline__ u32 phase(const PrimeMod& pm, u64 a, u64 us_lo, u64 us_hi) {
+    u32 am = mod128(a, 0, pm);
+    u32 um = mod128(us_lo, us_hi, pm);
+    u32 t = mod52((u64)am * pm.winv, pm.q, pm.rcp);
+    u32 s = um + t;
+    if (s >= pm.q) s -= pm.q;
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// work units
+// ---------------------------------------------------------------------------
+
+struct Unit {
+    u64 a;
+    u64 us_lo, us_hi;
+    u32 n;       // candidates in the unit
+    u32 owner;   // class or job index
+    u32 j0;      // j of the first candidate
+};
+
+__device__ __forceinline__ void add128(u64& lo, u64& hi, u64 v) {
+    u64 l = lo + v;
+    hi += (l < lo);
+    lo = l;
+}
+
+// job owning a unit: 64-ary search over the unit prefix array, all lanes loading
+__device__ __forceinline__ u32 find_job(const JobSrc& J, u64 unit, u32 lane) {
+    u32 lo = 0, hi = J.n_jobs;   // invariant: prefix[lo] <= unit < prefix[hi]
+    while (hi - lo > 1) {
+        u32 span = hi - lo;
+        u32 step = (span + 63) / 64;
+        u32 idx = lo + lane * step;
+        bool le = (idx < hi) && (J.unit_prefix[idx] <= unit);
+        u64 m = ballot(le);
+        u32 last = 63 - __builtin_clzll(m);     // lane 0 is always <= (prefix[lo] <= unit)
+        u32 nlo = lo + last * step;
+        u32 nhi = nlo + step < hi ? nlo + step : hi;
+        lo = nlo;
+        hi = nhi;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool resolve(const SieveArgs& A, u64 unit, u32 lane, Unit& U) {
+    if (A.mode == 0) {
+        const PlainSrc& P = A.plain;
+        u64 c = unit / P.chunks_per_class;
+        u64 ch = unit - c * P.chunks_per_class;
+        ClassRec cr = P.cls[c];
+        u64 jlo = cr.rho < P.off ? 1 : 0;
+        u64 jhi = ((u64)P.tile_count + P.off + 5 - cr.rho) / 6;
+        u64 beg = ch * A.unit_j, end = beg + A.unit_j;
+        if (beg < jlo) beg = jlo;
+        if (end > jhi) end = jhi;
+        if (end <= beg) return false;
+        U.a = cr.a;
+        U.us_lo = P.U0_lo;
+        U.us_hi = P.U0_hi;
+        add128(U.us_lo, U.us_hi, beg);
+        U.n = (u32)(end - beg);
+        U.owner = (u32)c;
+        U.j0 = (u32)beg;
+        return true;
+    }
+    const JobSrc& J = A.jobsrc;
+    u32 job = find_job(J, unit, lane);
+    JobRec jr = J.jobs[job];
+    u64 beg = (unit - J.unit_prefix[job]) * A.unit_j;
+    if (beg >= jr.count) return false;
+    u64 n = jr.count - beg;
+    if (n > A.unit_j) n = A.unit_j;
+    U.a = jr.a;
+    U.us_lo = jr.u0_lo;
+    U.us_hi = jr.u0_hi;
+    add128(U.us_lo, U.us_hi, beg);
+    U.n = (u32)n;
+    U.owner = job;
+    U.j0 = (u32)beg;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// K1: sieve kernel
+// ---------------------------------------------------------------------------
+
+template <int NBV>
+__device__ __forceinline__ void stage_b(const SieveArgs& A, const u32* __restrict__ lds, u32* queue,
+                                        u32 cnt, const u32 (&sb)[NBV], u32 lane, const Unit& U,
+                                        u64& cnt_b) {
+    wave_sync();
+    bool alive = lane < cnt;
+    u32 j = alive ? queue[lane] : 0u;
+    cnt_b += alive ? 1 : 0;
+#pragma unroll
+    for (int i = 0; i < NBV; ++i) {
+        const u32 p0 = (u32)i * 64u;
+        const u32 pend = A.nb > p0 ? min(64u, A.nb - p0) : 0u;
+        for (u32 pl = 0; pl < pend; ++pl) {
+            const u32 p = p0 + pl;
+            if (ballot(alive) == 0) break;
+            const u32 s = __builtin_amdgcn_readlane(sb[i], pl);
+            const TierBPrime tp = A.tierb[p];
+            if (alive) {
+                const u32 x = j + s;
+                const u32 qt = (u32)((float)x * tp.rcp);
+                u32 r = x - __umul24(qt, tp.q);
+                const u32 r1 = r - tp.q, r2 = r + tp.q;
+                r = min(r, min(r1, r2));
+                const u32 w = lds[tp.lds_off + (r >> 5)];
+                if ((w >> (r & 31)) & 1u) alive = false;
+            }
+        }
+    }
+    {
+        const u64 m = ballot(alive);
+        if (m) {
+            u32 base = 0;
+            if (lane == 0) base = atomicAdd(A.mid_count, (u32)__popcll(m));
+            base = bcast32(base);
+            if (alive) {
+                const u32 slot = base + lanes_below(m);
+                if (slot < A.mid_cap) {
+                    MidRec rec;
+                    rec.owner = U.owner;
+                    rec.j = U.j0 + j;
+                    A.mid[slot] = rec;
+                }
+            }
+        }
+    }
+    wave_sync();
+}
+
+template <int RG, int NBV>
+__global__ __launch_bounds__(256) void sieve_kernel(SieveArgs A) {
+    extern __shared__ u32 lds[];
+    const u32 tid = threadIdx.x;
+    for (u32 i = tid; i < A.lds_words; i += blockDim.x) lds[i] = A.lds_image[i];
+    __syncthreads();
+
+    const u32 lane = lane_id();
+    const u32 wave = tid >> 6;
+    u32* queue = lds + A.lds_words + wave * kQueueCap;
+
+    // dense prime `lane` (lane < nd): modulus, pattern offset, per-row-group advance
+    u32 dq = 1, doff = 0, dstep = 0;
+    if (lane < A.nd) {
+        const DensePrime d = A.dense[lane];
+        dq = d.q;
+        doff = d.lds_off;
+        dstep = d.step;
+    }
+    u64 cnt_dense = 0, cnt_b = 0;
+
+    for (;;) {
+        u64 unit = 0;
+        if (lane == 0) unit = atomicAdd(A.unit_counter, 1ull);
+        unit = bcast64(unit);
+        if (unit >= A.n_units) break;
+        Unit U;
+        if (!resolve(A, unit, lane, U)) continue;
+        U.a = bcast64(U.a);
+        U.us_lo = bcast64(U.us_lo);
+        U.us_hi = bcast64(U.us_hi);
+        U.n = bcast32(U.n);
+        U.owner = bcast32(U.owner);
+        U.j0 = bcast32(U.j0);
+
+        // phases: dense prime p held by lane p as an LDS bit address, stage-B prime
+        // (64*i + lane) held in sb[i]
+        u32 sd = 0;
+        if (lane < A.nd) sd = doff * 32u + phase(A.pmod[lane], U.a, U.us_lo, U.us_hi);
+        u32 sb[NBV];
+#pragma unroll
+        for (int i = 0; i < NBV; ++i) {
+            const u32 p = (u32)i * 64u + lane;
+            sb[i] = p < A.nb ? phase(A.pmod[A.nd + p], U.a, U.us_lo, U.us_hi) : 0u;
+        }
+
+        const u32 rows = (U.n + kRowBits - 1) / kRowBits;
+        u32 qn = 0;
+        for (u32 row0 = 0; row0 < rows; row0 += RG) {
+            u32 acc[RG];
+#pragma unroll
+            for (int r = 0; r < RG; ++r) acc[r] = 0;
+            for (u32 p = 0; p < A.nd; ++p) {
+                const u32 ba = __builtin_amdgcn_readlane(sd, p);
+                const u32* src = lds + (ba >> 5) + lane;
+#pragma unroll
+                for (int r = 0; r < RG; ++r) {
+                    const u32 lo = src[64 * r];
+                    const u32 hi = src[64 * r + 1];
+                    acc[r] |= __builtin_amdgcn_alignbit(hi, lo, ba & 31u);
+                }
+            }
+            if (lane < A.nd) {
+                u32 s = sd - doff * 32u + dstep;
+                if (s >= dq) s -= dq;
+                sd = doff * 32u + s;
+            }
+            u32 alive[RG];
+#pragma unroll
+            for (int r = 0; r < RG; ++r) {
+                const u32 jb = (row0 + (u32)r) * (u32)kRowBits + lane * 32u;
+                u32 v = ~acc[r];
+                if (jb >= U.n) v = 0;
+                else if (U.n - jb < 32u) v &= (1u << (U.n - jb)) - 1u;
+                alive[r] = v;
+                cnt_dense += __popc(v);
+            }
+            // compaction: each round every lane with a pending candidate pushes its lowest
+            // one (first non-empty row) into the wave's LDS queue at its ballot rank
+            for (;;) {
+                u32 any = 0;
+#pragma unroll
+                for (int r = 0; r < RG; ++r) any |= alive[r];
+                const u64 m = ballot(any != 0);
+                if (m == 0) break;
+                if (any) {
+                    u32 jv = 0;
+                    bool taken = false;
+#pragma unroll
+                    for (int r = 0; r < RG; ++r) {
+                        if (!taken && alive[r]) {
+                            taken = true;
+                            jv = (row0 + (u32)r) * (u32)kRowBits + lane * 32u + (u32)__builtin_ctz(alive[r]);
+                            alive[r] &= alive[r] - 1u;
+                        }
+                    }
+                    queue[qn + lanes_below(m)] = jv;
+                }
+                qn += (u32)__popcll(m);
+                if (qn >= 64) {
+                    stage_b<NBV>(A, lds, queue, 64, sb, lane, U, cnt_b);
+                    const u32 rest = qn - 64;
+                    const u32 v = lane < rest ? queue[64 + lane] : 0u;
+                    wave_sync();
+                    if (lane < rest) queue[lane] = v;
+                    qn = rest;
+                }
+            }
+        }
+        if (qn) stage_b<NBV>(A, lds, queue, qn, sb, lane, U, cnt_b);
+    }
+
+    if (A.stats) {
+        for (int o = 32; o > 0; o >>= 1) {
+            cnt_dense += __shfl_xor(cnt_dense, o);
+            cnt_b += __shfl_xor(cnt_b, o);
+        }
+        if (lane == 0) {
+            atomicAdd(&A.stats[0], (unsigned long long)cnt_dense);
+            atomicAdd(&A.stats[1], (unsigned long long)cnt_b);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// u128 helpers
+// ---------------------------------------------------------------------------
+
+struct U2 {
+    u64 lo, hi;
+};
+
+__device__ __forceinline__ U2 mk(u64 lo, u64 hi) { U2 r; r.lo = lo; r.hi = hi; return r; }
+__device__ __forceinline__ bool ge(U2 a, U2 b) { return a.hi > b.hi || (a.hi == b.hi && a.lo >= b.lo); }
+__device__ __forceinline__ bool eq(U2 a, U2 b) { return a.hi == b.hi && a.lo == b.lo; }
+__device__ __forceinline__ bool iszero(U2 a) { return (a.lo | a.hi) == 0; }
+__device__ __forceinline__ U2 add(U2 a, U2 b) { U2 r; r.lo = a.lo + b.lo; r.hi = a.hi + b.hi + (r.lo < a.lo); return r; }
+__device__ __forceinline__ U2 sub(U2 a, U2 b) { U2 r; r.lo = a.lo - b.lo; r.hi = a.hi - b.hi - (a.lo < b.lo); return r; }
+__device__ __forceinline__ U2 shr1(U2 a) { return mk((a.lo >> 1) | (a.hi << 63), a.hi >> 1); }
+__device__ __forceinline__ U2 shl1(U2 a) { return mk(a.lo << 1, (a.hi << 1) | (a.lo >> 63)); }
+__device__ __forceinline__ int bitlen(U2 a) {
+    return a.hi ? 128 - __clzll(a.hi) : (a.lo ? 64 - __clzll(a.lo) : 0);
+}
+__device__ __forceinline__ bool bit(U2 a, int i) { return i >= 64 ? ((a.hi >> (i - 64)) & 1) : ((a.lo >> i) & 1); }
+// low 128 bits of a*b
+__device__ __forceinline__ U2 mullo(U2 a, U2 b) {
+    return mk(a.lo * b.lo, __umul64hi(a.lo, b.lo) + a.lo * b.hi + a.hi * b.lo);
+}
+// a (u128) * b (u64), low 128 bits
+__device__ __forceinline__ U2 mul_u64(U2 a, u64 b) { return mk(a.lo * b, __umul64hi(a.lo, b) + a.hi * b); }
+
+// ---------------------------------------------------------------------------
+// u128 Montgomery arithmetic, R = 2^128, n odd, n < 2^127
+// ---------------------------------------------------------------------------
+
+struct Mont {
+    U2 n, ni;   // ni = -n^-1 mod 2^128
+    U2 one;     // R mod n
+};
+
+__device__ __forceinline__ void mont_init(Mont& c, U2 n) {
+    c.n = n;
+    U2 x = n;   // n*n = 1 (mod 8)
+    const U2 two = mk(2, 0);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) x = mullo(x, sub(two, mullo(n, x)));
+    c.ni = sub(mk(0, 0), x);
+    // R mod n: start from 2^(b-1) < n and double up to 2^128
+    const int b = bitlen(n);
+    U2 r = b - 1 >= 64 ? mk(0, 1ull << (b - 1 - 64)) : mk(1ull << (b - 1), 0);
+    for (int i = b - 1; i < 128; ++i) {
+        r = shl1(r);
+        if (ge(r, n)) r = sub(r, n);
+    }
+    c.one = r;
+}
+
+__device__ __forceinline__ U2 mont_mul(const Mont& c, U2 a, U2 b) {
+    // t = a*b, four 64-bit limbs
+    u64 l00 = a.lo * b.lo, h00 = __umul64hi(a.lo, b.lo);
+    u64 l01 = a.lo * b.hi, h01 = __umul64hi(a.lo, b.hi);
+    u64 l10 = a.hi * b.lo, h10 = __umul64hi(a.hi, b.lo);
+    u64 l11 = a.hi * b.hi, h11 = __umul64hi(a.hi, b.hi);
+    u64 t0 = l00;
+    u64 t1 = h00 + l01;
+    u64 c1 = t1 < l01;
+    t1 += l10;
+    c1 += t1 < l10;
+    u64 t2 = h01 + h10;
+    u64 c2 = t2 < h10;
+    t2 += l11;
+    c2 += t2 < l11;
+    t2 += c1;
+    c2 += t2 < c1;
+    u64 t3 = h11 + c2;
+    // m = t * ni mod 2^128
+    u64 m0 = t0 * c.ni.lo;
+    u64 m1 = __umul64hi(t0, c.ni.lo) + t0 * c.ni.hi + t1 * c.ni.lo;
+    // s = m * n, limbs 1..3 (limb 0 cancels t0)
+    l00 = m0 * c.n.lo; h00 = __umul64hi(m0, c.n.lo);
+    l01 = m0 * c.n.hi; h01 = __umul64hi(m0, c.n.hi);
+    l10 = m1 * c.n.lo; h10 = __umul64hi(m1, c.n.lo);
+    l11 = m1 * c.n.hi; h11 = __umul64hi(m1, c.n.hi);
+    u64 s1 = h00 + l01;
+    u64 d1 = s1 < l01;
+    s1 += l10;
+    d1 += s1 < l10;
+    u64 s2 = h01 + h10;
+    u64 d2 = s2 < h10;
+    s2 += l11;
+    d2 += s2 < l11;
+    s2 += d1;
+    d2 += s2 < d1;
+    u64 s3 = h11 + d2;
+    (void)s1;
+    // (t + s) / 2^128: the low 128 bits sum to 0 mod 2^128 with a carry iff t_low != 0
+    const u64 k = (t0 | t1) != 0;
+    u64 r0 = t2 + s2;
+    u64 k2 = r0 < s2;
+    r0 += k;
+    k2 += r0 < k;
+    u64 r1 = t3 + s3 + k2;
+    U2 r = mk(r0, r1);
+    if (ge(r, c.n)) r = sub(r, c.n);
+    return r;
+}
+
+__device__ __forceinline__ U2 madd(const Mont& c, U2 a, U2 b) { U2 r = add(a, b); if (ge(r, c.n)) r = sub(r, c.n); return r; }
+__device__ __forceinline__ U2 msub(const Mont& c, U2 a, U2 b) { return ge(a, b) ? sub(a, b) : add(sub(a, b), c.n); }
+__device__ __forceinline__ U2 mhalf(const Mont& c, U2 a) { if (a.lo & 1) a = add(a, c.n); return shr1(a); }
+
+// Montgomery form of a small constant x: x*R mod n by double-and-add of R mod n
+__device__ __forceinline__ U2 mont_small(const Mont& c, u32 x) {
+    U2 r = mk(0, 0);
+    for (int i = 31; i >= 0; --i) {
+        r = madd(c, r, r);
+        if ((x >> i) & 1) r = madd(c, r, c.one);
+    }
+    return r;
+}
+
+__device__ __forceinline__ U2 mont_pow(const Mont& c, U2 bm, U2 e) {
+    U2 r = c.one;
+    for (int i = bitlen(e) - 1; i >= 0; --i) {
+        r = mont_mul(c, r, r);
+        if (bit(e, i)) r = mont_mul(c, r, bm);
+    }
+    return r;
+}
+
+__device__ bool strong_mr(const Mont& c, u32 a) {
+    U2 d = sub(c.n, mk(1, 0));
+    int s = 0;
+    while (!(d.lo & 1)) { d = shr1(d); ++s; }
+    const U2 one = c.one, mone = sub(c.n, c.one);
+    U2 x = mont_pow(c, mont_small(c, a), d);
+    if (eq(x, one) || eq(x, mone)) return true;
+    for (int i = 1; i < s; ++i) {
+        x = mont_mul(c, x, x);
+        if (eq(x, mone)) return true;
+        if (eq(x, one)) return false;
+    }
+    return false;
+}
+
+// n mod m for a small odd m (< 2^16)
+__device__ __forceinline__ u32 mod_small(U2 n, u32 m) {
+    PrimeMod pm;
+    pm.q = m;
+    u64 c32 = (0xffffffffull % m + 1) % m;
+    pm.c32 = (u32)c32;
+    pm.c64 = (u32)(c32 * c32 % m);
+    pm.c96 = (u32)((u64)pm.c64 * c32 % m);
+    pm.rcp = 1.0 / (double)m;
+    return mod128(n.lo, n.hi, pm);
+}
+
+// Jacobi symbol (a/m), m odd positive, both small
+__device__ __forceinline__ int jacobi_small(u32 a, u32 m) {
+    int res = 1;
+    a %= m;
+    while (a) {
+        while (!(a & 1)) {
+            a >>= 1;
+            const u32 r8 = m & 7;
+            if (r8 == 3 || r8 == 5) res = -res;
+        }
+        const u32 t = a; a = m; m = t;
+        if ((a & 3) == 3 && (m & 3) == 3) res = -res;
+        a %= m;
+    }
+    return m == 1 ? res : 0;
+}
+
+// (D/n) for odd D, |D| small, n odd large
+__device__ __forceinline__ int jacobi_D(int D, U2 n) {
+    const u32 ad = (u32)(D < 0 ? -D : D);
+    int res = 1;
+    if (D < 0 && (n.lo & 3) == 3) res = -res;          // (-1/n)
+    // (|D|/n) = (n/|D|) * (-1)^(((|D|-1)/2) * ((n-1)/2))
+    if ((ad & 3) == 3 && (n.lo & 3) == 3) res = -res;
+    return res * jacobi_small(mod_small(n, ad), ad);
+}
+
+// strong Lucas-Selfridge test (P = 1, Q = (1-D)/4, D = 5, -7, 9, ... first with (D/n) = -1).
+// `ok` is cleared when no parameter was found in range (never happens for the chain
+// values tested here: they are 2 mod 3, so never perfect squares).
+__device__ bool strong_lucas(const Mont& c, bool& ok) {
+    int D = 5;
+    int tries = 0;
+    for (;;) {
+        const int j = jacobi_D(D, c.n);
+        if (j == -1) break;
+        if (j == 0) {
+            const u32 ad = (u32)(D < 0 ? -D : D);
+            if (!(c.n.hi == 0 && c.n.lo == ad)) return false;
+        }
+        D = D > 0 ? -(D + 2) : -(D - 2);
+        if (++tries > 200) { ok = false; return false; }
+    }
+    const int Q = (1 - D) / 4;
+    U2 Dm = mont_small(c, (u32)(D < 0 ? -D : D));
+    if (D < 0) Dm = msub(c, mk(0, 0), Dm);
+    U2 Qm = mont_small(c, (u32)(Q < 0 ? -Q : Q));
+    if (Q < 0) Qm = msub(c, mk(0, 0), Qm);
+    U2 d = add(c.n, mk(1, 0));
+    int s = 0;
+    while (!(d.lo & 1)) { d = shr1(d); ++s; }
+    U2 U = c.one, V = c.one, Qk = Qm;
+    for (int i = bitlen(d) - 2; i >= 0; --i) {
+        U = mont_mul(c, U, V);
+        V = msub(c, mont_mul(c, V, V), madd(c, Qk, Qk));
+        Qk = mont_mul(c, Qk, Qk);
+        if (bit(d, i)) {
+            const U2 nu = mhalf(c, madd(c, U, V));
+            const U2 nv = mhalf(c, madd(c, mont_mul(c, Dm, U), V));
+            U = nu;
+            V = nv;
+            Qk = mont_mul(c, Qk, Qm);
+        }
+    }
+    if (iszero(U) || iszero(V)) return true;
+    for (int r = 1; r < s; ++r) {
+        V = msub(c, mont_mul(c, V, V), madd(c, Qk, Qk));
+        if (iszero(V)) return true;
+        Qk = mont_mul(c, Qk, Qk);
+    }
+    return false;
+}
+
+__constant__ PrimeMod c_small[18];   // 2..61
+__constant__ PrimeMod c_trial[62];   // 67..409 (v13.cu:180-199, CHAIN_SCREEN_PRIMES[0..61])
+
+// Baillie-PSW: the decision of GMP's mpz_probab_prime_p(n, 1) (v13.cu:1841)
+__device__ bool is_prime_bpsw(U2 n, bool& ok) {
+    if (n.hi == 0 && n.lo < 2) return false;
+#pragma unroll 1
+    for (int i = 0; i < 18; ++i) {
+        const u32 p = c_small[i].q;
+        if (n.hi == 0 && n.lo == p) return true;
+        if (mod128(n.lo, n.hi, c_small[i]) == 0) return false;
+    }
+    if (n.hi == 0 && n.lo < 61ull * 61ull) return true;
+    Mont c;
+    mont_init(c, n);
+    if (!strong_mr(c, 2)) return false;
+    return strong_lucas(c, ok);
+}
+
+__device__ bool is_prime_mr(U2 n, const ProveArgs& A) {
+    if (n.hi == 0 && n.lo < 2) return false;
+    if (n.hi == 0 && n.lo < 4) return true;
+    if (!(n.lo & 1)) return false;
+    Mont c;
+    mont_init(c, n);
+    for (int i = 0; i < A.n_bases; ++i) {
+        const u32 b = A.bases_mr[i];
+        if (n.hi == 0 && (u64)b % n.lo == 0) continue;
+        if (!strong_mr(c, b)) return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool test_prime(U2 n, const ProveArgs& A, bool& ok) {
+    return A.test_kind == 0 ? is_prime_bpsw(n, ok) : is_prime_mr(n, A);
+}
+
+// ---------------------------------------------------------------------------
+// K1c: deep stage
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ U2 cand_n(u64 W, u64 a, u64 u_lo, u64 u_hi) {
+    U2 n = mul_u64(mk(u_lo, u_hi), W);
+    return add(n, mk(a, 0));
+}
+
+__global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
+    const u32 lane = lane_id();
+    u32 total = *A.mid_count;
+    if (total > A.mid_cap) total = A.mid_cap;
+    const u32 stride = gridDim.x * blockDim.x;
+    // grid-stride with a wave-uniform trip count so ballots stay convergent
+    const u32 first = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    for (u32 base = first; base < total; base += stride) {
+        const u32 idx = base + lane;
+        bool alive = idx < total;
+        U2 n = mk(0, 0);
+        u32 owner = 0, j = 0;
+        if (alive) {
+            const MidRec rec = A.mid[idx];
+            owner = rec.owner;
+            j = rec.j;
+            u64 ulo, uhi, a;
+            if (A.mode == 0) {
+                const ClassRec cr = A.plain.cls[owner];
+                a = cr.a;
+                ulo = A.plain.U0_lo;
+                uhi = A.plain.U0_hi;
+            } else {
+                const JobRec jr = A.jobsrc.jobs[owner];
+                a = jr.a;
+                ulo = jr.u0_lo;
+                uhi = jr.u0_hi;
+            }
+            add128(ulo, uhi, j);
+            n = cand_n(A.W, a, ulo, uhi);
+        }
+        for (u32 p = 0; p < A.nc; ++p) {
+            if (ballot(alive) == 0) break;
+            const DeepPrime dp = A.deep[p];
+            if (alive) {
+                const u32 r = mod128(n.lo, n.hi, dp.pm);
+                const u32* h = A.hash + dp.hash_off;
+                u32 slot = r & (kMaxHashSlots - 1);
+                for (int probe = 0; probe < kMaxHashSlots; ++probe) {
+                    const u32 v = h[slot];
+                    if (v == r) { alive = false; break; }
+                    if (v == 0xffffffffu) break;
+                    slot = (slot + 1) & (kMaxHashSlots - 1);
+                }
+            }
+        }
+        const u64 m = ballot(alive);
+        if (m) {
+            u32 ob = 0;
+            if (lane == 0) ob = atomicAdd(A.out_count, (u32)__popcll(m));
+            ob = bcast32(ob);
+            if (alive) {
+                const u32 slot = ob + lanes_below(m);
+                if (slot < A.out_cap) {
+                    if (A.mode == 0) {
+                        const ClassRec cr = A.plain.cls[owner];
+                        ccs_survivor s;
+                        s.base_idx = cr.base_idx;
+                        s.k_offset = cr.k;
+                        s.tile_idx = 6u * j + cr.rho - A.plain.off;
+                        s._pad = 0;
+                        ((ccs_survivor*)A.out)[slot] = s;
+                    } else {
+                        ccs_job_survivor s;
+                        s.job_idx = owner;
+                        s.u_off = j;
+                        ((ccs_job_survivor*)A.out)[slot] = s;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K2: prove kernel — GMP confirmation engine per survivor
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(128) void prove_kernel(ProveArgs A) {
+    u32 total = *A.surv_count;
+    if (total > A.surv_cap) total = A.surv_cap;
+    const u32 stride = gridDim.x * blockDim.x;
+    for (u32 idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+        U2 n;
+        if (A.mode == 0) {
+            const ccs_survivor s = ((const ccs_survivor*)A.surv)[idx];
+            U2 tile = mk(A.T0_lo, A.T0_hi);
+            tile = add(tile, mk(s.tile_idx, 0));
+            U2 k = add(mul_u64(tile, A.WP), mk(s.k_offset, 0));
+            n = add(mul_u64(k, A.M), mk(A.bases[s.base_idx], 0));
+        } else {
+            const ccs_job_survivor s = ((const ccs_job_survivor*)A.surv)[idx];
+            const JobRec jr = A.jobs[s.job_idx];
+            u64 ulo = jr.u0_lo, uhi = jr.u0_hi;
+            add128(ulo, uhi, s.u_off);
+            n = cand_n(A.W, jr.a, ulo, uhi);
+        }
+        ccs_chain_result res;
+        res.survivor_idx = idx;
+        res.chain_len = 0;
+        res.back_steps = 0;
+        res.flags = 0;
+        res.root_lo = 0;
+        res.root_hi = 0;
+        bool ok = true;
+        do {
+            if (A.bits && bitlen(n) != A.bits) break;
+            res.flags |= CCS_F_BITS_OK;
+            bool div = false;
+            for (int i = 0; i < 62; ++i)
+                if (mod128(n.lo, n.hi, c_trial[i]) == 0) { div = true; break; }
+            if (div) break;
+            res.flags |= CCS_F_TRIAL_OK;
+            if (n.hi >> 63) { res.flags |= CCS_F_OVERFLOW; break; }
+            if (!test_prime(n, A, ok)) break;
+            res.flags |= CCS_F_PRIME;
+            U2 root = n;
+            u32 back = 0;
+            U2 pred = shr1(sub(n, mk(1, 0)));
+            if (ge(pred, mk(2, 0)) && test_prime(pred, A, ok)) {
+                res.flags |= CCS_F_NONROOT;
+                root = pred;
+                back = 1;
+                for (int steps = 0; steps < 64; ++steps) {
+                    pred = shr1(sub(root, mk(1, 0)));
+                    if (!ge(pred, mk(2, 0))) break;
+                    if (!test_prime(pred, A, ok)) break;
+                    root = pred;
+                    ++back;
+                }
+            }
+            u32 len = 1;
+            U2 cur = root;
+            while (len < 64) {
+                if (cur.hi >> 62) { res.flags |= CCS_F_OVERFLOW; break; }
+                const U2 nx = add(shl1(cur), mk(1, 0));
+                if (!test_prime(nx, A, ok)) break;
+                ++len;
+                cur = nx;
+            }
+            res.root_lo = root.lo;
+            res.root_hi = root.hi;
+            res.chain_len = len;
+            res.back_steps = back;
+        } while (0);
+        if (!ok) res.flags |= CCS_F_UNSUPPORTED;
+        ((ccs_chain_result*)A.out)[idx] = res;
+    }
+}
+
+__global__ void zero_kernel(u32* p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+
+template <int RG, int NBV>
+static hipError_t launch_sieve_t(const SieveArgs& a, int grid, int block, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((sieve_kernel<RG, NBV>), dim3(grid), dim3(block), lds, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sieve(const SieveArgs& a, int rg, int nbv, int grid, int block, size_t lds_bytes,
+                        hipStream_t s) {
+#define CCS_SIEVE_CASE(R, B) \
+    if (rg == R && nbv == B) return launch_sieve_t<R, B>(a, grid, block, lds_bytes, s);
+    CCS_SIEVE_CASE(1, 1) CCS_SIEVE_CASE(1, 2) CCS_SIEVE_CASE(1, 3) CCS_SIEVE_CASE(1, 4)
+    CCS_SIEVE_CASE(2, 1) CCS_SIEVE_CASE(2, 2) CCS_SIEVE_CASE(2, 3) CCS_SIEVE_CASE(2, 4)
+    CCS_SIEVE_CASE(4, 1) CCS_SIEVE_CASE(4, 2) CCS_SIEVE_CASE(4, 3) CCS_SIEVE_CASE(4, 4)
+#undef CCS_SIEVE_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_deep(const DeepArgs& a, u32 max_entries, hipStream_t s) {
+    u32 blocks = (max_entries + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(deep_kernel, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_prove(const ProveArgs& a, u32 max_entries, hipStream_t s) {
+    u32 blocks = (max_entries + 127) / 128;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(prove_kernel, dim3(blocks), dim3(128), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(u32* p, size_t n_words, hipStream_t s) {
+    u32 blocks = (u32)((n_words + 255) / 256);
+    if (blocks < 1) blocks = 1;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(zero_kernel, dim3(blocks), dim3(256), 0, s, p, n_words);
+    return hipGetLastError();
+}
+
+hipError_t upload_prove_constants(const PrimeMod* small18, const PrimeMod* trial62) {
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_small), small18, sizeof(PrimeMod) * 18);
+    if (e != hipSuccess) return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_trial), trial62, sizeof(PrimeMod) * 62);
+}
+
+}  // namespace ccs

@@ -1,0 +1,218 @@
+/*
+ * ccs.h — C ABI of the MI355X-native Cunningham-chain (first kind) candidate pipeline.
+ *
+ * The reference (nmicic/cunningham-chain-search) has no plugin or FFI layer: its
+ * boundary is the in-file contract between the host driver, the filter kernel K1 and
+ * the prover.  Each entry point below names the reference code it replaces.
+ *   v13 = src/cuda/cc18_filter_cuda_CpC_v13.cu
+ *   v16 = src/cuda/experimental/cc20_first_kind_immune_v16.cu
+ *
+ * Conventions: plain pointers and sizes only; device pointers are prefixed d_, host
+ * pointers h_; streams are hipStream_t passed as void* (NULL = the context's stream).
+ * Every call returns CCS_OK (0), a negative error, or CCS_OVERFLOW (> 0) when an output
+ * buffer was too small — the true count is still reported.  Nothing calls exit().
+ * One host thread per context.
+ */
+#ifndef CCS_H
+#define CCS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CCS_OK 0
+#define CCS_OVERFLOW 1
+#define CCS_ERR_INVALID (-1)
+#define CCS_ERR_HIP (-2)
+#define CCS_ERR_NOMEM (-3)
+#define CCS_ERR_UNSUPPORTED (-4)
+#define CCS_ERR_BUSY (-5)
+
+typedef struct ccs_ctx ccs_ctx;       /* one GPU: stream, scratch buffers, timing events   */
+typedef struct ccs_shape ccs_shape;   /* CRT shape + filter profile, device-resident tables */
+
+/* 128-bit unsigned as two 64-bit halves */
+typedef struct {
+    uint64_t lo, hi;
+} ccs_u128;
+
+/* Shape parameters.  Candidates n = (tile*WP + k)*M + base with M = prod(crt_primes),
+ * WP = prod(wheel_primes), base admissible mod M and k admissible mod the wheel primes
+ * for chains of length `target`, and n = 5 (mod 6) (v13.cu:907-1059).  The filter
+ * rejects n when a filter prime q divides p_i = 2^i (n+1) - 1 for i in [0, depth) if
+ * q <= pos0_max_q, else for i in [1, depth) (v13.cu:793-847, L2/ext vs line sieve). */
+typedef struct {
+    uint32_t n_crt;
+    const uint32_t* crt_primes;     /* v13 CRT_PRIMES {5,...,19}                       */
+    uint32_t n_wheel;
+    const uint32_t* wheel_primes;   /* v13 WHEEL_PRIMES {23,29,31}                     */
+    int32_t target;                 /* v13 target_len                                   */
+    int32_t depth;                  /* v13 sieve_len, 1..32                             */
+    uint32_t n_filter;
+    const uint32_t* filter_primes;  /* ascending, all above the largest wheel prime, < 2^18 */
+    uint32_t n_stages;              /* reporting stages: stage s ends at filter index  */
+    const uint32_t* stage_end;      /*   stage_end[s] (exclusive); may be 0/NULL        */
+    uint32_t pos0_max_q;            /* reference: 97                                   */
+    uint32_t flags;                 /* CCS_SHAPE_*                                      */
+} ccs_shape_params;
+
+#define CCS_SHAPE_NO_CLASSES 1u     /* build filter tables only (class-form jobs)      */
+
+typedef struct {
+    uint64_t M, WP, W;              /* W = 6*WP*M                                       */
+    uint32_t n_bases;
+    uint32_t n_filter;
+    uint64_t n_classes;             /* admissible (base, k) pairs = classes mod W       */
+    uint32_t n_dense, n_tierb, n_deep;   /* how the filter primes are split on the GPU  */
+    uint32_t rows_per_group, unit_j;
+} ccs_shape_info;
+
+/* Survivor record — identical to v13 `Survivor` (v13.cu:207-212): tile_idx is relative
+ * to the launch's tile_start. */
+typedef struct {
+    uint32_t base_idx, k_offset, tile_idx, _pad;
+} ccs_survivor;
+
+/* Class-form survivor (batched/served jobs): n = a + W*(u0 + u_off) of job job_idx.
+ * The 8-byte analogue of v16 `QSurvivor` (v16...cu:103-108). */
+typedef struct {
+    uint32_t job_idx, u_off;
+} ccs_job_survivor;
+
+/* Class-form job ("fingerprint shape"): residue class a mod W and u in [u0, u0+count). */
+typedef struct {
+    uint64_t a;
+    ccs_u128 u0;
+    uint64_t count;
+} ccs_job;
+
+/* Prove result per survivor (same index as the survivor array): the outcome of the
+ * GMP confirmation engine prover_worker (v13.cu:1794-1924); chain_len counts from the
+ * true root reached by the backward walk. */
+typedef struct {
+    uint32_t survivor_idx;
+    uint32_t chain_len;     /* 0 unless CCS_F_PRIME                              */
+    uint32_t back_steps;    /* 0 = the survivor is the root                       */
+    uint32_t flags;         /* CCS_F_*                                             */
+    uint64_t root_lo, root_hi;
+} ccs_chain_result;
+
+#define CCS_F_BITS_OK 1u       /* passed the bit-size gate (v13.cu:1828)          */
+#define CCS_F_TRIAL_OK 2u      /* no factor among 67..409 (v13.cu:1832-1838)      */
+#define CCS_F_PRIME 4u         /* probable prime (Baillie-PSW or MR bases)        */
+#define CCS_F_NONROOT 8u       /* (n-1)/2 prime: walked back (v13.cu:1849-1867)   */
+#define CCS_F_OVERFLOW 16u     /* a chain value reached 2^127: not followed       */
+#define CCS_F_UNSUPPORTED 32u  /* no Lucas parameter (perfect square): unreachable */
+
+#define CCS_TEST_BPSW 0        /* mpz_probab_prime_p(n, 1) decision (v13.cu:1841) */
+#define CCS_TEST_MR 1          /* strong Miller-Rabin over explicit bases          */
+
+typedef struct {
+    int32_t bits;              /* bit-size gate (--bits), 0 = off                  */
+    int32_t test_kind;         /* CCS_TEST_*                                       */
+    uint32_t n_bases;          /* for CCS_TEST_MR, <= 16                           */
+    uint32_t bases[16];
+} ccs_prove_params;
+
+/* Filter statistics of the last launch on a context. */
+typedef struct {
+    uint64_t candidates;       /* lattice points in the range (v33.c:1900 local_wheel++) */
+    uint64_t alive_dense;      /* after the dense stage                            */
+    uint64_t alive_tierb;      /* after stage B                                    */
+    uint64_t survivors;        /* after all filter primes                          */
+    uint32_t dense_end, tierb_end;   /* filter-prime indices where the GPU stages end */
+    float ms_sieve, ms_deep, ms_prove;   /* device time of each kernel (last call)   */
+} ccs_filter_stats;
+
+/* Kernel tiling knobs (chosen by the autotuner, or set explicitly). */
+typedef struct {
+    uint32_t n_dense;          /* filter primes handled by the dense pattern stage  */
+    uint32_t n_tierb;          /* next primes handled by stage B (<= 256)           */
+    uint32_t rows_per_group;   /* 1, 2 or 4                                         */
+    uint32_t unit_j;           /* candidates per work unit, multiple of 2048        */
+    uint32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu             */
+} ccs_tuning;
+
+/* ---- context (replaces the module statics + streams of v13.cu:268-327) ---- */
+int ccs_ctx_create(int device, ccs_ctx** out);
+void ccs_ctx_destroy(ccs_ctx* ctx);
+int ccs_device_count(int* out);
+const char* ccs_last_error(void);
+
+/* ---- device memory helpers (plain pointers for callers without a framework) ---- */
+int ccs_dmalloc(ccs_ctx* ctx, void** d_ptr, uint64_t bytes);
+int ccs_dfree(ccs_ctx* ctx, void* d_ptr);
+int ccs_memcpy_h2d(ccs_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes);
+int ccs_memcpy_d2h(ccs_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes);
+int ccs_memset_d(ccs_ctx* ctx, void* d_dst, int value, uint64_t bytes);
+int ccs_synchronize(ccs_ctx* ctx);
+
+/* ---- shape (replaces init_filter_tables v13.cu:793-901, generate_bases :907-1059,
+ *      gpu_upload_tables :1065-1222) ---- */
+int ccs_shape_build(ccs_ctx* ctx, const ccs_shape_params* params, const ccs_tuning* tuning,
+                    ccs_shape** out);
+void ccs_shape_free(ccs_shape* shape);
+int ccs_shape_query(const ccs_shape* shape, ccs_shape_info* out);
+int ccs_shape_bases(const ccs_shape* shape, uint64_t* h_out, uint32_t cap);
+/* residue class a mod W and mod-6 tile residue of every (base, k) class, base-major */
+int ccs_shape_classes(const ccs_shape* shape, uint64_t* h_a, uint32_t* h_base_idx,
+                      uint32_t* h_k, uint32_t* h_rho, uint64_t cap);
+int ccs_shape_set_tuning(ccs_shape* shape, const ccs_tuning* tuning);
+int ccs_shape_get_tuning(const ccs_shape* shape, ccs_tuning* out);
+uint64_t ccs_count_candidates(const ccs_shape* shape, ccs_u128 tile_start, uint32_t tile_count);
+
+/* ---- plain filter: K1 over tiles [tile_start, tile_start + tile_count), all bases
+ *      (replaces prepare_batch v13.cu:1232-1259 + cc18_filter_kernel :1551-1714 and its
+ *      launch :3769-3796).  Survivors go to d_out in unspecified order; *d_count (device)
+ *      receives the true count.  Asynchronous on `stream`. ---- */
+int ccs_filter_launch(ccs_ctx* ctx, const ccs_shape* shape, ccs_u128 tile_start,
+                      uint32_t tile_count, ccs_survivor* d_out, uint32_t cap, uint32_t* d_count,
+                      void* stream);
+/* Synchronous host form with statistics. */
+int ccs_filter(ccs_ctx* ctx, const ccs_shape* shape, ccs_u128 tile_start, uint32_t tile_count,
+               ccs_survivor* h_out, uint32_t cap, uint32_t* h_count, ccs_filter_stats* stats);
+int ccs_last_stats(ccs_ctx* ctx, ccs_filter_stats* out);
+
+/* ---- prove (replaces cc18_prove_kernel v13.cu:1462-1538 and the GMP prover pool
+ *      prover_worker :1794-1924): one ccs_chain_result per survivor ---- */
+int ccs_prove_launch(ccs_ctx* ctx, const ccs_shape* shape, ccs_u128 tile_start,
+                     const ccs_survivor* d_surv, const uint32_t* d_count, uint32_t cap,
+                     const ccs_prove_params* params, ccs_chain_result* d_out, void* stream);
+int ccs_prove(ccs_ctx* ctx, const ccs_shape* shape, ccs_u128 tile_start,
+              const ccs_survivor* h_surv, uint32_t n, const ccs_prove_params* params,
+              ccs_chain_result* h_out);
+
+/* Format accepted chains like the v13 results file (v13.cu:2030-2033):
+ * "[TARGET ][NON-ROOT ]CC<len> 0x<HEX> <bits>\n" for len >= log_threshold or >= target.
+ * Returns the number of bytes needed (excluding NUL). */
+int ccs_format_results(const ccs_chain_result* res, uint32_t n, int32_t log_threshold,
+                       int32_t target, char* buf, uint64_t buf_size, uint64_t* needed);
+
+/* ---- kernel timing: while collecting, every sieve / deep / prove launch on the context
+ *      records HIP events on the stream it runs on (the reference times K1 with CUDA events,
+ *      v13.cu:3786-3796) ---- */
+typedef struct {
+    uint32_t n_sieve, n_deep, n_prove;
+    double ms_sieve, ms_deep, ms_prove;   /* summed device time per kernel kind */
+} ccs_kernel_times;
+
+int ccs_profile_start(ccs_ctx* ctx);
+int ccs_profile_stop(ccs_ctx* ctx, ccs_kernel_times* out);   /* synchronises the stream */
+
+/* ---- batched form: many class-form jobs in one persistent, work-stealing launch
+ *      (replaces the v16 per-(variant, chunk) launch loop v16...cu:5611-5965) ---- */
+int ccs_batch_launch(ccs_ctx* ctx, const ccs_shape* shape, const ccs_job* h_jobs, uint32_t n_jobs,
+                     ccs_job_survivor* d_out, uint32_t cap, uint32_t* d_count, void* stream);
+int ccs_batch(ccs_ctx* ctx, const ccs_shape* shape, const ccs_job* h_jobs, uint32_t n_jobs,
+              ccs_job_survivor* h_out, uint32_t cap, uint32_t* h_count, ccs_filter_stats* stats);
+int ccs_prove_jobs(ccs_ctx* ctx, const ccs_shape* shape, const ccs_job* h_jobs, uint32_t n_jobs,
+                   const ccs_job_survivor* h_surv, uint32_t n, const ccs_prove_params* params,
+                   ccs_chain_result* h_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CCS_H */

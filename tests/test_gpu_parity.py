@@ -1,0 +1,300 @@
+"""Parity of the HIP pipeline (through the C ABI) with the CPU oracle.
+
+Bit-exact: survivor sets (sorted, since the GPU emits in arbitrary order), the stage
+counters at the GPU's stage boundaries, and every field of the prove results.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import ccs
+from ccs import workloads as WL
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_shape(oracle, name, **over):
+    p = WL.shape_params(name)
+    p.update(over)
+    return oracle.Shape(p["crt"], p["wheel"], p["target"], p["depth"], p["filter_primes"], p["pos0_max_q"])
+
+
+def sorted_keys(surv):
+    """(tile, base, k) keys of plain survivor records, sorted"""
+    if isinstance(surv, np.ndarray) and surv.dtype.names:
+        k = list(zip(surv["tile_idx"].tolist(), surv["base_idx"].tolist(), surv["k_offset"].tolist()))
+    else:
+        k = [(int(s[2]), int(s[0]), int(s[1])) for s in surv]
+    return sorted(k)
+
+
+def check_plain(ctx, oracle, name, tile_start, tile_count, shape=None, oshape=None, **tuning):
+    say(f"[plain] {name} tile_start={tile_start} tile_count={tile_count} tuning={tuning}")
+    s = shape or WL.make_shape(ctx, name)
+    if tuning:
+        s.set_tuning(**tuning)
+    o = oshape or oracle_shape(oracle, name)
+    g_surv, st = s.filter(tile_start, tile_count, cap=1 << 20)
+    cuts = [st["dense_end"], st["tierb_end"], len(s.filter_primes)]
+    o_surv, counts = o.filter(tile_start, tile_count, cap=1 << 20, cuts=cuts)
+    assert st["candidates"] == counts[0] == s.count_candidates(tile_start, tile_count)
+    assert st["alive_dense"] == counts[1]
+    assert st["alive_tierb"] == counts[2]
+    assert st["survivors"] == counts[3] == len(o_surv)
+    assert sorted_keys(g_surv) == sorted_keys(o_surv)
+    return g_surv, o_surv, st
+
+
+@pytest.fixture(scope="module")
+def ref(ctx):
+    return WL.make_shape(ctx, "reference")
+
+
+@pytest.fixture(scope="module")
+def oref(oracle):
+    return oracle_shape(oracle, "reference")
+
+
+def test_reference_shape_tables(ref, oref):
+    assert (ref.M, ref.WP, ref.n_bases, ref.n_classes) == (1616615, 20677, 36, 123552)
+    assert [int(b) for b in ref.bases()] == [int(b) for b in oref.bases()]
+    a, b, k, r = ref.classes()
+    bases = ref.bases()
+    assert np.all(a == (r.astype(object) * 20677 * 1616615 + k.astype(object) * 1616615 +
+                        bases[b].astype(object)).astype(np.uint64))
+    assert np.all(a % 6 == 5)
+
+
+def test_reference_campaign_batch(ctx, oracle, ref, oref):
+    t0, _ = WL.reference_range(seed=0x13, bits=89, n_tiles=16384)
+    g, o, st = check_plain(ctx, oracle, "reference", t0, 2400, shape=ref, oshape=oref)
+    assert len(g) > 300
+
+
+@pytest.mark.parametrize("off", range(6))
+@pytest.mark.parametrize("tile_count", [1, 2, 5, 7, 13, 6 * 2048 + 3])
+def test_ragged_tile_ranges(ctx, oracle, ref, oref, off, tile_count):
+    t0 = (1 << 86) // (20677 * 1616615) // 6 * 6 + off
+    check_plain(ctx, oracle, "reference", t0, tile_count, shape=ref, oshape=oref)
+
+
+def test_empty_range(ref):
+    say("[plain] empty range")
+    surv, st = ref.filter(12345, 0)
+    assert len(surv) == 0 and st["candidates"] == 0 and st["survivors"] == 0
+
+
+def test_u128_carry_and_high_bits(ctx, oracle, ref, oref):
+    # u = U0 + j crosses 2^64, and n reaches ~2^125
+    t0 = 6 * ((1 << 64) - 3000)
+    check_plain(ctx, oracle, "reference", t0, 36000, shape=ref, oshape=oref)
+    t1 = ((1 << 125) // (20677 * 1616615))
+    check_plain(ctx, oracle, "reference", t1, 600, shape=ref, oshape=oref)
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(rows_per_group=1), dict(rows_per_group=4), dict(n_dense=0, n_tierb=143 - 0 if False else 100),
+    dict(n_dense=7, n_tierb=7), dict(n_dense=43, n_tierb=100), dict(n_dense=14, n_tierb=0),
+    dict(n_dense=21, n_tierb=64), dict(n_dense=21, n_tierb=65), dict(unit_j=2048), dict(unit_j=6144),
+    dict(blocks_per_cu=1),
+])
+def test_tuning_does_not_change_results(ctx, oracle, oref, tuning):
+    s = WL.make_shape(ctx, "reference")
+    t0 = (1 << 88) // (20677 * 1616615) + 777
+    check_plain(ctx, oracle, "reference", t0, 900, shape=s, oshape=oref, **tuning)
+
+
+def test_config1_shape(ctx, oracle):
+    t0, tc, _ = WL.c1_range(seed=0xC1, t_count=1 << 11)
+    check_plain(ctx, oracle, "c1", t0, tc)
+
+
+def test_config2_shape_counters(ctx, oracle):
+    s = WL.make_shape(ctx, "c2")
+    o = oracle_shape(oracle, "c2")
+    t0, _, _ = WL.c2_range(seed=0xC2)
+    check_plain(ctx, oracle, "c2", t0, 600, shape=s, oshape=o)
+
+
+def test_deep_stage_with_survivors(ctx, oracle):
+    # shallow depth and few dense/stage-B primes: thousands of survivors pass through the
+    # deep stage's hash tests
+    s = WL.make_shape(ctx, "c2", depth=8, tuning=dict(n_dense=5, n_tierb=10, rows_per_group=2,
+                                                        unit_j=65536, blocks_per_cu=4))
+    o = oracle_shape(oracle, "c2", depth=8)
+    t0, _, _ = WL.c2_range(seed=0x77)
+    g, og, st = check_plain(ctx, oracle, "c2", t0, 300, shape=s, oshape=o)
+    assert len(g) > 100
+
+
+def test_config3_shape_counters(ctx, oracle):
+    s = WL.make_shape(ctx, "c3")
+    o = oracle_shape(oracle, "c3")
+    t0, _, _ = WL.c3_range(seed=0xC3)
+    check_plain(ctx, oracle, "c3", t0, 12, shape=s, oshape=o)
+
+
+# --- prove ------------------------------------------------------------------------
+
+PROVE_FIELDS = ("chain_len", "back_steps", "flags", "root_lo", "root_hi")
+
+
+def compare_prove(g, o):
+    assert len(g) == len(o)
+    for f, col in zip(("flags", "chain_len", "back_steps"), (0, 1, 2)):
+        assert np.array_equal(g[f].astype(np.uint64), o[:, col]), f
+    assert np.array_equal(g["root_lo"], o[:, 4])
+    assert np.array_equal(g["root_hi"], o[:, 5])
+
+
+def test_prove_reference_survivors(ctx, oracle, ref, oref):
+    t0, _ = WL.reference_range(seed=0x99, bits=89)
+    o_surv, _ = oref.filter(t0, 3000, cap=1 << 20)
+    surv = np.zeros(len(o_surv), ccs.SURVIVOR_DTYPE)
+    surv["base_idx"], surv["k_offset"], surv["tile_idx"] = o_surv[:, 0], o_surv[:, 1], o_surv[:, 2]
+    say(f"[prove] {len(surv)} reference survivors, bits=89")
+    for bits in (89, 0):
+        g = ref.prove(t0, surv, bits=bits)
+        o = oref.prove(t0, o_surv, bits=bits)
+        compare_prove(g, o)
+        assert np.any(g["flags"] & ccs.F_PRIME)
+
+
+def test_prove_mr_base2_config1(ctx, oracle):
+    s = WL.make_shape(ctx, "c1")
+    o = oracle_shape(oracle, "c1")
+    t0, tc, _ = WL.c1_range(seed=0x1C, t_count=1 << 10)
+    o_surv, _ = o.filter(t0, tc, cap=1 << 20)
+    surv = np.zeros(len(o_surv), ccs.SURVIVOR_DTYPE)
+    surv["base_idx"], surv["k_offset"], surv["tile_idx"] = o_surv[:, 0], o_surv[:, 1], o_surv[:, 2]
+    say(f"[prove] {len(surv)} config-1 survivors, MR base 2")
+    g = s.prove(t0, surv, bits=60, test="mr", bases=(2,))
+    compare_prove(g, o.prove(t0, o_surv, bits=60, kind=1, bases=(2,)))
+
+
+def known_roots():
+    import json
+    import os
+    from conftest import GOLDEN
+    vec = json.load(open(os.path.join(GOLDEN, "reference_vectors.json")))
+    roots = [(int(h, 16), L) for h, L in vec["v16_qvectors"]]
+    roots += [(int(r), L) for r, L in vec["classic_chains"] if int(r) > 1000]
+    roots.append((int(vec["cc13_hex"][0], 16), 13))
+    with open(os.path.join(GOLDEN, "cc10plus_roots_sample_1000.txt")) as f:
+        for i, line in enumerate(f):
+            if i % 10 == 0:
+                tag, hx, _ = line.split()
+                roots.append((int(hx, 16), int(tag[2:])))
+    return roots
+
+
+def test_prove_known_chains_as_jobs(ctx, oracle):
+    # class-form prove of arbitrary n: job (a = n mod W, u0 = n div W, count 1)
+    s = WL.make_shape(ctx, "c4", classes=False)
+    W = s.W
+    roots = known_roots()
+    root17 = 2759832934171386593519
+    members = [((root17 + 1) * 32 - 1, 17)]   # chain position 5 of the CC17: walks back
+    cases = roots + members
+    jobs = [(n % W, n // W, 1) for n, _ in cases]
+    surv = np.zeros(len(jobs), ccs.JOB_SURVIVOR_DTYPE)
+    surv["job_idx"] = np.arange(len(jobs))
+    say(f"[prove] {len(jobs)} known chains")
+    g = s.prove_jobs(jobs, surv)
+    for (n, L), r in zip(roots, g[: len(roots)]):
+        assert r["chain_len"] == L and r["back_steps"] == 0, hex(n)
+        assert (int(r["root_hi"]) << 64 | int(r["root_lo"])) == n
+    m = g[len(roots)]
+    assert m["chain_len"] == 17 and m["back_steps"] == 5 and m["flags"] & ccs.F_NONROOT
+    assert (int(m["root_hi"]) << 64 | int(m["root_lo"])) == root17
+    # the same numbers through the oracle
+    for (n, _), r in zip(cases, g):
+        f, root, ln, back = oracle.prove_one(n)
+        assert (int(r["flags"]), int(r["chain_len"]), int(r["back_steps"])) == (f, ln, back)
+
+
+def test_prove_random_large_values_vs_oracle(ctx, oracle):
+    # random odd values of 60..126 bits (primes among them), BPSW and MR-{2,3,5}
+    rng = np.random.default_rng(3)
+    s = WL.make_shape(ctx, "c4", classes=False)
+    W = s.W
+    vals = []
+    import sympy
+    for bits in (62, 64, 66, 90, 100, 120, 126):
+        for _ in range(30):
+            vals.append(int(rng.integers(1 << 62)) << (bits - 62) | 1 | (1 << (bits - 1)))
+        for _ in range(10):
+            vals.append(int(sympy.randprime(1 << (bits - 1), 1 << bits)))
+    jobs = [(n % W, n // W, 1) for n in vals]
+    surv = np.zeros(len(jobs), ccs.JOB_SURVIVOR_DTYPE)
+    surv["job_idx"] = np.arange(len(jobs))
+    for test, bases, kind in (("bpsw", (2,), 0), ("mr", (2, 3, 5), 1)):
+        say(f"[prove] random values, {test}")
+        g = s.prove_jobs(jobs, surv, test=test, bases=bases)
+        for n, r in zip(vals, g):
+            f, root, ln, back = oracle.prove_one(n, kind=kind, bases=bases)
+            assert (int(r["flags"]), int(r["chain_len"]), int(r["back_steps"])) == (f, ln, back), hex(n)
+            assert (int(r["root_hi"]) << 64 | int(r["root_lo"])) == root
+
+
+# --- batched form ------------------------------------------------------------------
+
+def test_batched_jobs_vs_oracle(ctx, oracle):
+    jobs, W = WL.batched_jobs(n_jobs=24, seed=0xB4, lo_exp=10, hi_exp=20)
+    s = WL.make_shape(ctx, "c4", classes=False, depth=10, target=18)
+    o = oracle_shape(oracle, "c4", depth=10, target=18)
+    assert s.W == W
+    say(f"[batch] {len(jobs)} jobs, {sum(j[2] for j in jobs)} candidates")
+    g, st = s.batch(jobs, cap=1 << 20)
+    want = []
+    alive = [0, 0, 0, 0]
+    for ji, (a, u0, count) in enumerate(jobs):
+        js, al = o.filter_class(a, W, u0, count, cuts=[st["dense_end"], st["tierb_end"], len(s.filter_primes)])
+        want += [(ji, int(x)) for x in js]
+        alive = [x + y for x, y in zip(alive, al)]
+    got = sorted(zip(g["job_idx"].tolist(), g["u_off"].tolist()))
+    assert st["candidates"] == alive[0]
+    assert (st["alive_dense"], st["alive_tierb"], st["survivors"]) == tuple(alive[1:])
+    assert got == sorted(want)
+    assert len(got) > 50
+
+
+# --- full-size properties (config 2, 2^36 candidates per launch) ---------------------
+
+def test_config2_full_launch_properties(ctx, oracle):
+    s = WL.make_shape(ctx, "c2")
+    t0, tc, _ = WL.c2_range()
+    say(f"[full] config 2 tiles={tc}")
+    g1, st1 = s.filter(t0, tc)
+    g2, st2 = s.filter(t0, tc)
+    assert st1["candidates"] == s.count_candidates(t0, tc) >= 1 << 36
+    # deterministic survivor set and counters
+    assert sorted_keys(g1) == sorted_keys(g2)
+    assert {k: st1[k] for k in ("alive_dense", "alive_tierb", "survivors")} == \
+        {k: st2[k] for k in ("alive_dense", "alive_tierb", "survivors")}
+    # every survivor satisfies the predicate (direct divisibility check on the CPU)
+    o = oracle_shape(oracle, "c2")
+    for rec in g1:
+        n = s.n_of(t0, rec)
+        assert o.first_kill_brute(n) == len(s.filter_primes)
+        assert o.on_lattice(n)
+    # survivor count against the analytic rate (position 0 excluded above 97)
+    p = 1.0
+    for q in s.filter_primes:
+        lo = 0 if q <= 97 else 1
+        p *= 1 - len({(pow(2, -i, q) - 1) % q for i in range(lo, 18)}) / q
+    exp = st1["candidates"] * p
+    assert abs(len(g1) - exp) < 6 * math.sqrt(exp) + 3
+    # the dense-stage survivors also follow the analytic rate
+    pd = 1.0
+    for q in s.filter_primes[: st1["dense_end"]]:
+        lo = 0 if q <= 97 else 1
+        pd *= 1 - len({(pow(2, -i, q) - 1) % q for i in range(lo, 18)}) / q
+    assert abs(st1["alive_dense"] / st1["candidates"] - pd) < 0.01 * pd
