@@ -35,25 +35,25 @@ def cpu_threads():
 
 
 def cpu_baseline(name, t0, seconds):
-    """The oracle's CPU restatement of the reference CPU path, all host threads, on a
-    bounded sample of the same workload (consecutive tiles from the same start)."""
+    """The oracle's CPU restatement of the reference CPU path (the v34-style 64-candidate
+    bit-vector filter, per-candidate for the larger primes), all host threads (<= 16), on a
+    bounded sample of the same workload: consecutive tiles from the same start."""
     from oracle import pyoracle
     from ccs import workloads as WL
     p = WL.shape_params(name)
     o = pyoracle.Shape(p["crt"], p["wheel"], p["target"], p["depth"], p["filter_primes"], p["pos0_max_q"])
     nt = cpu_threads()
-    tiles = 6 * nt * 4
+    tiles = 6 * nt * 50
+    o.filter_bv(t0, 6, cap=1 << 16, nthreads=1)        # one-time table build (startup work)
     t = time.perf_counter()
-    o.filter(t0, tiles, cap=1 << 20, nthreads=nt)
+    o.filter_bv(t0, tiles, cap=1 << 20, nthreads=nt)
     dt = time.perf_counter() - t
-    tiles = int(max(tiles, min(6 * 10 ** 6, tiles * seconds / max(dt, 1e-3))))
+    tiles = int(max(tiles, min(6 * 10 ** 7, tiles * seconds / max(dt, 1e-3))))
     tiles -= tiles % 6
     t = time.perf_counter()
-    surv, counts = o.filter(t0, tiles, cap=1 << 22, cuts=[], nthreads=nt)
+    surv, cand = o.filter_bv(t0, tiles, cap=1 << 22, nthreads=nt)
     dt = time.perf_counter() - t
-    cand = counts[0]
-    import platform
-    model = platform.processor() or "cpu"
+    model = "cpu"
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
@@ -62,8 +62,10 @@ def cpu_baseline(name, t0, seconds):
     except OSError:
         pass
     return {"value": cand / dt, "unit": "candidates/s", "cores": nt, "kind": "port",
-            "sample": f"{tiles} tiles = {cand} candidates of the same shape and start, "
-                      f"{dt:.1f} s on {nt} threads ({model}); v33-structured per-candidate filter, sieve only"}
+            "sample": f"{tiles} consecutive tiles = {cand} candidates of the same shape and start, "
+                      f"{dt:.1f} s on {nt} threads ({model}); v34-style bit-vector filter restated "
+                      f"in C, sieve only (on the reference shape it runs at 2.86 G/s on 8 Xeon "
+                      f"threads where the reference v34 binary ran at 4.07 G/s)"}
 
 
 def main():

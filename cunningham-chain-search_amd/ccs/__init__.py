@@ -14,7 +14,7 @@ from . import _lib as L
 from ._lib import (CHAIN_RESULT_DTYPE, JOB_SURVIVOR_DTYPE, SURVIVOR_DTYPE, CcsError,  # noqa: F401
                    F_BITS_OK, F_NONROOT, F_OVERFLOW, F_PRIME, F_TRIAL_OK, F_UNSUPPORTED)
 
-__all__ = ["Context", "Shape", "CcsError", "device_count", "format_results", "u128_split"]
+__all__ = ["Context", "Shape", "JobSet", "Server", "CcsError", "device_count", "format_results", "u128_split"]
 
 
 def u128_split(v):
@@ -137,6 +137,15 @@ class Shape:
         L.check(self.lib.ccs_shape_set_tuning(self.h, C.byref(t)), "ccs_shape_set_tuning")
         self._refresh_info()
 
+    def autotune(self, tile_start, tile_count, max_trials=40):
+        """Pick the fastest tuning on a sample range (ccs_autotune); returns (tuning, ms)."""
+        t = L.Tuning()
+        ms = C.c_float(0)
+        L.check(self.lib.ccs_autotune(self.ctx.h, self.h, L.U128.of(tile_start), tile_count, max_trials,
+                                      C.byref(t), C.byref(ms)), "ccs_autotune")
+        self._refresh_info()
+        return t.as_dict(), float(ms.value)
+
     def bases(self):
         out = np.zeros(self.n_bases, np.uint64)
         L.check(self.lib.ccs_shape_bases(self.h, out.ctypes.data, self.n_bases), "bases")
@@ -214,6 +223,119 @@ class Shape:
         """n of a plain survivor record (v13.cu:1819-1826)."""
         return ((tile_start + int(s["tile_idx"])) * self.WP + int(s["k_offset"])) * self.M + \
             int(self.bases()[int(s["base_idx"])])
+
+
+class JobSet:
+    """Class-form jobs uploaded once (ccs_jobset): batched launches, the per-job loop, graphs."""
+
+    def __init__(self, shape, jobs):
+        self.shape, self.ctx, self.lib = shape, shape.ctx, shape.lib
+        self.jobs = jobs_array(jobs)
+        self.h = C.c_void_p()
+        L.check(self.lib.ccs_jobset_create(self.ctx.h, shape.h, self.jobs.ctypes.data, len(self.jobs),
+                                           C.byref(self.h)), "ccs_jobset_create")
+        self.candidates = int(self.jobs["count"].sum())
+        self._graphs = []
+
+    def close(self):
+        for g in self._graphs:
+            self.lib.ccs_graph_free(g)
+        self._graphs = []
+        if self.h:
+            self.lib.ccs_jobset_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def launch(self, d_out, cap, d_count, stream=None):
+        L.check(self.lib.ccs_jobset_launch(self.ctx.h, self.h, d_out, cap, d_count, stream), "jobset_launch")
+
+    def launch_each(self, d_out, cap, d_count, stream=None):
+        L.check(self.lib.ccs_jobset_launch_each(self.ctx.h, self.h, d_out, cap, d_count, stream),
+                "jobset_launch_each")
+
+    def stats(self, d_count):
+        st = L.FilterStats()
+        L.check(self.lib.ccs_jobset_stats(self.ctx.h, self.h, d_count, C.byref(st)), "jobset_stats")
+        return st.as_dict()
+
+    def graph(self, d_out, cap, d_count):
+        g = C.c_void_p()
+        L.check(self.lib.ccs_graph_create(self.ctx.h, self.h, d_out, cap, d_count, C.byref(g)), "graph_create")
+        self._graphs.append(g)
+        return g
+
+    def graph_launch(self, g, stream=None):
+        L.check(self.lib.ccs_graph_launch(self.ctx.h, g, stream), "graph_launch")
+
+    def prove_launch(self, d_surv, d_count, cap, d_out, bits=0, test="bpsw", bases=(2,), stream=None):
+        pp = _prove_params(bits, test, bases)
+        L.check(self.lib.ccs_prove_jobset_launch(self.ctx.h, self.h, d_surv, d_count, cap, C.byref(pp),
+                                                 d_out, stream), "prove_jobset_launch")
+
+
+class Server:
+    """Continuous-batching request loop (ccs_server): submit jobs of any depth / width /
+    length, poll completions with their timestamps, fetch per-job results."""
+
+    def __init__(self, ctx, shapes_by_depth, mem_budget_bytes=256 << 20, max_active=1024, surv_per_job=16,
+                 max_slice_survivors=1 << 16, slice_candidates=1 << 31, min_chunk=1 << 18,
+                 max_chunk=1 << 32, bits=0, test="bpsw", bases=(2,)):
+        self.ctx, self.lib = ctx, ctx.lib
+        self.depths = sorted(shapes_by_depth)
+        self.shapes = [shapes_by_depth[d] for d in self.depths]
+        arr = (C.c_void_p * len(self.shapes))(*[sh.h.value for sh in self.shapes])
+        dep = (C.c_int32 * len(self.depths))(*self.depths)
+        p = L.ServerParams(mem_budget_bytes, max_active, surv_per_job, max_slice_survivors, 0,
+                           slice_candidates, min_chunk, max_chunk, _prove_params(bits, test, bases))
+        self.h = C.c_void_p()
+        L.check(self.lib.ccs_server_create(ctx.h, arr, dep, len(self.shapes), C.byref(p), C.byref(self.h)),
+                "ccs_server_create")
+
+    def submit(self, depth, a, u0, count, tag=0, bits=0):
+        r = L.Request(tag, depth, bits, a, L.U128.of(u0), count)
+        jid = C.c_uint64(0)
+        L.check(self.lib.ccs_server_submit(self.h, C.byref(r), C.byref(jid)), "ccs_server_submit")
+        return jid.value
+
+    def poll(self, cap=4096):
+        buf = (L.JobDone * cap)()
+        n = C.c_uint32(0)
+        L.check(self.lib.ccs_server_poll(self.h, buf, cap, C.byref(n)), "ccs_server_poll")
+        return [buf[i].as_dict() for i in range(n.value)]
+
+    def results(self, job_id):
+        n = C.c_uint32(0)
+        L.check(self.lib.ccs_server_job_results(self.h, job_id, None, None, 0, C.byref(n)), "results")
+        surv = np.zeros(n.value, JOB_SURVIVOR_DTYPE)
+        res = np.zeros(n.value, CHAIN_RESULT_DTYPE)
+        L.check(self.lib.ccs_server_job_results(self.h, job_id, surv.ctypes.data, res.ctypes.data, n.value,
+                                                C.byref(n)), "results")
+        return surv, res
+
+    def drain(self):
+        L.check(self.lib.ccs_server_drain(self.h), "ccs_server_drain")
+
+    def stats(self):
+        c = L.ServerCounters()
+        L.check(self.lib.ccs_server_stats(self.h, C.byref(c)), "ccs_server_stats")
+        return c.as_dict()
+
+    def shutdown(self):
+        if self.h:
+            rc = self.lib.ccs_server_shutdown(self.h)
+            self.h = C.c_void_p()
+            L.check(rc, "ccs_server_shutdown")
+
+    def __del__(self):
+        try:
+            self.shutdown()
+        except Exception:
+            pass
 
 
 def jobs_array(jobs):

@@ -24,6 +24,11 @@ EXPORTS = [
     "ccs_count_candidates", "ccs_filter_launch", "ccs_filter", "ccs_last_stats",
     "ccs_prove_launch", "ccs_prove", "ccs_format_results", "ccs_batch_launch", "ccs_batch",
     "ccs_prove_jobs", "ccs_profile_start", "ccs_profile_stop",
+    "ccs_jobset_create", "ccs_jobset_free", "ccs_jobset_launch", "ccs_jobset_launch_each",
+    "ccs_jobset_stats", "ccs_graph_create", "ccs_graph_launch", "ccs_graph_free",
+    "ccs_prove_jobset_launch", "ccs_autotune", "ccs_jobset_update",
+    "ccs_server_create", "ccs_server_submit", "ccs_server_poll", "ccs_server_job_results",
+    "ccs_server_drain", "ccs_server_stats", "ccs_server_shutdown",
 ]
 
 
@@ -91,6 +96,34 @@ class KernelTimes(C.Structure):
                 for f, t in self._fields_}
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("mem_budget_bytes", C.c_uint64), ("max_active", C.c_uint32), ("surv_per_job", C.c_uint32),
+                ("max_slice_survivors", C.c_uint32), ("pad", C.c_uint32), ("slice_candidates", C.c_uint64),
+                ("min_chunk", C.c_uint64), ("max_chunk", C.c_uint64), ("prove", ProveParams)]
+
+
+class Request(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("depth", C.c_int32), ("bits", C.c_int32), ("a", C.c_uint64),
+                ("u0", U128), ("count", C.c_uint64)]
+
+
+class JobDone(C.Structure):
+    _fields_ = [("job_id", C.c_uint64), ("tag", C.c_uint64), ("candidates", C.c_uint64),
+                ("n_survivors", C.c_uint32), ("n_primes", C.c_uint32), ("best_len", C.c_uint32),
+                ("pad", C.c_uint32), ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+
+
+class ServerCounters(C.Structure):
+    _fields_ = [("candidates", C.c_uint64), ("slices", C.c_uint64), ("launches", C.c_uint64),
+                ("resident_bytes", C.c_uint64), ("active", C.c_uint32), ("pending", C.c_uint32)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
 class Job(C.Structure):
     _fields_ = [("a", C.c_uint64), ("u0", U128), ("count", C.c_uint64)]
 
@@ -150,6 +183,26 @@ def load():
     L.ccs_batch.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_uint32, C.POINTER(C.c_uint32),
                             C.POINTER(FilterStats)]
     L.ccs_prove_jobs.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_uint32, C.POINTER(ProveParams), vp]
+    L.ccs_jobset_create.argtypes = [vp, vp, vp, C.c_uint32, C.POINTER(vp)]
+    L.ccs_jobset_free.argtypes = [vp]
+    L.ccs_jobset_free.restype = None
+    L.ccs_jobset_launch.argtypes = [vp, vp, vp, C.c_uint32, vp, vp]
+    L.ccs_jobset_launch_each.argtypes = [vp, vp, vp, C.c_uint32, vp, vp]
+    L.ccs_jobset_stats.argtypes = [vp, vp, vp, C.POINTER(FilterStats)]
+    L.ccs_graph_create.argtypes = [vp, vp, vp, C.c_uint32, vp, C.POINTER(vp)]
+    L.ccs_graph_launch.argtypes = [vp, vp, vp]
+    L.ccs_graph_free.argtypes = [vp]
+    L.ccs_graph_free.restype = None
+    L.ccs_prove_jobset_launch.argtypes = [vp, vp, vp, vp, C.c_uint32, C.POINTER(ProveParams), vp, vp]
+    L.ccs_autotune.argtypes = [vp, vp, U128, C.c_uint32, C.c_uint32, C.POINTER(Tuning), C.POINTER(C.c_float)]
+    L.ccs_jobset_update.argtypes = [vp, vp, vp, C.c_uint32, vp]
+    L.ccs_server_create.argtypes = [vp, vp, vp, C.c_uint32, C.POINTER(ServerParams), C.POINTER(vp)]
+    L.ccs_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
+    L.ccs_server_poll.argtypes = [vp, vp, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.ccs_server_job_results.argtypes = [vp, C.c_uint64, vp, vp, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.ccs_server_drain.argtypes = [vp]
+    L.ccs_server_stats.argtypes = [vp, C.POINTER(ServerCounters)]
+    L.ccs_server_shutdown.argtypes = [vp]
     L.ccs_profile_start.argtypes = [vp]
     L.ccs_profile_stop.argtypes = [vp, C.POINTER(KernelTimes)]
     _lib = L

@@ -634,7 +634,17 @@ static int ensure_mid(ccs_ctx* c, u64 want) {
     return CCS_OK;
 }
 
-static void fill_sieve_common(const ccs_shape* s, ccs_ctx* c, SieveArgs& a) {
+// stage-B buffer, counters ([0] stage-B count, [2..3] unit cursor) and statistics of a launch
+struct Scratch {
+    MidRec* mid;
+    u32 mid_cap;
+    u32* counters;
+    unsigned long long* stats;
+};
+
+static Scratch ctx_scratch(ccs_ctx* c) { return Scratch{c->mid, c->mid_cap, c->counters, c->stats}; }
+
+static void fill_sieve_common(const ccs_shape* s, const Scratch& x, SieveArgs& a) {
     memset(&a, 0, sizeof(a));
     a.W = s->W;
     a.lds_image = s->d_lds_image;
@@ -645,19 +655,22 @@ static void fill_sieve_common(const ccs_shape* s, ccs_ctx* c, SieveArgs& a) {
     a.dense = s->d_dense;
     a.tierb = s->d_tierb;
     a.pmod = s->d_pmod;
-    a.unit_counter = (u64*)(c->counters + 2);
-    a.mid = c->mid;
-    a.mid_cap = c->mid_cap;
-    a.mid_count = c->counters + 0;
-    a.stats = c->stats;
+    a.unit_counter = (u64*)(x.counters + 2);
+    a.mid = x.mid;
+    a.mid_cap = x.mid_cap;
+    a.mid_count = x.counters + 0;
+    a.stats = x.stats;
 }
 
-static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, SieveArgs& sa, DeepArgs& da, u64 n_units,
-                              u32* d_count, hipStream_t st) {
-    // zero: mid_count, unit counter, stats, caller's count (fill kernels, graph-safe)
-    if (launch_zero(c->counters, 16, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
-    if (launch_zero((u32*)c->stats, 16, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
-    if (launch_zero(d_count, 1, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
+// zero the launch counters, then sieve + deep.  zero_out also clears the caller's count
+// (the per-job loop appends several launches into one output).
+static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, SieveArgs& sa, DeepArgs& da,
+                              u64 n_units, u32* d_count, hipStream_t st, bool zero_out = true,
+                              bool zero_stats = true) {
+    // fill kernels rather than memset nodes, so the sequence can be graph-captured
+    if (launch_zero(x.counters, 16, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
+    if (zero_stats && launch_zero((u32*)x.stats, 16, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
+    if (zero_out && launch_zero(d_count, 1, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
     sa.n_units = n_units;
     const int nbv = s->nb ? (int)((s->nb + 63) / 64) : 1;
     const int block = 256;
@@ -671,12 +684,12 @@ static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, SieveArgs& sa, Dee
     if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("sieve launch: ") + hipGetErrorString(e));
     prof_end(c, st);
     if (c->timed) HIPCHK(hipEventRecord(c->ev[1], st));
-    da.mid = c->mid;
-    da.mid_count = c->counters + 0;
-    da.mid_cap = c->mid_cap;
+    da.mid = x.mid;
+    da.mid_count = x.counters + 0;
+    da.mid_cap = x.mid_cap;
     da.out_count = d_count;
     prof_begin(c, 1, st);
-    e = launch_deep(da, c->mid_cap, st);
+    e = launch_deep(da, x.mid_cap, st);
     if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("deep launch: ") + hipGetErrorString(e));
     prof_end(c, st);
     if (c->timed) HIPCHK(hipEventRecord(c->ev[2], st));
@@ -696,8 +709,9 @@ static int filter_launch_impl(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint3
     const u64 cand = ccs_count_candidates(s, ts, tile_count);
     int rc = ensure_mid(c, (u64)(cand * s->est_survival_b * 2.0) + (1u << 20));
     if (rc) return rc;
+    const Scratch x = ctx_scratch(c);
     SieveArgs sa;
-    fill_sieve_common(s, c, sa);
+    fill_sieve_common(s, x, sa);
     sa.mode = 0;
     sa.plain.cls = s->d_cls;
     sa.plain.nclasses = s->classes.size();
@@ -720,7 +734,7 @@ static int filter_launch_impl(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint3
     c->last.candidates = cand;
     c->last.dense_end = s->nd;
     c->last.tierb_end = s->nd + s->nb;
-    return run_filter_kernels(c, s, sa, da, n_units, d_count, st);
+    return run_filter_kernels(c, s, x, sa, da, n_units, d_count, st);
 }
 
 extern "C" int ccs_filter_launch(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint32_t tile_count,
@@ -742,18 +756,18 @@ static int ensure_buf(void** p, u64* have, u64 want) {
     return CCS_OK;
 }
 
-static int collect_stats(ccs_ctx* c, u32 count) {
+static int collect_stats(ccs_ctx* c, u32 count, const Scratch& x) {
     unsigned long long st[2] = {0, 0};
     u32 midc = 0;
-    HIPCHK(hipMemcpy(st, c->stats, sizeof(st), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(&midc, c->counters, 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(st, x.stats, sizeof(st), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&midc, x.counters, 4, hipMemcpyDeviceToHost));
     c->last.alive_dense = st[0];
     c->last.alive_tierb = midc;
     c->last.survivors = count;
     float ms = 0;
     if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->last.ms_sieve = ms;
     if (hipEventElapsedTime(&ms, c->ev[1], c->ev[2]) == hipSuccess) c->last.ms_deep = ms;
-    if (midc > c->mid_cap) return fail(CCS_ERR_NOMEM, "stage-B survivor buffer overflow");
+    if (midc > x.mid_cap) return fail(CCS_ERR_NOMEM, "stage-B survivor buffer overflow");
     return CCS_OK;
 }
 
@@ -775,7 +789,7 @@ extern "C" int ccs_filter(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint32_t 
     const u32 keep = std::min(n, cap);
     if (keep && h_out) HIPCHK(hipMemcpy(h_out, c->out, (size_t)keep * sizeof(ccs_survivor), hipMemcpyDeviceToHost));
     *h_count = n;
-    rc = collect_stats(c, n);
+    rc = collect_stats(c, n, ctx_scratch(c));
     if (stats) *stats = c->last;
     if (rc) return rc;
     return n > cap ? CCS_OVERFLOW : CCS_OK;
@@ -920,60 +934,120 @@ extern "C" int ccs_format_results(const ccs_chain_result* res, uint32_t n, int32
 }
 
 // ---------------------------------------------------------------------------
-// batched form
+// batched form: job sets (uploaded once), one persistent launch, per-job loop, graphs
 // ---------------------------------------------------------------------------
 
-static int upload_jobs(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, u32 n_jobs, hipStream_t st,
-                       u64* n_units_out, u64* cand_out) {
-    if (!jobs || n_jobs == 0) return fail(CCS_ERR_INVALID, "no jobs");
-    if (c->jobs_cap < n_jobs) {
-        (void)hipFree(c->jobs);
-        (void)hipFree(c->unit_prefix);
-        c->jobs = nullptr;
-        c->unit_prefix = nullptr;
-        c->jobs_cap = 0;
-        HIPCHK(hipMalloc((void**)&c->jobs, (size_t)n_jobs * sizeof(JobRec)));
-        HIPCHK(hipMalloc((void**)&c->unit_prefix, ((size_t)n_jobs + 1) * 8));
-        c->jobs_cap = n_jobs;
-    }
+struct ccs_jobset {
+    int device = 0;
+    const ccs_shape* shape = nullptr;
+    u32 n_jobs = 0;
+    JobRec* d_jobs = nullptr;
+    u64* d_prefix = nullptr;
+    std::vector<u64> prefix;
+    u64 n_units = 0, candidates = 0;
+    u32 cap_jobs = 0;
+    std::vector<JobRec> staging;
+    Scratch x = {nullptr, 0, nullptr, nullptr};
+};
+
+extern "C" void ccs_jobset_free(ccs_jobset* js) {
+    if (!js) return;
+    (void)hipSetDevice(js->device);
+    (void)hipFree(js->d_jobs);
+    (void)hipFree(js->d_prefix);
+    (void)hipFree(js->x.mid);
+    (void)hipFree(js->x.counters);
+    (void)hipFree(js->x.stats);
+    delete js;
+}
+
+extern "C" int ccs_jobset_create(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, uint32_t n_jobs,
+                                 ccs_jobset** out) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!s || !jobs || !out || n_jobs == 0) return fail(CCS_ERR_INVALID, "null argument or no jobs");
+    *out = nullptr;
     std::vector<JobRec> jr(n_jobs);
-    std::vector<u64> pre(n_jobs + 1);
-    u64 units = 0, cand = 0;
+    ccs_jobset* js = new ccs_jobset();
+    js->device = c->device;
+    js->shape = s;
+    js->n_jobs = n_jobs;
+    js->prefix.resize((size_t)n_jobs + 1);
     for (u32 i = 0; i < n_jobs; ++i) {
-        if (jobs[i].a >= s->W) return fail(CCS_ERR_INVALID, "job residue a must be < W");
-        if (jobs[i].count > (1ull << 32)) return fail(CCS_ERR_INVALID, "job count must be <= 2^32");
+        if (jobs[i].a >= s->W) { delete js; return fail(CCS_ERR_INVALID, "job residue a must be < W"); }
+        if (jobs[i].count > (1ull << 32)) { delete js; return fail(CCS_ERR_INVALID, "job count must be <= 2^32"); }
         jr[i].a = jobs[i].a;
         jr[i].u0_lo = jobs[i].u0.lo;
         jr[i].u0_hi = jobs[i].u0.hi;
         jr[i].count = jobs[i].count;
-        pre[i] = units;
-        units += (jobs[i].count + s->tune.unit_j - 1) / s->tune.unit_j;
-        cand += jobs[i].count;
+        js->prefix[i] = js->n_units;
+        js->n_units += (jobs[i].count + s->tune.unit_j - 1) / s->tune.unit_j;
+        js->candidates += jobs[i].count;
     }
-    pre[n_jobs] = units;
-    HIPCHK(hipMemcpyAsync(c->jobs, jr.data(), jr.size() * sizeof(JobRec), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(c->unit_prefix, pre.data(), pre.size() * 8, hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
-    *n_units_out = units;
-    *cand_out = cand;
+    js->prefix[n_jobs] = js->n_units;
+    js->cap_jobs = n_jobs;
+    u64 want = (u64)(js->candidates * s->est_survival_b * 2.0) + (1u << 20);
+    if (want > (1u << 28)) want = 1u << 28;
+    hipError_t e = hipMalloc((void**)&js->d_jobs, (size_t)n_jobs * sizeof(JobRec));
+    if (e == hipSuccess) e = hipMalloc((void**)&js->d_prefix, ((size_t)n_jobs + 1) * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&js->x.mid, want * sizeof(MidRec));
+    if (e == hipSuccess) e = hipMalloc((void**)&js->x.counters, 64);
+    if (e == hipSuccess) e = hipMalloc((void**)&js->x.stats, 64);
+    if (e == hipSuccess) e = hipMemcpy(js->d_jobs, jr.data(), jr.size() * sizeof(JobRec), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(js->d_prefix, js->prefix.data(), js->prefix.size() * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        ccs_jobset_free(js);
+        return fail(CCS_ERR_HIP, std::string("jobset upload: ") + hipGetErrorString(e));
+    }
+    js->x.mid_cap = (u32)want;
+    *out = js;
     return CCS_OK;
 }
 
-static int batch_launch_impl(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, uint32_t n_jobs,
-                             ccs_job_survivor* d_out, uint32_t cap, uint32_t* d_count, hipStream_t st) {
-    if (!s || !d_out || !d_count) return fail(CCS_ERR_INVALID, "null argument");
-    u64 n_units = 0, cand = 0;
-    int rc = upload_jobs(c, s, jobs, n_jobs, st, &n_units, &cand);
+// replace the jobs of a set (n_jobs <= the count it was created with; same shape) with an
+// asynchronous upload on `stream`; h_jobs must stay valid until the stream reaches it
+extern "C" int ccs_jobset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jobs, uint32_t n_jobs,
+                                 void* stream) {
+    int rc = ctx_check(c);
     if (rc) return rc;
-    rc = ensure_mid(c, (u64)(cand * s->est_survival_b * 2.0) + (1u << 20));
-    if (rc) return rc;
-    SieveArgs sa;
-    fill_sieve_common(s, c, sa);
+    if (!js || !h_jobs || n_jobs == 0) return fail(CCS_ERR_INVALID, "null argument or no jobs");
+    if (n_jobs > js->cap_jobs) return fail(CCS_ERR_INVALID, "more jobs than the set was created with");
+    const ccs_shape* s = js->shape;
+    js->staging.resize(n_jobs);
+    js->prefix.resize((size_t)n_jobs + 1);
+    js->n_units = 0;
+    js->candidates = 0;
+    for (u32 i = 0; i < n_jobs; ++i) {
+        if (h_jobs[i].a >= s->W) return fail(CCS_ERR_INVALID, "job residue a must be < W");
+        if (h_jobs[i].count > (1ull << 32)) return fail(CCS_ERR_INVALID, "job count must be <= 2^32");
+        JobRec& r = js->staging[i];
+        r.a = h_jobs[i].a;
+        r.u0_lo = h_jobs[i].u0.lo;
+        r.u0_hi = h_jobs[i].u0.hi;
+        r.count = h_jobs[i].count;
+        js->prefix[i] = js->n_units;
+        js->n_units += (h_jobs[i].count + s->tune.unit_j - 1) / s->tune.unit_j;
+        js->candidates += h_jobs[i].count;
+    }
+    js->prefix[n_jobs] = js->n_units;
+    if ((u64)(js->candidates * s->est_survival_b * 2.0) + (1u << 16) > js->x.mid_cap)
+        return fail(CCS_ERR_INVALID, "jobs exceed the stage-B buffer of the set");
+    js->n_jobs = n_jobs;
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    HIPCHK(hipMemcpyAsync(js->d_jobs, js->staging.data(), (size_t)n_jobs * sizeof(JobRec), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(js->d_prefix, js->prefix.data(), ((size_t)n_jobs + 1) * 8, hipMemcpyHostToDevice, st));
+    // the staging vectors must outlive the copies: synchronise before returning
+    HIPCHK(hipStreamSynchronize(st));
+    return CCS_OK;
+}
+
+static void jobset_args(const ccs_jobset* js, SieveArgs& sa, DeepArgs& da, void* d_out, u32 cap) {
+    const ccs_shape* s = js->shape;
+    fill_sieve_common(s, js->x, sa);
     sa.mode = 1;
-    sa.jobsrc.jobs = c->jobs;
-    sa.jobsrc.unit_prefix = c->unit_prefix;
-    sa.jobsrc.n_jobs = n_jobs;
-    DeepArgs da;
+    sa.jobsrc.jobs = js->d_jobs;
+    sa.jobsrc.unit_prefix = js->d_prefix;
+    sa.jobsrc.n_jobs = js->n_jobs;
     memset(&da, 0, sizeof(da));
     da.W = s->W;
     da.deep = s->d_deep;
@@ -983,19 +1057,141 @@ static int batch_launch_impl(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs
     da.jobsrc = sa.jobsrc;
     da.out = d_out;
     da.out_cap = cap;
-    c->last = ccs_filter_stats();
-    c->last.candidates = cand;
-    c->last.dense_end = s->nd;
-    c->last.tierb_end = s->nd + s->nb;
-    return run_filter_kernels(c, s, sa, da, n_units, d_count, st);
 }
 
-extern "C" int ccs_batch_launch(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, uint32_t n_jobs,
-                                ccs_job_survivor* d_out, uint32_t cap, uint32_t* d_count, void* stream) {
+static void jobset_stats_begin(ccs_ctx* c, const ccs_jobset* js) {
+    c->last = ccs_filter_stats();
+    c->last.candidates = js->candidates;
+    c->last.dense_end = js->shape->nd;
+    c->last.tierb_end = js->shape->nd + js->shape->nb;
+}
+
+// one persistent, work-stealing launch over every job of the set
+extern "C" int ccs_jobset_launch(ccs_ctx* c, ccs_jobset* js, ccs_job_survivor* d_out, uint32_t cap,
+                                 uint32_t* d_count, void* stream) {
     int rc = ctx_check(c);
     if (rc) return rc;
+    if (!js || !d_out || !d_count) return fail(CCS_ERR_INVALID, "null argument");
+    SieveArgs sa;
+    DeepArgs da;
+    jobset_args(js, sa, da, d_out, cap);
+    jobset_stats_begin(c, js);
+    return run_filter_kernels(c, js->shape, js->x, sa, da, js->n_units, d_count,
+                              stream ? (hipStream_t)stream : c->stream);
+}
+
+// the plain per-shape loop: one launch per job (the v16 dispatch pattern, v16...cu:5611-5965),
+// appending into the same output
+extern "C" int ccs_jobset_launch_each(ccs_ctx* c, ccs_jobset* js, ccs_job_survivor* d_out, uint32_t cap,
+                                      uint32_t* d_count, void* stream) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!js || !d_out || !d_count) return fail(CCS_ERR_INVALID, "null argument");
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    jobset_stats_begin(c, js);
+    if (launch_zero(d_count, 1, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
+    if (launch_zero((u32*)js->x.stats, 16, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
+    for (u32 i = 0; i < js->n_jobs; ++i) {
+        const u64 units = js->prefix[i + 1] - js->prefix[i];
+        if (!units) continue;
+        SieveArgs sa;
+        DeepArgs da;
+        jobset_args(js, sa, da, d_out, cap);
+        sa.unit_base = js->prefix[i];
+        rc = run_filter_kernels(c, js->shape, js->x, sa, da, units, d_count, st, false, false);
+        if (rc) return rc;
+    }
+    return CCS_OK;
+}
+
+extern "C" int ccs_jobset_stats(ccs_ctx* c, ccs_jobset* js, uint32_t* d_count, ccs_filter_stats* out) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!js || !out) return fail(CCS_ERR_INVALID, "null");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    u32 n = 0;
+    if (d_count) HIPCHK(hipMemcpy(&n, d_count, 4, hipMemcpyDeviceToHost));
+    jobset_stats_begin(c, js);
+    rc = collect_stats(c, n, js->x);
+    *out = c->last;
+    return rc;
+}
+
+struct ccs_graph {
+    int device = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" void ccs_graph_free(ccs_graph* g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+}
+
+// capture the persistent batched launch (counter fills, sieve, deep) as a HIP graph
+extern "C" int ccs_graph_create(ccs_ctx* c, ccs_jobset* js, ccs_job_survivor* d_out, uint32_t cap,
+                                uint32_t* d_count, ccs_graph** out) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!js || !d_out || !d_count || !out) return fail(CCS_ERR_INVALID, "null argument");
+    *out = nullptr;
+    ccs_graph* g = new ccs_graph();
+    g->device = c->device;
+    const bool prof = c->profiling, timed = c->timed;
+    c->profiling = false;
     c->timed = false;
-    return batch_launch_impl(c, s, jobs, n_jobs, d_out, cap, d_count, stream ? (hipStream_t)stream : c->stream);
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        c->profiling = prof;
+        c->timed = timed;
+        delete g;
+        return fail(CCS_ERR_HIP, std::string("begin capture: ") + hipGetErrorString(e));
+    }
+    SieveArgs sa;
+    DeepArgs da;
+    jobset_args(js, sa, da, d_out, cap);
+    rc = run_filter_kernels(c, js->shape, js->x, sa, da, js->n_units, d_count, c->stream);
+    e = hipStreamEndCapture(c->stream, &g->graph);
+    c->profiling = prof;
+    c->timed = timed;
+    if (rc || e != hipSuccess) {
+        ccs_graph_free(g);
+        return rc ? rc : fail(CCS_ERR_HIP, std::string("end capture: ") + hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        ccs_graph_free(g);
+        return fail(CCS_ERR_HIP, std::string("instantiate: ") + hipGetErrorString(e));
+    }
+    *out = g;
+    return CCS_OK;
+}
+
+extern "C" int ccs_graph_launch(ccs_ctx* c, ccs_graph* g, void* stream) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!g) return fail(CCS_ERR_INVALID, "null graph");
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    prof_begin(c, 0, st);
+    HIPCHK(hipGraphLaunch(g->exec, st));
+    prof_end(c, st);
+    return CCS_OK;
+}
+
+// convenience forms: upload, launch, synchronise
+extern "C" int ccs_batch_launch(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, uint32_t n_jobs,
+                                ccs_job_survivor* d_out, uint32_t cap, uint32_t* d_count, void* stream) {
+    ccs_jobset* js = nullptr;
+    int rc = ccs_jobset_create(c, s, jobs, n_jobs, &js);
+    if (rc) return rc;
+    rc = ccs_jobset_launch(c, js, d_out, cap, d_count, stream);
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = fail(CCS_ERR_HIP, "synchronize");
+    ccs_jobset_free(js);
+    return rc;
 }
 
 extern "C" int ccs_batch(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, uint32_t n_jobs,
@@ -1006,18 +1202,23 @@ extern "C" int ccs_batch(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, ui
     rc = ensure_buf(&c->out, &c->out_bytes, (u64)(cap ? cap : 1) * sizeof(ccs_job_survivor) + 256);
     if (rc) return rc;
     u32* d_count = (u32*)((char*)c->out + (u64)(cap ? cap : 1) * sizeof(ccs_job_survivor));
-    c->timed = true;
-    rc = batch_launch_impl(c, s, jobs, n_jobs, (ccs_job_survivor*)c->out, cap, d_count, c->stream);
-    c->timed = false;
+    ccs_jobset* js = nullptr;
+    rc = ccs_jobset_create(c, s, jobs, n_jobs, &js);
     if (rc) return rc;
-    HIPCHK(hipStreamSynchronize(c->stream));
+    c->timed = true;
+    rc = ccs_jobset_launch(c, js, (ccs_job_survivor*)c->out, cap, d_count, c->stream);
+    c->timed = false;
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(CCS_ERR_HIP, "synchronize");
     u32 n = 0;
-    HIPCHK(hipMemcpy(&n, d_count, 4, hipMemcpyDeviceToHost));
+    if (!rc && hipMemcpy(&n, d_count, 4, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(CCS_ERR_HIP, "count");
     const u32 keep = std::min(n, cap);
-    if (keep && h_out) HIPCHK(hipMemcpy(h_out, c->out, (size_t)keep * sizeof(ccs_job_survivor), hipMemcpyDeviceToHost));
+    if (!rc && keep && h_out &&
+        hipMemcpy(h_out, c->out, (size_t)keep * sizeof(ccs_job_survivor), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(CCS_ERR_HIP, "copy");
     *h_count = n;
-    rc = collect_stats(c, n);
+    if (!rc) rc = collect_stats(c, n, js->x);
     if (stats) *stats = c->last;
+    ccs_jobset_free(js);
     if (rc) return rc;
     return n > cap ? CCS_OVERFLOW : CCS_OK;
 }
@@ -1030,15 +1231,41 @@ extern "C" int ccs_prove_jobs(ccs_ctx* c, const ccs_shape* s, const ccs_job* job
     if (!s || !jobs || (n && (!h_surv || !h_out))) return fail(CCS_ERR_INVALID, "null argument");
     for (u32 i = 0; i < n; ++i)
         if (h_surv[i].job_idx >= n_jobs) return fail(CCS_ERR_INVALID, "survivor job_idx out of range");
-    u64 units = 0, cand = 0;
-    rc = upload_jobs(c, s, jobs, n_jobs, c->stream, &units, &cand);
+    ccs_jobset* js = nullptr;
+    rc = ccs_jobset_create(c, s, jobs, n_jobs, &js);
     if (rc) return rc;
     ProveArgs a;
     rc = fill_prove(s, p, a);
+    if (!rc) {
+        a.mode = 1;
+        a.jobs = js->d_jobs;
+        rc = prove_host_common(c, a, h_surv, sizeof(ccs_job_survivor), n, h_out);
+    }
+    ccs_jobset_free(js);
+    return rc;
+}
+
+extern "C" int ccs_prove_jobset_launch(ccs_ctx* c, ccs_jobset* js, const ccs_job_survivor* d_surv,
+                                       const uint32_t* d_count, uint32_t cap, const ccs_prove_params* p,
+                                       ccs_chain_result* d_out, void* stream) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!js || !d_surv || !d_count || !d_out) return fail(CCS_ERR_INVALID, "null argument");
+    ProveArgs a;
+    rc = fill_prove(js->shape, p, a);
     if (rc) return rc;
     a.mode = 1;
-    a.jobs = c->jobs;
-    return prove_host_common(c, a, h_surv, sizeof(ccs_job_survivor), n, h_out);
+    a.jobs = js->d_jobs;
+    a.surv = d_surv;
+    a.surv_count = d_count;
+    a.surv_cap = cap;
+    a.out = d_out;
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    prof_begin(c, 2, st);
+    hipError_t e = launch_prove(a, cap, st);
+    if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("prove launch: ") + hipGetErrorString(e));
+    prof_end(c, st);
+    return CCS_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1071,5 +1298,105 @@ extern "C" int ccs_profile_stop(ccs_ctx* c, ccs_kernel_times* out) {
     c->profiling = false;
     c->marks.clear();
     c->ev_used = 0;
+    return CCS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: coordinate descent over the kernel knobs, each trial timed with HIP events
+// on a sample tile range of the shape (filter launch only: sieve + deep)
+// ---------------------------------------------------------------------------
+
+static int time_filter(ccs_ctx* c, ccs_shape* s, ccs_u128 ts, uint32_t tile_count, int reps, float* ms_out) {
+    int rc = ensure_buf(&c->out, &c->out_bytes, (u64)(1u << 20) * sizeof(ccs_survivor) + 256);
+    if (rc) return rc;
+    u32* d_count = (u32*)((char*)c->out + (u64)(1u << 20) * sizeof(ccs_survivor));
+    float best = 1e30f;
+    for (int r = 0; r < reps + 1; ++r) {
+        c->timed = true;
+        rc = filter_launch_impl(c, s, ts, tile_count, (ccs_survivor*)c->out, 1u << 20, d_count, c->stream);
+        c->timed = false;
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(c->stream));
+        float a = 0, b = 0;
+        HIPCHK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+        HIPCHK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+        if (r > 0 && a + b < best) best = a + b;   // first run warms up
+    }
+    *ms_out = best;
+    return CCS_OK;
+}
+
+extern "C" int ccs_autotune(ccs_ctx* c, ccs_shape* s, ccs_u128 ts, uint32_t tile_count, uint32_t max_trials,
+                            ccs_tuning* best_out, float* best_ms_out) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!s || s->classes.empty()) return fail(CCS_ERR_INVALID, "autotune needs a shape with classes");
+    ccs_tuning best = s->tune;
+    float best_ms = 0;
+    rc = time_filter(c, s, ts, tile_count, 2, &best_ms);
+    if (rc) return rc;
+    u32 trials = 1;
+    // candidate values per knob
+    std::vector<u32> dense_cut = {48, 64, 80, 100, 128, 160, 200, 256, 320};
+    auto count_below = [&](u32 lim) {
+        u32 n = 0;
+        while (n < s->fp.size() && n < (u32)kMaxDense && s->fp[n] < lim) ++n;
+        return n;
+    };
+    auto try_tuning = [&](ccs_tuning t) -> int {
+        if (trials >= max_trials) return CCS_OK;
+        u32 room = 0;   // stage B takes every prime below 4096 that fits (<= 256)
+        t.n_tierb = 0;
+        while (t.n_dense + t.n_tierb < s->fp.size() && t.n_tierb < 256 && s->fp[t.n_dense + t.n_tierb] < 4096)
+            ++t.n_tierb;
+        (void)room;
+        for (;;) {
+            ccs_tuning tt = t;
+            if (validate_tuning(s, &tt) == CCS_OK) break;
+            if (t.n_tierb == 0) return CCS_OK;   // does not fit in LDS: skip
+            t.n_tierb = t.n_tierb > 16 ? t.n_tierb - 16 : 0;
+        }
+        s->tune = t;
+        int r2 = upload_tables(s);
+        if (r2) return r2;
+        float ms = 0;
+        r2 = time_filter(c, s, ts, tile_count, 2, &ms);
+        if (r2) return r2;
+        ++trials;
+        if (ms < best_ms) { best_ms = ms; best = t; }
+        return CCS_OK;
+    };
+    for (int pass = 0; pass < 2; ++pass) {
+        ccs_tuning base = best;
+        for (u32 cut : dense_cut) {
+            ccs_tuning t = best;
+            t.n_dense = count_below(cut);
+            if (t.n_dense == base.n_dense) continue;
+            if ((rc = try_tuning(t))) return rc;
+        }
+        for (u32 rg : {1u, 2u, 4u}) {
+            ccs_tuning t = best;
+            if (rg == t.rows_per_group) continue;
+            t.rows_per_group = rg;
+            if ((rc = try_tuning(t))) return rc;
+        }
+        for (u32 uj : {16384u, 32768u, 65536u, 131072u, 262144u}) {
+            ccs_tuning t = best;
+            if (uj == t.unit_j) continue;
+            t.unit_j = uj;
+            if ((rc = try_tuning(t))) return rc;
+        }
+        for (u32 bpc : {1u, 2u, 3u, 4u, 6u, 8u}) {
+            ccs_tuning t = best;
+            if (bpc == t.blocks_per_cu) continue;
+            t.blocks_per_cu = bpc;
+            if ((rc = try_tuning(t))) return rc;
+        }
+    }
+    s->tune = best;
+    rc = upload_tables(s);
+    if (rc) return rc;
+    if (best_out) *best_out = best;
+    if (best_ms_out) *best_ms_out = best_ms;
     return CCS_OK;
 }

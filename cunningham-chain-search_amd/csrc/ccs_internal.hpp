@@ -106,6 +106,7 @@ struct SieveArgs {
     const PrimeMod* pmod;     // [nd + nb] residue constants for dense then tier-B primes
     // work
     u64 n_units;
+    u64 unit_base;            // added to the cursor value: first unit of this launch
     u64* unit_counter;        // work-stealing cursor (zeroed before launch)
     int mode;                 // 0 = plain, 1 = jobs
     PlainSrc plain;

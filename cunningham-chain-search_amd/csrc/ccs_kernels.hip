@@ -230,6 +230,7 @@ __global__ __launch_bounds__(256) void sieve_kernel(SieveArgs A) {
         if (lane == 0) unit = atomicAdd(A.unit_counter, 1ull);
         unit = bcast64(unit);
         if (unit >= A.n_units) break;
+        unit += A.unit_base;
         Unit U;
         if (!resolve(A, unit, lane, U)) continue;
         U.a = bcast64(U.a);

@@ -161,6 +161,11 @@ int ccs_shape_classes(const ccs_shape* shape, uint64_t* h_a, uint32_t* h_base_id
                       uint32_t* h_k, uint32_t* h_rho, uint64_t cap);
 int ccs_shape_set_tuning(ccs_shape* shape, const ccs_tuning* tuning);
 int ccs_shape_get_tuning(const ccs_shape* shape, ccs_tuning* out);
+/* Autotuner: coordinate descent over the dense-prime cutoff, rows per group, work-unit
+ * size and blocks per CU, each trial timed (HIP events, sieve + deep) on the sample tile
+ * range; leaves the fastest tuning set on the shape.  At most max_trials timed trials. */
+int ccs_autotune(ccs_ctx* ctx, ccs_shape* shape, ccs_u128 tile_start, uint32_t tile_count,
+                 uint32_t max_trials, ccs_tuning* best, float* best_ms);
 uint64_t ccs_count_candidates(const ccs_shape* shape, ccs_u128 tile_start, uint32_t tile_count);
 
 /* ---- plain filter: K1 over tiles [tile_start, tile_start + tile_count), all bases
@@ -203,6 +208,30 @@ int ccs_profile_stop(ccs_ctx* ctx, ccs_kernel_times* out);   /* synchronises the
 
 /* ---- batched form: many class-form jobs in one persistent, work-stealing launch
  *      (replaces the v16 per-(variant, chunk) launch loop v16...cu:5611-5965) ---- */
+typedef struct ccs_jobset ccs_jobset;   /* jobs + their unit prefix, uploaded once */
+typedef struct ccs_graph ccs_graph;     /* a captured batched launch               */
+
+int ccs_jobset_create(ccs_ctx* ctx, const ccs_shape* shape, const ccs_job* h_jobs, uint32_t n_jobs,
+                      ccs_jobset** out);
+void ccs_jobset_free(ccs_jobset* js);
+/* replace the jobs of a set (at most as many as it was created with, within its stage-B
+ * buffer): the serving loop's per-slice job table */
+int ccs_jobset_update(ccs_ctx* ctx, ccs_jobset* js, const ccs_job* h_jobs, uint32_t n_jobs, void* stream);
+/* one persistent work-stealing launch over all jobs (kernels only: graph-capturable) */
+int ccs_jobset_launch(ccs_ctx* ctx, ccs_jobset* js, ccs_job_survivor* d_out, uint32_t cap,
+                      uint32_t* d_count, void* stream);
+/* the plain per-shape launch loop over the same jobs (comparator; v16...cu:5611-5965) */
+int ccs_jobset_launch_each(ccs_ctx* ctx, ccs_jobset* js, ccs_job_survivor* d_out, uint32_t cap,
+                           uint32_t* d_count, void* stream);
+/* counters of the last launch on the set (synchronises) */
+int ccs_jobset_stats(ccs_ctx* ctx, ccs_jobset* js, uint32_t* d_count, ccs_filter_stats* out);
+int ccs_graph_create(ccs_ctx* ctx, ccs_jobset* js, ccs_job_survivor* d_out, uint32_t cap,
+                     uint32_t* d_count, ccs_graph** out);
+int ccs_graph_launch(ccs_ctx* ctx, ccs_graph* graph, void* stream);
+void ccs_graph_free(ccs_graph* graph);
+int ccs_prove_jobset_launch(ccs_ctx* ctx, ccs_jobset* js, const ccs_job_survivor* d_surv,
+                            const uint32_t* d_count, uint32_t cap, const ccs_prove_params* params,
+                            ccs_chain_result* d_out, void* stream);
 int ccs_batch_launch(ccs_ctx* ctx, const ccs_shape* shape, const ccs_job* h_jobs, uint32_t n_jobs,
                      ccs_job_survivor* d_out, uint32_t cap, uint32_t* d_count, void* stream);
 int ccs_batch(ccs_ctx* ctx, const ccs_shape* shape, const ccs_job* h_jobs, uint32_t n_jobs,
@@ -210,6 +239,54 @@ int ccs_batch(ccs_ctx* ctx, const ccs_shape* shape, const ccs_job* h_jobs, uint3
 int ccs_prove_jobs(ccs_ctx* ctx, const ccs_shape* shape, const ccs_job* h_jobs, uint32_t n_jobs,
                    const ccs_job_survivor* h_surv, uint32_t n, const ccs_prove_params* params,
                    ccs_chain_result* h_out);
+
+/* ---- served form: continuous-batching request loop (no counterpart in the reference;
+ *      closest is the v16 dispatch ring + prove pool, v16...cu:2938-3279, :5594-5965).
+ *      The context is owned by the server's thread until ccs_server_shutdown; submit,
+ *      poll, results and drain may be called from any one client thread. ---- */
+typedef struct ccs_server ccs_server;
+
+typedef struct {
+    uint64_t mem_budget_bytes;      /* device bytes for admitted jobs' resident state      */
+    uint32_t max_active;            /* admitted jobs at most (and slice table size)         */
+    uint32_t surv_per_job;          /* survivor + result slots reserved per admitted job    */
+    uint32_t max_slice_survivors;   /* survivor buffer per slice and depth group            */
+    uint32_t pad;
+    uint64_t slice_candidates;      /* target candidates per slice over all admitted jobs   */
+    uint64_t min_chunk, max_chunk;  /* bounds of one job's chunk per slice (<= 2^32)        */
+    ccs_prove_params prove;         /* prove settings for every job                         */
+} ccs_server_params;
+
+typedef struct {
+    uint64_t tag;                   /* caller's identifier                                  */
+    int32_t depth;                  /* selects the shape (filter profile) of that depth     */
+    int32_t bits;                   /* informational: the width the range was drawn from    */
+    uint64_t a;                     /* residue class mod W of the depth's shape              */
+    ccs_u128 u0;                    /* n = a + W*(u0 + j), j in [0, count)                   */
+    uint64_t count;                 /* any length; sliced by the server                      */
+} ccs_request;
+
+typedef struct {
+    uint64_t job_id, tag, candidates;
+    uint32_t n_survivors, n_primes, best_len, pad;
+    double t_submit, t_admit, t_done;   /* steady-clock seconds */
+} ccs_job_done;
+
+typedef struct {
+    uint64_t candidates, slices, launches, resident_bytes;
+    uint32_t active, pending;
+} ccs_server_counters;
+
+int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, const int32_t* depths,
+                      uint32_t n_shapes, const ccs_server_params* params, ccs_server** out);
+int ccs_server_submit(ccs_server* server, const ccs_request* req, uint64_t* job_id);
+int ccs_server_poll(ccs_server* server, ccs_job_done* h_out, uint32_t cap, uint32_t* n);
+/* survivors (u_off within the request; job_idx 0) and their prove results of a finished job */
+int ccs_server_job_results(ccs_server* server, uint64_t job_id, ccs_job_survivor* h_surv,
+                           ccs_chain_result* h_res, uint32_t cap, uint32_t* n);
+int ccs_server_drain(ccs_server* server);
+int ccs_server_stats(ccs_server* server, ccs_server_counters* out);
+int ccs_server_shutdown(ccs_server* server);
 
 #ifdef __cplusplus
 }

@@ -115,6 +115,8 @@ typedef struct {
     u32 *basemod;        /* [nbases][nfp] base mod q                     */
     int nlut;            /* primes with a kM LUT                         */
     u16 *kMlut;          /* [WP][nlut] (k*M) mod q                       */
+    void *bv;            /* cached bit-vector tables (orc_filter_bv)     */
+    int bv_n;
 } OrcShape;
 
 static int is_forbidden_small(u32 q, int target, u64 r) {
@@ -136,6 +138,7 @@ static u8 rho_of(u64 WP, u64 M, u64 base, u64 k) {
 }
 
 ORC_EXPORT void orc_shape_free(void *hp);
+static void orc_bv_release(void *v);
 
 /* Restates v13.cu:907-1059 (generate_bases): bases = residues mod M admissible for the
  * CRT primes at target_len; per base the wheel offsets k in [0, WP) whose
@@ -248,6 +251,7 @@ ORC_EXPORT void orc_shape_free(void *hp) {
     free(s->bstart); free(s->bcount); free(s->bk);
     for (int i = 0; i < s->nfp; i++) free(s->fbits[i]);
     free(s->fbits); free(s->fp); free(s->WPMq); free(s->Mq); free(s->basemod); free(s->kMlut);
+    orc_bv_release(s->bv);
     free(s);
 }
 
@@ -797,4 +801,152 @@ ORC_EXPORT int64_t orc_filter_class(void *hp, u64 a, u64 W, u64 u0_lo, u64 u0_hi
     }
     free(r0); free(step);
     return ns;
+}
+
+/* ------------------------------------------------------------------------- */
+/* 64-candidate bit-vector filter (the v34 CPU baseline)                      */
+/* ------------------------------------------------------------------------- */
+
+/* Restates the bit-vector stage of v34 (cc_gmp_v34_bit-vector_10.c:642-780 precompute,
+ * :2270-2300 search): per (base, mod-6 bucket, block of 64 wheel offsets) and per leading
+ * filter prime q, a table kill[j] (j = (base + tile*WP*M) mod q) of the 64 candidates q
+ * kills; a block's survivors are ~OR_q kill[q][j_q].  The remaining primes are tested per
+ * candidate as in orc_filter.  Same survivors, same (tile, base, k) order. */
+typedef struct {
+    int nbv;            /* primes covered by kill tables */
+    u32 *qoff;          /* [nbv] offset of prime i's q entries in a block record */
+    u32 sumq;
+    u32 *bm_blk0;       /* [(b*6+m)] first block record of bucket (b, m) */
+    u64 *kill;          /* [block record][sumq] */
+} OrcBV;
+
+static OrcBV *orc_bv_build(const OrcShape *s, int nbv) {
+    OrcBV *v = (OrcBV *)calloc(1, sizeof(OrcBV));
+    if (nbv > s->nfp) nbv = s->nfp;
+    if (nbv > s->nlut) nbv = s->nlut;
+    v->nbv = nbv;
+    v->qoff = (u32 *)malloc((nbv ? nbv : 1) * 4);
+    for (int i = 0; i < nbv; i++) { v->qoff[i] = v->sumq; v->sumq += s->fp[i]; }
+    v->bm_blk0 = (u32 *)malloc((size_t)s->nbases * 6 * 4 + 4);
+    u64 nrec = 0;
+    for (int bm = 0; bm < s->nbases * 6; bm++) { v->bm_blk0[bm] = (u32)nrec; nrec += (s->bcount[bm] + 63) / 64; }
+    v->kill = (u64 *)calloc((size_t)(nrec ? nrec : 1) * (v->sumq ? v->sumq : 1), 8);
+    for (int bm = 0; bm < s->nbases * 6; bm++) {
+        const u32 *bk = s->bk + s->bstart[bm];
+        u32 nb = s->bcount[bm];
+        for (u32 blk = 0; blk * 64 < nb; blk++) {
+            u64 *rec = v->kill + (size_t)(v->bm_blk0[bm] + blk) * v->sumq;
+            u32 cnt = nb - blk * 64 < 64 ? nb - blk * 64 : 64;
+            u64 pad = cnt < 64 ? ~((1ull << cnt) - 1) : 0;
+            for (int i = 0; i < nbv; i++) {
+                u32 q = s->fp[i];
+                u64 *kv = rec + v->qoff[i];
+                for (u32 t = 0; t < cnt; t++) {
+                    u32 km = s->kMlut[(size_t)bk[blk * 64 + t] * s->nlut + i];
+                    for (u32 j = 0; j < q; j++) {
+                        u32 r = j + km; if (r >= q) r -= q;
+                        if (killed_at(s, i, r)) kv[j] |= 1ull << t;
+                    }
+                }
+                for (u32 j = 0; j < q; j++) kv[j] |= pad;
+            }
+        }
+    }
+    return v;
+}
+
+static void orc_bv_free(OrcBV *v) { if (!v) return; free(v->qoff); free(v->bm_blk0); free(v->kill); free(v); }
+static void orc_bv_release(void *v) { orc_bv_free((OrcBV *)v); }
+
+/* build the tables once per shape and prime count (startup work, like v34's precompute) */
+ORC_EXPORT void orc_bv_prepare(void *hp, int nbv) {
+    OrcShape *s = (OrcShape *)hp;
+    if (s->bv && s->bv_n == nbv) return;
+    orc_bv_free((OrcBV *)s->bv);
+    s->bv = orc_bv_build(s, nbv);
+    s->bv_n = nbv;
+}
+
+typedef struct {
+    const OrcShape *s; const OrcBV *v;
+    u128 t0; u64 tbeg, tend;
+    u32 *out; int64_t cap; int64_t n; u64 cand;
+} OrcBVWork;
+
+static void *orc_bv_worker(void *arg) {
+    OrcBVWork *w = (OrcBVWork *)arg;
+    const OrcShape *s = w->s;
+    const OrcBV *v = w->v;
+    int nfp = s->nfp, nlut = s->nlut, nbv = v->nbv;
+    u32 *T = (u32 *)malloc((nfp ? nfp : 1) * 4);
+    u32 *BT = (u32 *)malloc((nfp ? nfp : 1) * 4);
+    u128 tile = w->t0 + w->tbeg;
+    for (int i = 0; i < nfp; i++) T[i] = (u32)((u128)umod128(tile, s->fp[i]) * s->WPMq[i] % s->fp[i]);
+    for (u64 tr = w->tbeg; tr < w->tend; tr++, tile++) {
+        u32 rho = (u32)(tile % 6);
+        if (tr != w->tbeg)
+            for (int i = 0; i < nfp; i++) { u32 x = T[i] + s->WPMq[i]; if (x >= s->fp[i]) x -= s->fp[i]; T[i] = x; }
+        for (int b = 0; b < s->nbases; b++) {
+            const u32 *bm = s->basemod + (size_t)b * nfp;
+            for (int i = 0; i < nlut; i++) { u32 x = bm[i] + T[i]; if (x >= s->fp[i]) x -= s->fp[i]; BT[i] = x; }
+            int bmi = b * 6 + (int)rho;
+            const u32 *bk = s->bk + s->bstart[bmi];
+            u32 nb = s->bcount[bmi];
+            w->cand += nb;
+            for (u32 blk = 0; blk * 64 < nb; blk++) {
+                const u64 *rec = v->kill + (size_t)(v->bm_blk0[bmi] + blk) * v->sumq;
+                u64 c = 0;
+                for (int i = 0; i < nbv; i++) c |= rec[v->qoff[i] + BT[i]];
+                u64 alive = ~c;
+                while (alive) {
+                    int t = __builtin_ctzll(alive);
+                    alive &= alive - 1;
+                    u32 k = bk[blk * 64 + t];
+                    const u16 *lut = s->kMlut + (size_t)k * nlut;
+                    int kill = 0;
+                    for (int i = nbv; i < nfp; i++) {
+                        u32 q = s->fp[i], r;
+                        if (i < nlut) { r = BT[i] + lut[i]; if (r >= q) r -= q; }
+                        else r = (u32)(((u64)bm[i] + T[i] + (u64)(k % q) * s->Mq[i] % q) % q);
+                        if (killed_at(s, i, r)) { kill = 1; break; }
+                    }
+                    if (kill) continue;
+                    if (w->n < w->cap) { u32 *o = w->out + 4 * w->n; o[0] = (u32)b; o[1] = k; o[2] = (u32)tr; o[3] = 0; }
+                    w->n++;
+                }
+            }
+        }
+    }
+    free(T); free(BT);
+    return NULL;
+}
+
+/* survivors (count returned, cap written) and the candidate count, like orc_filter */
+ORC_EXPORT int64_t orc_filter_bv(void *hp, u64 t0_lo, u64 t0_hi, u64 tile_count, u32 *out, int64_t cap,
+                                 int nbv, u64 *candidates, int nthreads) {
+    OrcShape *s = (OrcShape *)hp;
+    orc_bv_prepare(hp, nbv);
+    OrcBV *v = (OrcBV *)s->bv;
+    if (nthreads < 1) nthreads = 1;
+    if ((u64)nthreads > tile_count) nthreads = tile_count ? (int)tile_count : 1;
+    OrcBVWork *w = (OrcBVWork *)calloc(nthreads, sizeof(OrcBVWork));
+    pthread_t *th = (pthread_t *)calloc(nthreads, sizeof(pthread_t));
+    u128 t0 = ((u128)t0_hi << 64) | t0_lo;
+    for (int t = 0; t < nthreads; t++) {
+        w[t].s = s; w[t].v = v; w[t].t0 = t0;
+        w[t].tbeg = tile_count * t / nthreads; w[t].tend = tile_count * (t + 1) / nthreads;
+        w[t].cap = cap; w[t].out = (u32 *)malloc((size_t)(cap > 0 ? cap : 1) * 16);
+        pthread_create(&th[t], NULL, orc_bv_worker, &w[t]);
+    }
+    int64_t total = 0; u64 cand = 0;
+    for (int t = 0; t < nthreads; t++) pthread_join(th[t], NULL);
+    for (int t = 0; t < nthreads; t++) {
+        int64_t keep = w[t].n < w[t].cap ? w[t].n : w[t].cap;
+        for (int64_t i = 0; i < keep && total + i < cap; i++) memcpy(out + 4 * (total + i), w[t].out + 4 * i, 16);
+        total += w[t].n; cand += w[t].cand;
+        free(w[t].out);
+    }
+    if (candidates) *candidates = cand;
+    free(w); free(th);
+    return total;
 }

@@ -20,6 +20,17 @@ u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 BPSW, MR = 0, 1
+
+
+def default_threads():
+    """worker threads: the process's CPU affinity, at most 16 (one GPU's CPU share)"""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 F_BITS_OK, F_TRIAL_OK, F_PRIME, F_NONROOT, F_OVERFLOW = 1, 2, 4, 8, 16
 
 
@@ -78,6 +89,10 @@ def lib():
     L.orc_filter_class.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                    C.c_uint64, u32p, C.c_int64, C.c_void_p, C.c_int, C.c_void_p]
     L.orc_filter_class.restype = C.c_int64
+    L.orc_filter_bv.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, u32p, C.c_int64,
+                                C.c_int, C.POINTER(C.c_uint64), C.c_int]
+    L.orc_filter_bv.restype = C.c_int64
+    L.orc_bv_prepare.argtypes = [C.c_void_p, C.c_int]
     _lib = L
     return L
 
@@ -190,7 +205,7 @@ class Shape:
         """Survivors as an (n, 4) uint32 array of {base_idx, k_offset, tile_idx, 0}
         in (tile, base, k) order, plus the stage counters."""
         if nthreads is None:
-            nthreads = os.cpu_count() or 1
+            nthreads = default_threads()
         out = np.zeros((cap, 4), np.uint32)
         cut = np.asarray(list(cuts) or [0], np.int32)
         counts = np.zeros(len(cut) + 1, np.uint64)
@@ -199,6 +214,22 @@ class Shape:
         if n > cap:
             raise RuntimeError(f"oracle survivor overflow: {n} > {cap}")
         return out[:n].copy(), [int(x) for x in counts[: len(cuts) + 1]]
+
+    def filter_bv(self, tile_start, tile_count, cap=1 << 20, nbv=None, nthreads=None):
+        """v34-style 64-candidate bit-vector filter over the primes <= 97 (or the first nbv),
+        per-candidate for the rest: the same survivors as filter(), and the candidate count."""
+        if nthreads is None:
+            nthreads = default_threads()
+        if nbv is None:
+            nbv = int(sum(1 for q in self.fp if q <= 97))
+        lib().orc_bv_prepare(self.h, nbv)
+        out = np.zeros((cap, 4), np.uint32)
+        cand = C.c_uint64(0)
+        n = lib().orc_filter_bv(self.h, *split(tile_start), tile_count, out.reshape(-1), cap, nbv,
+                                C.byref(cand), nthreads)
+        if n > cap:
+            raise RuntimeError(f"oracle survivor overflow: {n} > {cap}")
+        return out[:n].copy(), cand.value
 
     def filter_class(self, a, W, u0, count, cap=1 << 20, cuts=()):
         out = np.zeros(cap, np.uint32)
@@ -213,7 +244,7 @@ class Shape:
     def prove(self, tile_start, surv, bits=0, kind=BPSW, bases=(2,), nthreads=None):
         """Per survivor (flags, len, back, 0, root_lo, root_hi), restating prover_worker."""
         if nthreads is None:
-            nthreads = os.cpu_count() or 1
+            nthreads = default_threads()
         surv = np.ascontiguousarray(surv, np.uint32)
         out = np.zeros((len(surv), 6), np.uint64)
         b = np.asarray(bases, np.uint32)

@@ -1,0 +1,97 @@
+"""Batched form: one persistent work-stealing launch over many class-form jobs, the
+per-job launch loop and the captured HIP graph give the same survivors as the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+import ccs
+from ccs import workloads as WL
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_jobs(oracle, jobs, W, depth, cuts):
+    p = WL.shape_params("c4")
+    o = oracle.Shape(p["crt"], p["wheel"], 18, depth, p["filter_primes"], p["pos0_max_q"])
+    want, alive = [], [0] * (len(cuts) + 1)
+    for ji, (a, u0, count) in enumerate(jobs):
+        js, al = o.filter_class(a, W, u0, count, cuts=cuts)
+        want += [(ji, int(x)) for x in js]
+        alive = [x + y for x, y in zip(alive, al)]
+    return sorted(want), alive
+
+
+def read_out(ctx, d_out, d_count):
+    n = np.zeros(1, np.uint32)
+    ctx.d2h(n, d_count)
+    out = np.zeros(int(n[0]), ccs.JOB_SURVIVOR_DTYPE)
+    if len(out):
+        ctx.d2h(out, d_out)
+    return sorted(zip(out["job_idx"].tolist(), out["u_off"].tolist()))
+
+
+def test_jobset_launch_each_and_graph(ctx, oracle):
+    jobs, W = WL.batched_jobs(n_jobs=40, seed=0xB5, lo_exp=8, hi_exp=19)
+    s = WL.make_shape(ctx, "c4", classes=False, depth=9)
+    js = ccs.JobSet(s, jobs)
+    cap = 1 << 18
+    d_out = ctx.dalloc(cap * 8)
+    d_count = ctx.dalloc(64)
+    say(f"[batch] {len(jobs)} jobs, {js.candidates} candidates")
+    js.launch(d_out, cap, d_count)
+    ctx.synchronize()
+    got = read_out(ctx, d_out, d_count)
+    st = js.stats(d_count)
+    want, alive = oracle_jobs(oracle, jobs, W, 9, [st["dense_end"], st["tierb_end"], len(s.filter_primes)])
+    assert got == want and len(got) > 20
+    assert (st["candidates"], st["alive_dense"], st["alive_tierb"], st["survivors"]) == tuple(alive)
+    say("[batch] per-job loop")
+    js.launch_each(d_out, cap, d_count)
+    ctx.synchronize()
+    assert read_out(ctx, d_out, d_count) == want
+    st2 = js.stats(d_count)
+    assert (st2["alive_dense"], st2["alive_tierb"]) == (st["alive_dense"], st["alive_tierb"])
+    say("[batch] graph replays")
+    g = js.graph(d_out, cap, d_count)
+    for rep in range(4):
+        js.graph_launch(g)
+        ctx.synchronize()
+        assert read_out(ctx, d_out, d_count) == want, rep
+        # eager work between replays
+        js.launch_each(d_out, cap, d_count)
+        ctx.synchronize()
+    js.close()
+    ctx.dfree(d_out)
+    ctx.dfree(d_count)
+
+
+def test_jobs_with_empty_and_tiny_counts(ctx, oracle):
+    jobs, W = WL.batched_jobs(n_jobs=12, seed=0xB6, lo_exp=0, hi_exp=12)
+    jobs[3] = (jobs[3][0], jobs[3][1], 0)
+    s = WL.make_shape(ctx, "c4", classes=False, depth=6)
+    say("[batch] empty and tiny jobs")
+    g, st = s.batch(jobs)
+    want, alive = oracle_jobs(oracle, jobs, W, 6, [st["dense_end"], st["tierb_end"], len(s.filter_primes)])
+    assert sorted(zip(g["job_idx"].tolist(), g["u_off"].tolist())) == want
+    assert st["candidates"] == alive[0]
+
+
+def test_autotune_keeps_results(ctx, oracle):
+    s = WL.make_shape(ctx, "reference")
+    t0, _ = WL.reference_range(seed=0x31, bits=89)
+    say("[autotune] reference shape, 6 trials")
+    tuning, ms = s.autotune(t0, 6000, max_trials=6)
+    assert ms > 0 and tuning == s.tuning
+    p = WL.shape_params("reference")
+    o = oracle.Shape(p["crt"], p["wheel"], p["target"], p["depth"], p["filter_primes"], p["pos0_max_q"])
+    g, st = s.filter(t0, 1200)
+    o_surv, counts = o.filter(t0, 1200, cap=1 << 18, cuts=[st["dense_end"], st["tierb_end"]])
+    assert sorted(zip(g["tile_idx"].tolist(), g["base_idx"].tolist(), g["k_offset"].tolist())) == \
+        sorted((int(x[2]), int(x[0]), int(x[1])) for x in o_surv)
+    assert (st["candidates"], st["alive_dense"], st["alive_tierb"]) == tuple(counts)

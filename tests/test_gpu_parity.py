@@ -100,8 +100,8 @@ def test_u128_carry_and_high_bits(ctx, oracle, ref, oref):
 
 
 @pytest.mark.parametrize("tuning", [
-    dict(rows_per_group=1), dict(rows_per_group=4), dict(n_dense=0, n_tierb=143 - 0 if False else 100),
-    dict(n_dense=7, n_tierb=7), dict(n_dense=43, n_tierb=100), dict(n_dense=14, n_tierb=0),
+    dict(rows_per_group=1), dict(rows_per_group=4), dict(n_dense=0, n_tierb=100),
+    dict(n_dense=7, n_tierb=7), dict(n_dense=43, n_tierb=96), dict(n_dense=14, n_tierb=0),
     dict(n_dense=21, n_tierb=64), dict(n_dense=21, n_tierb=65), dict(unit_j=2048), dict(unit_j=6144),
     dict(blocks_per_cu=1),
 ])

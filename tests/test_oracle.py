@@ -356,3 +356,19 @@ def test_survival_rate_matches_analytic(ref_shape):
     assert abs(p - 1.1505e-5) < 1e-8          # the analytic survival of the reference shape
     expected = counts[0] * p
     assert abs(len(surv) - expected) < 5 * math.sqrt(expected)
+
+
+@pytest.mark.parametrize("name", ["reference", "c1", "c2"])
+def test_bitvector_filter_equals_per_candidate(oracle, name):
+    import sys
+    from conftest import PKG
+    sys.path.insert(0, PKG)
+    from ccs import workloads as WL
+    p = WL.shape_params(name)
+    s = oracle.Shape(p["crt"], p["wheel"], p["target"], p["depth"], p["filter_primes"], p["pos0_max_q"])
+    t0 = WL.c1_range()[0] if name == "c1" else (1 << 88) // (s.M * s.WP) + 3
+    for tc in (1, 7, 600):
+        a, c = s.filter(t0, tc, cap=1 << 18)
+        b, cand = s.filter_bv(t0, tc, cap=1 << 18)
+        assert cand == c[0]
+        assert np.array_equal(a, b)
