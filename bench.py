@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="c2", choices=["c2", "reference", "c3"])
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default tuning)")
+    ap.add_argument("--tune-trials", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this pipeline shards by disjoint tile ranges with no "
@@ -112,6 +114,11 @@ def main():
         shape.filter_launch(t0, tc, d_surv, cap, d_count)
         shape.prove_launch(t0, d_surv, d_count, pcap, d_res, bits=bits)
 
+    if not args.no_tune:
+        # autotuner on a 1/8 sample of the same range (setup work, not timed)
+        t_tune = time.perf_counter()
+        tuned, ms = shape.autotune(t0, max(6, (tc // 8) // 6 * 6), max_trials=args.tune_trials)
+        log(f"[bench] autotune {time.perf_counter() - t_tune:.1f}s -> {tuned} ({ms:.2f} ms on the sample)")
     log(f"[bench] {name}: tiles={tc} candidates/step={cand} tuning={shape.tuning}")
     for _ in range(args.warmup):
         step()

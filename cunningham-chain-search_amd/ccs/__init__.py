@@ -283,7 +283,7 @@ class Server:
     length, poll completions with their timestamps, fetch per-job results."""
 
     def __init__(self, ctx, shapes_by_depth, mem_budget_bytes=256 << 20, max_active=1024, surv_per_job=16,
-                 max_slice_survivors=1 << 16, slice_candidates=1 << 31, min_chunk=1 << 18,
+                 max_slice_survivors=1 << 16, slice_candidates=1 << 35, min_chunk=1 << 18,
                  max_chunk=1 << 32, bits=0, test="bpsw", bases=(2,)):
         self.ctx, self.lib = ctx, ctx.lib
         self.depths = sorted(shapes_by_depth)

@@ -63,7 +63,8 @@ class ShapeInfo(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [
         ("n_dense", C.c_uint32), ("n_tierb", C.c_uint32), ("rows_per_group", C.c_uint32),
-        ("unit_j", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+        ("unit_j", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("pair_max", C.c_uint32),
+        ("waves_per_block", C.c_uint32),
     ]
 
     def as_dict(self):

@@ -58,11 +58,14 @@ struct ccs_ctx {
     u64 out_bytes = 0;
     void* res = nullptr;                // prove result staging
     u64 res_bytes = 0;
+    void* chain = nullptr;              // prove chain queue: [0] count, [16..] indices
+    u64 chain_bytes = 0;
     JobRec* jobs = nullptr;
     u64* unit_prefix = nullptr;
     u32 jobs_cap = 0;
     ccs_filter_stats last = {};
     bool timed = false;
+    bool last_timed = false;            // events 0..2 hold the last launch's timing
     // kernel-time collection (ccs_profile_start/stop): event pairs per launch
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
@@ -186,6 +189,7 @@ extern "C" void ccs_ctx_destroy(ccs_ctx* c) {
     (void)hipFree(c->stats);
     (void)hipFree(c->out);
     (void)hipFree(c->res);
+    (void)hipFree(c->chain);
     (void)hipFree(c->jobs);
     (void)hipFree(c->unit_prefix);
     for (auto& ev : c->ev)
@@ -262,7 +266,7 @@ struct ccs_shape {
     PrimeMod* d_pmod = nullptr;
     DeepPrime* d_deep = nullptr;
     u32* d_hash = nullptr;
-    u32 lds_words = 0, nd = 0, nb = 0, nc = 0;
+    u32 lds_words = 0, nd = 0, nd_primes = 0, nb = 0, nc = 0;   // nd: dense slots
     double est_survival_b = 1.0;             // expected fraction alive after stage B
 };
 
@@ -304,9 +308,28 @@ static void free_device_tables(ccs_shape* s) {
     s->d_hash = nullptr;
 }
 
-static constexpr u32 kMaxLdsWords = (64u * 1024u) / 4u - 4u * kQueueCap;
+static constexpr u32 kMaxLdsWords = (64u * 1024u) / 4u - 16u * kQueueCap;
 
 static u32 dense_ext_words(u32 q, u32 rg) { return ((q - 1) >> 5) + 64u * rg + 1u; }
+
+// dense slots: consecutive dense primes folded in pairs while q1*q2 <= pair_max
+static std::vector<std::pair<int, int>> dense_slots(const ccs_shape* s, const ccs_tuning* t) {
+    std::vector<std::pair<int, int>> out;
+    for (u32 i = 0; i < t->n_dense;) {
+        if (i + 1 < t->n_dense && (u64)s->fp[i] * s->fp[i + 1] <= t->pair_max) {
+            out.push_back({(int)i, (int)i + 1});
+            i += 2;
+        } else {
+            out.push_back({(int)i, -1});
+            i += 1;
+        }
+    }
+    return out;
+}
+
+static u32 slot_period(const ccs_shape* s, std::pair<int, int> sl) {
+    return sl.second < 0 ? s->fp[sl.first] : s->fp[sl.first] * s->fp[sl.second];
+}
 
 // default split: dense primes below 128 (at most 64), stage B up to 256 primes below 2^12
 static void default_tuning(const ccs_shape* s, ccs_tuning* t) {
@@ -319,12 +342,17 @@ static void default_tuning(const ccs_shape* s, ccs_tuning* t) {
     t->rows_per_group = 2;
     t->unit_j = 65536;
     t->blocks_per_cu = 4;
+    t->pair_max = 0;
+    t->waves_per_block = 4;
 }
 
 static int validate_tuning(const ccs_shape* s, ccs_tuning* t) {
     if (t->rows_per_group != 1 && t->rows_per_group != 2 && t->rows_per_group != 4)
         return fail(CCS_ERR_INVALID, "rows_per_group must be 1, 2 or 4");
-    if (t->n_dense > (u32)kMaxDense) return fail(CCS_ERR_INVALID, "n_dense > 64");
+    if (t->n_dense > 2u * kMaxDense) return fail(CCS_ERR_INVALID, "n_dense > 128");
+    if (t->pair_max > (1u << 16)) return fail(CCS_ERR_INVALID, "pair_max > 65536");
+    const auto slots = dense_slots(s, t);
+    if (slots.size() > (size_t)kMaxDense) return fail(CCS_ERR_INVALID, "more than 64 dense patterns");
     if (t->n_tierb > 256) return fail(CCS_ERR_INVALID, "n_tierb > 256");
     if (t->n_dense + t->n_tierb > s->fp.size())
         return fail(CCS_ERR_INVALID, "n_dense + n_tierb exceeds the filter primes");
@@ -333,8 +361,12 @@ static int validate_tuning(const ccs_shape* s, ccs_tuning* t) {
     for (u32 i = 0; i < t->n_dense + t->n_tierb; ++i)
         if (s->fp[i] >= 4096) return fail(CCS_ERR_INVALID, "dense/stage-B primes must be < 4096");
     if (t->blocks_per_cu < 1 || t->blocks_per_cu > 16) return fail(CCS_ERR_INVALID, "blocks_per_cu");
+    if (t->waves_per_block == 0) t->waves_per_block = 4;
+    if (t->waves_per_block != 1 && t->waves_per_block != 2 && t->waves_per_block != 4 &&
+        t->waves_per_block != 8 && t->waves_per_block != 16)
+        return fail(CCS_ERR_INVALID, "waves_per_block must be 1, 2, 4, 8 or 16");
     u32 words = 0;
-    for (u32 i = 0; i < t->n_dense; ++i) words += dense_ext_words(s->fp[i], t->rows_per_group);
+    for (auto sl : slots) words += dense_ext_words(slot_period(s, sl), t->rows_per_group);
     for (u32 i = 0; i < t->n_tierb; ++i) words += (s->fp[t->n_dense + i] + 31) / 32;
     if (words > kMaxLdsWords) return fail(CCS_ERR_INVALID, "dense + stage-B tables exceed LDS");
     return CCS_OK;
@@ -345,8 +377,9 @@ static int upload_tables(ccs_shape* s) {
     const ccs_tuning& t = s->tune;
     const u32 nd = t.n_dense, nb = t.n_tierb, nfp = (u32)s->fp.size(), nc = nfp - nd - nb;
     const u32 rg = t.rows_per_group;
+    const auto slots = dense_slots(s, &t);
     std::vector<u32> image;
-    std::vector<DensePrime> dense(nd ? nd : 1);
+    std::vector<DensePrime> dense(slots.empty() ? 1 : slots.size());
     std::vector<TierBPrime> tierb(nb ? nb : 1);
     std::vector<PrimeMod> pmod(nd + nb ? nd + nb : 1);
     // u-space kill sets: u + (a * W^-1) = f * W^-1 (mod q)  for f in F_q
@@ -357,23 +390,32 @@ static int upload_tables(ccs_shape* s) {
         for (u32 f : s->forb[i]) kill[(u64)f * winv % q] = 1;
         return kill;
     };
-    for (u32 i = 0; i < nd; ++i) {
-        const u32 q = s->fp[i];
-        std::vector<u8> kill = uset(i);
-        const u32 words = dense_ext_words(q, rg);
+    for (u32 i = 0; i < nd; ++i) pmod[i] = make_primemod(s->fp[i], s->W);
+    for (size_t k = 0; k < slots.size(); ++k) {
+        const int i1 = slots[k].first, i2 = slots[k].second;
+        const u32 q1 = s->fp[i1], q2 = i2 < 0 ? 1 : s->fp[i2];
+        const u32 Q = q1 * q2;
+        std::vector<u8> k1 = uset(i1), k2;
+        if (i2 >= 0) k2 = uset(i2);
+        const u32 words = dense_ext_words(Q, rg);
         DensePrime d;
-        d.q = q;
+        d.q = Q;
         d.lds_off = (u32)image.size();
-        d.step = (u32)(((u64)kRowBits * rg) % q);
-        d.pad = 0;
-        dense[i] = d;
+        d.step = (u32)(((u64)kRowBits * rg) % Q);
+        d.q1 = q1;
+        d.q2 = q2;
+        d.inv = i2 < 0 ? 0 : (u32)mod_pow(q1 % q2, q2 - 2, q2);
+        d.pm1 = (u32)i1;
+        d.pm2 = (u32)(i2 < 0 ? i1 : i2);
+        dense[k] = d;
         for (u32 w = 0; w < words; ++w) {
             u32 v = 0;
-            for (u32 b = 0; b < 32; ++b)
-                if (kill[((u64)w * 32 + b) % q]) v |= 1u << b;
+            for (u32 b = 0; b < 32; ++b) {
+                const u64 x = (u64)w * 32 + b;
+                if (k1[x % q1] || (i2 >= 0 && k2[x % q2])) v |= 1u << b;
+            }
             image.push_back(v);
         }
-        pmod[i] = make_primemod(q, s->W);
     }
     for (u32 i = 0; i < nb; ++i) {
         const u32 q = s->fp[nd + i];
@@ -432,7 +474,8 @@ static int upload_tables(ccs_shape* s) {
     HIPCHK(up((void**)&s->d_bases, s->bases.data(), s->bases.size() * 8));
     HIPCHK(up((void**)&s->d_cls, s->classes.data(), s->classes.size() * sizeof(ClassRec)));
     s->lds_words = (u32)image.size();
-    s->nd = nd;
+    s->nd = (u32)slots.size();
+    s->nd_primes = nd;
     s->nb = nb;
     s->nc = nc;
     return CCS_OK;
@@ -574,7 +617,7 @@ extern "C" int ccs_shape_query(const ccs_shape* s, ccs_shape_info* o) {
     o->n_bases = (u32)s->bases.size();
     o->n_filter = (u32)s->fp.size();
     o->n_classes = s->classes.size();
-    o->n_dense = s->nd;
+    o->n_dense = s->nd_primes;
     o->n_tierb = s->nb;
     o->n_deep = s->nc;
     o->rows_per_group = s->tune.rows_per_group;
@@ -650,6 +693,7 @@ static void fill_sieve_common(const ccs_shape* s, const Scratch& x, SieveArgs& a
     a.lds_image = s->d_lds_image;
     a.lds_words = s->lds_words;
     a.nd = s->nd;
+    a.nd_primes = s->nd_primes;
     a.nb = s->nb;
     a.unit_j = s->tune.unit_j;
     a.dense = s->d_dense;
@@ -673,11 +717,13 @@ static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, 
     if (zero_out && launch_zero(d_count, 1, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
     sa.n_units = n_units;
     const int nbv = s->nb ? (int)((s->nb + 63) / 64) : 1;
-    const int block = 256;
-    const size_t lds = ((size_t)s->lds_words + (size_t)(block / 64) * kQueueCap) * 4;
+    const u32 wpb = s->tune.waves_per_block;
+    const int block = 64 * (int)wpb;
+    const size_t lds = ((size_t)s->lds_words + (size_t)wpb * kQueueCap) * 4;
     u64 want_grid = (u64)c->n_cu * s->tune.blocks_per_cu;
-    const u64 waves_needed = (n_units + 3) / 4;
-    if (want_grid > waves_needed) want_grid = waves_needed ? waves_needed : 1;
+    const u64 blocks_needed = (n_units + wpb - 1) / wpb;
+    if (want_grid > blocks_needed) want_grid = blocks_needed ? blocks_needed : 1;
+    c->last_timed = c->timed;
     if (c->timed) HIPCHK(hipEventRecord(c->ev[0], st));
     prof_begin(c, 0, st);
     hipError_t e = launch_sieve(sa, (int)s->tune.rows_per_group, nbv, (int)want_grid, block, lds, st);
@@ -732,8 +778,8 @@ static int filter_launch_impl(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint3
     da.out_cap = cap;
     c->last = ccs_filter_stats();
     c->last.candidates = cand;
-    c->last.dense_end = s->nd;
-    c->last.tierb_end = s->nd + s->nb;
+    c->last.dense_end = s->nd_primes;
+    c->last.tierb_end = s->nd_primes + s->nb;
     return run_filter_kernels(c, s, x, sa, da, n_units, d_count, st);
 }
 
@@ -765,8 +811,10 @@ static int collect_stats(ccs_ctx* c, u32 count, const Scratch& x) {
     c->last.alive_tierb = midc;
     c->last.survivors = count;
     float ms = 0;
-    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->last.ms_sieve = ms;
-    if (hipEventElapsedTime(&ms, c->ev[1], c->ev[2]) == hipSuccess) c->last.ms_deep = ms;
+    if (c->last_timed) {
+        if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) c->last.ms_sieve = ms;
+        if (hipEventElapsedTime(&ms, c->ev[1], c->ev[2]) == hipSuccess) c->last.ms_deep = ms;
+    }
     if (midc > x.mid_cap) return fail(CCS_ERR_NOMEM, "stage-B survivor buffer overflow");
     return CCS_OK;
 }
@@ -805,6 +853,15 @@ extern "C" int ccs_last_stats(ccs_ctx* c, ccs_filter_stats* out) {
 // prove
 // ---------------------------------------------------------------------------
 
+// point a prove launch at a chain-queue scratch buffer of at least cap entries
+static int prove_scratch(void** buf, u64* have, u32 cap, ProveArgs& a) {
+    int rc = ensure_buf(buf, have, ((u64)cap + 64) * 4);
+    if (rc) return rc;
+    a.chain_count = (u32*)*buf;
+    a.chain_queue = (u32*)*buf + 16;
+    return CCS_OK;
+}
+
 static int fill_prove(const ccs_shape* s, const ccs_prove_params* p, ProveArgs& a) {
     memset(&a, 0, sizeof(a));
     a.WP = s->WP;
@@ -841,6 +898,8 @@ extern "C" int ccs_prove_launch(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, con
     a.surv_count = d_count;
     a.surv_cap = cap;
     a.out = d_out;
+    rc = prove_scratch(&c->chain, &c->chain_bytes, cap, a);
+    if (rc) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     if (c->timed) HIPCHK(hipEventRecord(c->ev[3], st));
     prof_begin(c, 2, st);
@@ -866,6 +925,8 @@ static int prove_host_common(ccs_ctx* c, ProveArgs& a, const void* h_surv, size_
     a.surv_count = d_count;
     a.surv_cap = n;
     a.out = d_res;
+    rc = prove_scratch(&c->chain, &c->chain_bytes, n, a);
+    if (rc) return rc;
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
     hipError_t e = launch_prove(a, n, c->stream);
     if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("prove launch: ") + hipGetErrorString(e));
@@ -946,7 +1007,11 @@ struct ccs_jobset {
     std::vector<u64> prefix;
     u64 n_units = 0, candidates = 0;
     u32 cap_jobs = 0;
-    std::vector<JobRec> staging;
+    JobRec* staging = nullptr;      // pinned upload staging of ccs_jobset_update
+    u64* staging_prefix = nullptr;
+    hipEvent_t staged = nullptr;    // the last staging upload has been consumed
+    void* chain = nullptr;          // prove chain queue of launches on this set
+    u64 chain_bytes = 0;
     Scratch x = {nullptr, 0, nullptr, nullptr};
 };
 
@@ -958,6 +1023,10 @@ extern "C" void ccs_jobset_free(ccs_jobset* js) {
     (void)hipFree(js->x.mid);
     (void)hipFree(js->x.counters);
     (void)hipFree(js->x.stats);
+    (void)hipFree(js->chain);
+    if (js->staging) (void)hipHostFree(js->staging);
+    if (js->staging_prefix) (void)hipHostFree(js->staging_prefix);
+    if (js->staged) (void)hipEventDestroy(js->staged);
     delete js;
 }
 
@@ -1013,31 +1082,39 @@ extern "C" int ccs_jobset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jo
     if (!js || !h_jobs || n_jobs == 0) return fail(CCS_ERR_INVALID, "null argument or no jobs");
     if (n_jobs > js->cap_jobs) return fail(CCS_ERR_INVALID, "more jobs than the set was created with");
     const ccs_shape* s = js->shape;
-    js->staging.resize(n_jobs);
-    js->prefix.resize((size_t)n_jobs + 1);
-    js->n_units = 0;
-    js->candidates = 0;
+    if (!js->staging) {
+        HIPCHK(hipHostMalloc((void**)&js->staging, (size_t)js->cap_jobs * sizeof(JobRec)));
+        HIPCHK(hipHostMalloc((void**)&js->staging_prefix, ((size_t)js->cap_jobs + 1) * 8));
+        HIPCHK(hipEventCreateWithFlags(&js->staged, hipEventDisableTiming));
+    } else {
+        HIPCHK(hipEventSynchronize(js->staged));   // the previous upload has left the staging
+    }
+    u64 units = 0, cand = 0;
     for (u32 i = 0; i < n_jobs; ++i) {
         if (h_jobs[i].a >= s->W) return fail(CCS_ERR_INVALID, "job residue a must be < W");
         if (h_jobs[i].count > (1ull << 32)) return fail(CCS_ERR_INVALID, "job count must be <= 2^32");
+        cand += h_jobs[i].count;
+    }
+    if ((u64)(cand * s->est_survival_b * 2.0) + (1u << 16) > js->x.mid_cap)
+        return fail(CCS_ERR_INVALID, "jobs exceed the stage-B buffer of the set");
+    js->prefix.resize((size_t)n_jobs + 1);
+    for (u32 i = 0; i < n_jobs; ++i) {
         JobRec& r = js->staging[i];
         r.a = h_jobs[i].a;
         r.u0_lo = h_jobs[i].u0.lo;
         r.u0_hi = h_jobs[i].u0.hi;
         r.count = h_jobs[i].count;
-        js->prefix[i] = js->n_units;
-        js->n_units += (h_jobs[i].count + s->tune.unit_j - 1) / s->tune.unit_j;
-        js->candidates += h_jobs[i].count;
+        js->prefix[i] = js->staging_prefix[i] = units;
+        units += (h_jobs[i].count + s->tune.unit_j - 1) / s->tune.unit_j;
     }
-    js->prefix[n_jobs] = js->n_units;
-    if ((u64)(js->candidates * s->est_survival_b * 2.0) + (1u << 16) > js->x.mid_cap)
-        return fail(CCS_ERR_INVALID, "jobs exceed the stage-B buffer of the set");
+    js->prefix[n_jobs] = js->staging_prefix[n_jobs] = units;
+    js->n_units = units;
+    js->candidates = cand;
     js->n_jobs = n_jobs;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
-    HIPCHK(hipMemcpyAsync(js->d_jobs, js->staging.data(), (size_t)n_jobs * sizeof(JobRec), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(js->d_prefix, js->prefix.data(), ((size_t)n_jobs + 1) * 8, hipMemcpyHostToDevice, st));
-    // the staging vectors must outlive the copies: synchronise before returning
-    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemcpyAsync(js->d_jobs, js->staging, (size_t)n_jobs * sizeof(JobRec), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(js->d_prefix, js->staging_prefix, ((size_t)n_jobs + 1) * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipEventRecord(js->staged, st));
     return CCS_OK;
 }
 
@@ -1062,8 +1139,8 @@ static void jobset_args(const ccs_jobset* js, SieveArgs& sa, DeepArgs& da, void*
 static void jobset_stats_begin(ccs_ctx* c, const ccs_jobset* js) {
     c->last = ccs_filter_stats();
     c->last.candidates = js->candidates;
-    c->last.dense_end = js->shape->nd;
-    c->last.tierb_end = js->shape->nd + js->shape->nb;
+    c->last.dense_end = js->shape->nd_primes;
+    c->last.tierb_end = js->shape->nd_primes + js->shape->nb;
 }
 
 // one persistent, work-stealing launch over every job of the set
@@ -1260,6 +1337,8 @@ extern "C" int ccs_prove_jobset_launch(ccs_ctx* c, ccs_jobset* js, const ccs_job
     a.surv_count = d_count;
     a.surv_cap = cap;
     a.out = d_out;
+    rc = prove_scratch(&js->chain, &js->chain_bytes, cap, a);
+    if (rc) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     prof_begin(c, 2, st);
     hipError_t e = launch_prove(a, cap, st);
@@ -1337,19 +1416,41 @@ extern "C" int ccs_autotune(ccs_ctx* c, ccs_shape* s, ccs_u128 ts, uint32_t tile
     if (rc) return rc;
     u32 trials = 1;
     // candidate values per knob
-    std::vector<u32> dense_cut = {48, 64, 80, 100, 128, 160, 200, 256, 320};
+    std::vector<u32> dense_cut = {48, 64, 80, 100, 128, 160, 200, 256, 320, 400, 512, 640};
     auto count_below = [&](u32 lim) {
         u32 n = 0;
-        while (n < s->fp.size() && n < (u32)kMaxDense && s->fp[n] < lim) ++n;
+        while (n < s->fp.size() && n < 2u * kMaxDense && s->fp[n] < lim) ++n;
         return n;
+    };
+    // time one tuning; n_tierb is kept (clamped to the primes below 4096 and to LDS)
+    auto try_tuning_nb = [&](ccs_tuning t) -> int {
+        if (trials >= max_trials) return CCS_OK;
+        u32 avail = 0;
+        while (t.n_dense + avail < s->fp.size() && avail < 256 && s->fp[t.n_dense + avail] < 4096) ++avail;
+        if (t.n_tierb > avail) t.n_tierb = avail;
+        for (;;) {
+            ccs_tuning tt = t;
+            if (validate_tuning(s, &tt) == CCS_OK) break;
+            if (t.n_tierb == 0) return CCS_OK;
+            t.n_tierb = t.n_tierb > 16 ? t.n_tierb - 16 : 0;
+        }
+        s->tune = t;
+        int r2 = upload_tables(s);
+        if (r2) return r2;
+        float ms = 0;
+        r2 = time_filter(c, s, ts, tile_count, 2, &ms);
+        if (r2) return r2;
+        ++trials;
+        if (ms < best_ms) { best_ms = ms; best = t; }
+        return CCS_OK;
     };
     auto try_tuning = [&](ccs_tuning t) -> int {
         if (trials >= max_trials) return CCS_OK;
-        u32 room = 0;   // stage B takes every prime below 4096 that fits (<= 256)
+        // stage B keeps its size but follows the dense cutoff
+        const u32 keep = t.n_tierb;
         t.n_tierb = 0;
-        while (t.n_dense + t.n_tierb < s->fp.size() && t.n_tierb < 256 && s->fp[t.n_dense + t.n_tierb] < 4096)
+        while (t.n_dense + t.n_tierb < s->fp.size() && t.n_tierb < keep && s->fp[t.n_dense + t.n_tierb] < 4096)
             ++t.n_tierb;
-        (void)room;
         for (;;) {
             ccs_tuning tt = t;
             if (validate_tuning(s, &tt) == CCS_OK) break;
@@ -1374,6 +1475,12 @@ extern "C" int ccs_autotune(ccs_ctx* c, ccs_shape* s, ccs_u128 ts, uint32_t tile
             if (t.n_dense == base.n_dense) continue;
             if ((rc = try_tuning(t))) return rc;
         }
+        for (u32 pm : {0u, 4096u, 16384u, 65536u}) {
+            ccs_tuning t = best;
+            if (pm == t.pair_max) continue;
+            t.pair_max = pm;
+            if ((rc = try_tuning(t))) return rc;
+        }
         for (u32 rg : {1u, 2u, 4u}) {
             ccs_tuning t = best;
             if (rg == t.rows_per_group) continue;
@@ -1384,6 +1491,17 @@ extern "C" int ccs_autotune(ccs_ctx* c, ccs_shape* s, ccs_u128 ts, uint32_t tile
             ccs_tuning t = best;
             if (uj == t.unit_j) continue;
             t.unit_j = uj;
+            if ((rc = try_tuning(t))) return rc;
+        }
+        for (u32 nb : {32u, 64u, 128u, 192u, 256u}) {
+            ccs_tuning t = best;
+            t.n_tierb = nb;
+            if ((rc = try_tuning_nb(t))) return rc;
+        }
+        for (u32 wpb : {2u, 4u, 8u, 16u}) {
+            ccs_tuning t = best;
+            if (wpb == t.waves_per_block) continue;
+            t.waves_per_block = wpb;
             if ((rc = try_tuning(t))) return rc;
         }
         for (u32 bpc : {1u, 2u, 3u, 4u, 6u, 8u}) {

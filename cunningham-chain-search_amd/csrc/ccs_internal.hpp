@@ -34,11 +34,16 @@ struct PrimeMod {
 // Dense stage: kill pattern over u, periodic with period q, stored as an extended bit
 // string in LDS so that a row (64 lanes x 32 bits) starting at any phase s reads the two
 // dwords (s>>5)+lane and (s>>5)+lane+1 with no wrap-around.
+// A dense slot is one prime, or two primes q1*q2 folded into one pattern of period q1*q2
+// (bit x killed iff x mod q1 or x mod q2 is killed); its phase is the CRT combination of
+// the two primes' phases.
 struct DensePrime {
-    u32 q;
+    u32 q;          // period of the slot's pattern (q1 or q1*q2)
     u32 lds_off;    // dword offset of the extended pattern in LDS
     u32 step;       // (kRowBits * RG) mod q: phase advance per row group
-    u32 pad;
+    u32 q1, q2;     // q2 = 1 for a single prime
+    u32 inv;        // q1^-1 mod q2
+    u32 pm1, pm2;   // indices of the primes' PrimeMod records
 };
 
 // Stage B: per-candidate test with a q-bit forbidden bitset in LDS.
@@ -98,7 +103,8 @@ struct SieveArgs {
     // tables
     const u32* lds_image;     // dense patterns + tier-B bitsets, copied to LDS at start
     u32 lds_words;
-    u32 nd;                   // dense primes
+    u32 nd;                   // dense slots (one or two primes each)
+    u32 nd_primes;            // dense primes (PrimeMod records before the stage-B ones)
     u32 nb;                   // tier-B primes
     u32 unit_j;               // candidates (u values) per work unit
     const DensePrime* dense;
@@ -152,6 +158,8 @@ struct ProveArgs {
     int n_bases;
     u32 bases_mr[16];
     void* out;                 // ccs_chain_result per survivor
+    u32* chain_queue;          // [surv_cap] indices of prime survivors (scratch)
+    u32* chain_count;          // queue length (scratch, cleared by the launcher)
 };
 
 // kernel launchers (ccs_kernels.hip)

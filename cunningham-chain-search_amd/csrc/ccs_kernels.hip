@@ -205,7 +205,7 @@ __device__ __forceinline__ void stage_b(const SieveArgs& A, const u32* __restric
 }
 
 template <int RG, int NBV>
-__global__ __launch_bounds__(256) void sieve_kernel(SieveArgs A) {
+__global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
     extern __shared__ u32 lds[];
     const u32 tid = threadIdx.x;
     for (u32 i = tid; i < A.lds_words; i += blockDim.x) lds[i] = A.lds_image[i];
@@ -243,12 +243,21 @@ __global__ __launch_bounds__(256) void sieve_kernel(SieveArgs A) {
         // phases: dense prime p held by lane p as an LDS bit address, stage-B prime
         // (64*i + lane) held in sb[i]
         u32 sd = 0;
-        if (lane < A.nd) sd = doff * 32u + phase(A.pmod[lane], U.a, U.us_lo, U.us_hi);
+        if (lane < A.nd) {
+            const DensePrime d = A.dense[lane];
+            u32 ph = phase(A.pmod[d.pm1], U.a, U.us_lo, U.us_hi);
+            if (d.q2 > 1) {   // CRT: the slot phase is s1 (mod q1) and s2 (mod q2)
+                const u32 s2 = phase(A.pmod[d.pm2], U.a, U.us_lo, U.us_hi);
+                const u32 t = ((s2 + d.q2 - ph % d.q2) % d.q2) * d.inv % d.q2;
+                ph += d.q1 * t;
+            }
+            sd = doff * 32u + ph;
+        }
         u32 sb[NBV];
 #pragma unroll
         for (int i = 0; i < NBV; ++i) {
             const u32 p = (u32)i * 64u + lane;
-            sb[i] = p < A.nb ? phase(A.pmod[A.nd + p], U.a, U.us_lo, U.us_hi) : 0u;
+            sb[i] = p < A.nb ? phase(A.pmod[A.nd_primes + p], U.a, U.us_lo, U.us_hi) : 0u;
         }
 
         const u32 rows = (U.n + kRowBits - 1) / kRowBits;
@@ -601,76 +610,65 @@ __device__ __forceinline__ U2 cand_n(u64 W, u64 a, u64 u_lo, u64 u_hi) {
     return add(n, mk(a, 0));
 }
 
+// one wave per stage-B survivor; the 64 lanes test 64 deep primes at a time, so the rare
+// survivors are not serialised over thousands of dependent per-prime steps
 __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
     const u32 lane = lane_id();
     u32 total = *A.mid_count;
     if (total > A.mid_cap) total = A.mid_cap;
-    const u32 stride = gridDim.x * blockDim.x;
-    // grid-stride with a wave-uniform trip count so ballots stay convergent
-    const u32 first = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-    for (u32 base = first; base < total; base += stride) {
-        const u32 idx = base + lane;
-        bool alive = idx < total;
-        U2 n = mk(0, 0);
-        u32 owner = 0, j = 0;
-        if (alive) {
-            const MidRec rec = A.mid[idx];
-            owner = rec.owner;
-            j = rec.j;
-            u64 ulo, uhi, a;
-            if (A.mode == 0) {
-                const ClassRec cr = A.plain.cls[owner];
-                a = cr.a;
-                ulo = A.plain.U0_lo;
-                uhi = A.plain.U0_hi;
-            } else {
-                const JobRec jr = A.jobsrc.jobs[owner];
-                a = jr.a;
-                ulo = jr.u0_lo;
-                uhi = jr.u0_hi;
-            }
-            add128(ulo, uhi, j);
-            n = cand_n(A.W, a, ulo, uhi);
+    const u32 waves = gridDim.x * (blockDim.x >> 6);
+    const u32 wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    for (u32 e = wid; e < total; e += waves) {
+        const MidRec rec = A.mid[e];
+        const u32 owner = rec.owner, j = rec.j;
+        u64 ulo, uhi, a;
+        if (A.mode == 0) {
+            const ClassRec cr = A.plain.cls[owner];
+            a = cr.a;
+            ulo = A.plain.U0_lo;
+            uhi = A.plain.U0_hi;
+        } else {
+            const JobRec jr = A.jobsrc.jobs[owner];
+            a = jr.a;
+            ulo = jr.u0_lo;
+            uhi = jr.u0_hi;
         }
-        for (u32 p = 0; p < A.nc; ++p) {
-            if (ballot(alive) == 0) break;
-            const DeepPrime dp = A.deep[p];
-            if (alive) {
+        add128(ulo, uhi, j);
+        const U2 n = cand_n(A.W, a, ulo, uhi);
+        bool dead = false;
+        for (u32 p0 = 0; p0 < A.nc; p0 += 64) {
+            const u32 p = p0 + lane;
+            bool kill = false;
+            if (p < A.nc) {
+                const DeepPrime dp = A.deep[p];
                 const u32 r = mod128(n.lo, n.hi, dp.pm);
                 const u32* h = A.hash + dp.hash_off;
                 u32 slot = r & (kMaxHashSlots - 1);
                 for (int probe = 0; probe < kMaxHashSlots; ++probe) {
                     const u32 v = h[slot];
-                    if (v == r) { alive = false; break; }
+                    if (v == r) { kill = true; break; }
                     if (v == 0xffffffffu) break;
                     slot = (slot + 1) & (kMaxHashSlots - 1);
                 }
             }
+            if (ballot(kill)) { dead = true; break; }
         }
-        const u64 m = ballot(alive);
-        if (m) {
-            u32 ob = 0;
-            if (lane == 0) ob = atomicAdd(A.out_count, (u32)__popcll(m));
-            ob = bcast32(ob);
-            if (alive) {
-                const u32 slot = ob + lanes_below(m);
-                if (slot < A.out_cap) {
-                    if (A.mode == 0) {
-                        const ClassRec cr = A.plain.cls[owner];
-                        ccs_survivor s;
-                        s.base_idx = cr.base_idx;
-                        s.k_offset = cr.k;
-                        s.tile_idx = 6u * j + cr.rho - A.plain.off;
-                        s._pad = 0;
-                        ((ccs_survivor*)A.out)[slot] = s;
-                    } else {
-                        ccs_job_survivor s;
-                        s.job_idx = owner;
-                        s.u_off = j;
-                        ((ccs_job_survivor*)A.out)[slot] = s;
-                    }
-                }
-            }
+        if (dead || lane != 0) continue;
+        const u32 slot = atomicAdd(A.out_count, 1u);
+        if (slot >= A.out_cap) continue;
+        if (A.mode == 0) {
+            const ClassRec cr = A.plain.cls[owner];
+            ccs_survivor s;
+            s.base_idx = cr.base_idx;
+            s.k_offset = cr.k;
+            s.tile_idx = 6u * j + cr.rho - A.plain.off;
+            s._pad = 0;
+            ((ccs_survivor*)A.out)[slot] = s;
+        } else {
+            ccs_job_survivor s;
+            s.job_idx = owner;
+            s.u_off = j;
+            ((ccs_job_survivor*)A.out)[slot] = s;
         }
     }
 }
@@ -679,25 +677,29 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
 // K2: prove kernel — GMP confirmation engine per survivor
 // ---------------------------------------------------------------------------
 
+__device__ __forceinline__ U2 survivor_n(const ProveArgs& A, u32 idx) {
+    if (A.mode == 0) {
+        const ccs_survivor s = ((const ccs_survivor*)A.surv)[idx];
+        U2 tile = mk(A.T0_lo, A.T0_hi);
+        tile = add(tile, mk(s.tile_idx, 0));
+        const U2 k = add(mul_u64(tile, A.WP), mk(s.k_offset, 0));
+        return add(mul_u64(k, A.M), mk(A.bases[s.base_idx], 0));
+    }
+    const ccs_job_survivor s = ((const ccs_job_survivor*)A.surv)[idx];
+    const JobRec jr = A.jobs[s.job_idx];
+    u64 ulo = jr.u0_lo, uhi = jr.u0_hi;
+    add128(ulo, uhi, s.u_off);
+    return cand_n(A.W, jr.a, ulo, uhi);
+}
+
+// K2a: per survivor the gates of prover_worker (v13.cu:1828-1842): bit size, trial division
+// by 67..409, primality of n.  Composites are final here; primes go to the chain queue.
 __global__ __launch_bounds__(128) void prove_kernel(ProveArgs A) {
     u32 total = *A.surv_count;
     if (total > A.surv_cap) total = A.surv_cap;
     const u32 stride = gridDim.x * blockDim.x;
     for (u32 idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
-        U2 n;
-        if (A.mode == 0) {
-            const ccs_survivor s = ((const ccs_survivor*)A.surv)[idx];
-            U2 tile = mk(A.T0_lo, A.T0_hi);
-            tile = add(tile, mk(s.tile_idx, 0));
-            U2 k = add(mul_u64(tile, A.WP), mk(s.k_offset, 0));
-            n = add(mul_u64(k, A.M), mk(A.bases[s.base_idx], 0));
-        } else {
-            const ccs_job_survivor s = ((const ccs_job_survivor*)A.surv)[idx];
-            const JobRec jr = A.jobs[s.job_idx];
-            u64 ulo = jr.u0_lo, uhi = jr.u0_hi;
-            add128(ulo, uhi, s.u_off);
-            n = cand_n(A.W, jr.a, ulo, uhi);
-        }
+        const U2 n = survivor_n(A, idx);
         ccs_chain_result res;
         res.survivor_idx = idx;
         res.chain_len = 0;
@@ -705,7 +707,7 @@ __global__ __launch_bounds__(128) void prove_kernel(ProveArgs A) {
         res.flags = 0;
         res.root_lo = 0;
         res.root_hi = 0;
-        bool ok = true;
+        bool ok = true, queue = false;
         do {
             if (A.bits && bitlen(n) != A.bits) break;
             res.flags |= CCS_F_BITS_OK;
@@ -717,36 +719,100 @@ __global__ __launch_bounds__(128) void prove_kernel(ProveArgs A) {
             if (n.hi >> 63) { res.flags |= CCS_F_OVERFLOW; break; }
             if (!test_prime(n, A, ok)) break;
             res.flags |= CCS_F_PRIME;
-            U2 root = n;
-            u32 back = 0;
-            U2 pred = shr1(sub(n, mk(1, 0)));
-            if (ge(pred, mk(2, 0)) && test_prime(pred, A, ok)) {
-                res.flags |= CCS_F_NONROOT;
-                root = pred;
-                back = 1;
-                for (int steps = 0; steps < 64; ++steps) {
-                    pred = shr1(sub(root, mk(1, 0)));
-                    if (!ge(pred, mk(2, 0))) break;
-                    if (!test_prime(pred, A, ok)) break;
-                    root = pred;
-                    ++back;
-                }
+            queue = true;
+        } while (0);
+        if (!ok) res.flags |= CCS_F_UNSUPPORTED;
+        ((ccs_chain_result*)A.out)[idx] = res;
+        if (queue) {
+            const u32 q = atomicAdd(A.chain_count, 1u);
+            if (q < A.surv_cap) A.chain_queue[q] = idx;
+        }
+    }
+}
+
+// K2b: one wave per prime survivor.  Lane l tests chain position l - 16: lanes 0..15 the
+// backward values (n-1)/2, ((n-1)/2 - 1)/2, ... (16..1 steps back), lanes 16..63 the forward
+// values 2^i (n+1) - 1, i = 0..47, all at once; the backward walk and forward follow of
+// prover_worker (v13.cu:1849-1881) are then read off the ballot.  Walks longer than the
+// lanes cover continue sequentially, so the result is the same for any chain.
+__global__ __launch_bounds__(256) void chain_kernel(ProveArgs A) {
+    const u32 lane = lane_id();
+    u32 total = *A.chain_count;
+    if (total > A.surv_cap) total = A.surv_cap;
+    const u32 waves = gridDim.x * (blockDim.x >> 6);
+    const u32 wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    for (u32 e = wid; e < total; e += waves) {
+        const u32 idx = A.chain_queue[e];
+        const U2 n = survivor_n(A, idx);
+        bool ok = true;
+        // this lane's chain value
+        U2 v = n;
+        bool valid = true;        // value defined and below 2^127
+        if (lane < 16) {
+            const int k = 16 - (int)lane;          // steps back
+            for (int t = 0; t < k; ++t) {
+                if (!ge(v, mk(2, 0))) { valid = false; break; }
+                v = shr1(sub(v, mk(1, 0)));
             }
-            u32 len = 1;
-            U2 cur = root;
+            if (valid && !ge(v, mk(2, 0))) valid = false;
+        } else {
+            const int i = (int)lane - 16;          // steps forward
+            for (int t = 0; t < i; ++t) {
+                if (v.hi >> 62) { valid = false; break; }
+                v = add(shl1(v), mk(1, 0));
+            }
+        }
+        bool prime = valid && (lane == 16 || test_prime(v, A, ok));   // n itself is prime
+        const u64 pm = ballot(prime);
+        const u64 vm = ballot(valid);
+        const u64 okm = ballot(ok);
+        if (lane != 0) continue;
+        ccs_chain_result res = ((ccs_chain_result*)A.out)[idx];
+        // backward: consecutive primes at 1, 2, ... steps back (lanes 15, 14, ...)
+        u32 back = 0;
+        while (back < 16 && ((pm >> (15 - back)) & 1)) ++back;
+        U2 root = n;
+        for (u32 t = 0; t < back; ++t) root = shr1(sub(root, mk(1, 0)));
+        if (back == 16) {   // longer walk: continue one value at a time (cap: 65 steps)
+            for (;;) {
+                if (back >= 65) break;
+                const U2 pred = shr1(sub(root, mk(1, 0)));
+                if (!ge(pred, mk(2, 0))) break;
+                if (!test_prime(pred, A, ok)) break;
+                root = pred;
+                ++back;
+            }
+        }
+        // forward from n: consecutive primes at 0, 1, ... steps (lanes 16, 17, ...)
+        u32 fwd = 0;
+        bool over = false;
+        while (fwd < 48 && ((pm >> (16 + fwd)) & 1)) ++fwd;
+        u32 len = back + fwd;
+        if (len > 64) len = 64;
+        if (fwd < 48 && len < 64 && !((vm >> (16 + fwd)) & 1)) over = true;   // next value >= 2^127
+        if (fwd == 48 && len < 64) {   // longer chain: continue one value at a time
+            U2 cur = n;
+            for (u32 t = 0; t < 47; ++t) cur = add(shl1(cur), mk(1, 0));
             while (len < 64) {
-                if (cur.hi >> 62) { res.flags |= CCS_F_OVERFLOW; break; }
+                if (cur.hi >> 62) { over = true; break; }
                 const U2 nx = add(shl1(cur), mk(1, 0));
                 if (!test_prime(nx, A, ok)) break;
                 ++len;
                 cur = nx;
             }
-            res.root_lo = root.lo;
-            res.root_hi = root.hi;
-            res.chain_len = len;
-            res.back_steps = back;
-        } while (0);
-        if (!ok) res.flags |= CCS_F_UNSUPPORTED;
+        }
+        // lanes on the walk: the backward run and its first composite, n's forward run and its
+        // first composite; only their tests count for the unsupported flag
+        const u32 blo = back < 16 ? 15 - back : 0;
+        const u32 fhi = 16 + fwd < 63 ? 16 + fwd : 63;
+        const u64 walk = (fhi >= 63 ? ~0ull : ((1ull << (fhi + 1)) - 1)) & ~((1ull << blo) - 1);
+        if (back) res.flags |= CCS_F_NONROOT;
+        if (over) res.flags |= CCS_F_OVERFLOW;
+        if ((~okm & walk) || !ok) res.flags |= CCS_F_UNSUPPORTED;
+        res.root_lo = root.lo;
+        res.root_hi = root.hi;
+        res.chain_len = len;
+        res.back_steps = back;
         ((ccs_chain_result*)A.out)[idx] = res;
     }
 }
@@ -762,6 +828,7 @@ __global__ void zero_kernel(u32* p, size_t n) {
 
 template <int RG, int NBV>
 static hipError_t launch_sieve_t(const SieveArgs& a, int grid, int block, size_t lds, hipStream_t s) {
+    (void)hipGetLastError();   // launch errors only: drop any earlier non-launch error
     hipLaunchKernelGGL((sieve_kernel<RG, NBV>), dim3(grid), dim3(block), lds, s, a);
     return hipGetLastError();
 }
@@ -778,22 +845,30 @@ hipError_t launch_sieve(const SieveArgs& a, int rg, int nbv, int grid, int block
 }
 
 hipError_t launch_deep(const DeepArgs& a, u32 max_entries, hipStream_t s) {
-    u32 blocks = (max_entries + 255) / 256;
+    (void)hipGetLastError();
+    u32 blocks = (max_entries + 3) / 4;
     if (blocks < 1) blocks = 1;
-    if (blocks > 4096) blocks = 4096;
+    if (blocks > 2048) blocks = 2048;
     hipLaunchKernelGGL(deep_kernel, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_prove(const ProveArgs& a, u32 max_entries, hipStream_t s) {
+    (void)hipGetLastError();
     u32 blocks = (max_entries + 127) / 128;
     if (blocks < 1) blocks = 1;
     if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, a.chain_count, (size_t)1);
     hipLaunchKernelGGL(prove_kernel, dim3(blocks), dim3(128), 0, s, a);
+    u32 cblocks = (max_entries + 3) / 4;
+    if (cblocks < 1) cblocks = 1;
+    if (cblocks > 2048) cblocks = 2048;
+    hipLaunchKernelGGL(chain_kernel, dim3(cblocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_zero(u32* p, size_t n_words, hipStream_t s) {
+    (void)hipGetLastError();
     u32 blocks = (u32)((n_words + 255) / 256);
     if (blocks < 1) blocks = 1;
     if (blocks > 1024) blocks = 1024;

@@ -26,6 +26,8 @@
 
 namespace {
 
+constexpr uint32_t kEagerCopy = 1024;   // survivors copied back with every slice
+
 double now_s() {
     using namespace std::chrono;
     return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -112,6 +114,13 @@ static int collect_lane(ccs_server* s, int li) {
         if (!L.busy) continue;
         L.busy = false;
         uint32_t n = std::min(*L.h_count, s->p.max_slice_survivors);
+        if (n > kEagerCopy) {   // the rest of a large slice's survivors
+            if (hipMemcpy(L.h_surv + kEagerCopy, L.d_surv + kEagerCopy, (size_t)(n - kEagerCopy) * sizeof(ccs_job_survivor),
+                          hipMemcpyDeviceToHost) != hipSuccess ||
+                hipMemcpy(L.h_res + kEagerCopy, L.d_res + kEagerCopy, (size_t)(n - kEagerCopy) * sizeof(ccs_chain_result),
+                          hipMemcpyDeviceToHost) != hipSuccess)
+                return CCS_ERR_HIP;
+        }
         for (uint32_t i = 0; i < n; ++i) {
             const ccs_job_survivor sv = L.h_surv[i];
             const Entry& e = L.entries[sv.job_idx];
@@ -225,10 +234,12 @@ static void server_loop(ccs_server* s) {
             }
             if (!rc && hipMemcpyAsync(L.h_count, L.d_count, 4, hipMemcpyDeviceToHost, s->streams[li]) != hipSuccess)
                 rc = CCS_ERR_HIP;
-            if (!rc && hipMemcpyAsync(L.h_surv, L.d_surv, (size_t)s->p.max_slice_survivors * sizeof(ccs_job_survivor),
+            // survivors are rare: copy a bounded prefix now, the rest (if any) at collection
+            const size_t pre = std::min<uint32_t>(kEagerCopy, s->p.max_slice_survivors);
+            if (!rc && hipMemcpyAsync(L.h_surv, L.d_surv, pre * sizeof(ccs_job_survivor),
                                       hipMemcpyDeviceToHost, s->streams[li]) != hipSuccess)
                 rc = CCS_ERR_HIP;
-            if (!rc && hipMemcpyAsync(L.h_res, L.d_res, (size_t)s->p.max_slice_survivors * sizeof(ccs_chain_result),
+            if (!rc && hipMemcpyAsync(L.h_res, L.d_res, pre * sizeof(ccs_chain_result),
                                       hipMemcpyDeviceToHost, s->streams[li]) != hipSuccess)
                 rc = CCS_ERR_HIP;
             if (rc) { server_fail(s, rc, "slice launch"); return; }
@@ -255,7 +266,7 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
     if (!s->p.max_active) s->p.max_active = 1024;
     if (!s->p.surv_per_job) s->p.surv_per_job = 16;
     if (!s->p.max_slice_survivors) s->p.max_slice_survivors = 1u << 16;
-    if (!s->p.slice_candidates) s->p.slice_candidates = 1ull << 31;
+    if (!s->p.slice_candidates) s->p.slice_candidates = 1ull << 35;
     if (!s->p.min_chunk) s->p.min_chunk = 1u << 18;
     if (!s->p.max_chunk) s->p.max_chunk = 1ull << 32;
     if (!s->p.mem_budget_bytes) s->p.mem_budget_bytes = 256ull << 20;

@@ -128,11 +128,14 @@ typedef struct {
 
 /* Kernel tiling knobs (chosen by the autotuner, or set explicitly). */
 typedef struct {
-    uint32_t n_dense;          /* filter primes handled by the dense pattern stage  */
+    uint32_t n_dense;          /* filter primes handled by the dense pattern stage (<= 128, at most 64 patterns) */
     uint32_t n_tierb;          /* next primes handled by stage B (<= 256)           */
     uint32_t rows_per_group;   /* 1, 2 or 4                                         */
     uint32_t unit_j;           /* candidates per work unit, multiple of 2048        */
     uint32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu             */
+    uint32_t pair_max;         /* fold adjacent dense primes into one pattern when  */
+                               /*   q1*q2 <= pair_max (0 = one prime per pattern)   */
+    uint32_t waves_per_block;  /* 1, 2, 4, 8 or 16 wave64s share one LDS table copy */
 } ccs_tuning;
 
 /* ---- context (replaces the module statics + streams of v13.cu:268-327) ---- */
