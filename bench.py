@@ -68,6 +68,102 @@ def cpu_baseline(name, t0, seconds):
                       f"threads where the reference v34 binary ran at 4.07 G/s)"}
 
 
+def measure_batched(ccs, WL, ctx, n_jobs=4096, tune_trials=40):
+    """Config 4: n_jobs random admissible classes mod 37# at depth 18, log-uniform lengths
+    in [2^20, 2^32]; one persistent work-stealing launch (and its HIP graph) against the
+    plain per-shape launch loop over the same jobs and tuning."""
+    s = WL.make_shape(ctx, "c4", classes=False)
+    jobs, W = WL.batched_jobs(n_jobs=n_jobs)
+    sample = [(a, u0, min(c, 1 << 24)) for a, u0, c in jobs[:512]]
+    tuned, _ = s.autotune_jobs(sample, max_trials=tune_trials)
+    js = ccs.JobSet(s, jobs)
+    cap = 1 << 20
+    d_out, d_cnt = ctx.dalloc(cap * 8), ctx.dalloc(64)
+
+    def best_of(fn, reps=2):
+        fn()
+        ctx.synchronize()
+        best = 1e30
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ctx.synchronize()
+            best = min(best, time.perf_counter() - t)
+        return best
+
+    tb = best_of(lambda: js.launch(d_out, cap, d_cnt))
+    nb = js.stats(d_cnt)["survivors"]
+    tp = best_of(lambda: js.launch_each(d_out, cap, d_cnt))
+    n_p = js.stats(d_cnt)["survivors"]
+    g = js.graph(d_out, cap, d_cnt)
+    tg = best_of(lambda: js.graph_launch(g))
+    ng = js.stats(d_cnt)["survivors"]
+    js.close()
+    ctx.dfree(d_out)
+    ctx.dfree(d_cnt)
+    return {"workload": "config 4: 4096 admissible classes mod 37#, depth 18, lengths log-uniform in "
+                        "[2^20, 2^32], candidates in [2^84, 2^90), filter primes 41..65521",
+            "jobs": n_jobs, "candidates": js.candidates, "batched_s": tb, "graph_s": tg, "plain_loop_s": tp,
+            "batched_candidates_per_s": js.candidates / tb, "graph_candidates_per_s": js.candidates / tg,
+            "plain_loop_candidates_per_s": js.candidates / tp, "speedup_batched_vs_plain": tp / tb,
+            "speedup_graph_vs_plain": tp / tg, "survivors_equal": nb == n_p == ng, "survivors": nb,
+            "tuning": tuned}
+
+
+def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=30):
+    """Config 5: a Poisson stream of search jobs (depth U{12..20}, width U{64,80,96}, random
+    admissible class mod 31#, length log-uniform in [2^24, 2^34]) at `load` times the server's
+    measured capacity; p50 / p99 latency from submission to completion."""
+    import numpy as np
+    shapes = {d: WL.make_shape(ctx, f"c5d{d}", classes=False) for d in range(12, 21)}
+    tune_reqs = [r for r in WL.served_requests(n_jobs=600, seed=0x7E) if r["depth"] == 16][:256]
+    tuned, _ = shapes[16].autotune_jobs([(r["a"], r["u0"], min(r["count"], 1 << 26)) for r in tune_reqs],
+                                        max_trials=tune_trials)
+    for d, sh in shapes.items():
+        if d != 16:
+            sh.set_tuning(**tuned)
+    cal = WL.served_requests(n_jobs=400, seed=0xCA1)
+    srv = ccs.Server(ctx, shapes)
+    t = time.perf_counter()
+    for r in cal:
+        srv.submit(r["depth"], r["a"], r["u0"], r["count"])
+    srv.drain()
+    capacity = sum(r["count"] for r in cal) / (time.perf_counter() - t)
+    while srv.poll():
+        pass
+    reqs = WL.served_requests(n_jobs=n_jobs)
+    mean = sum(r["count"] for r in reqs) / len(reqs)
+    lam = load * capacity / mean
+    gaps = np.random.default_rng(0xC5).exponential(1.0 / lam, size=len(reqs))
+    t0 = time.perf_counter()
+    tn = t0
+    for r, gap in zip(reqs, gaps):
+        tn += gap
+        d = tn - time.perf_counter()
+        if d > 0:
+            time.sleep(d)
+        srv.submit(r["depth"], r["a"], r["u0"], r["count"], tag=r["id"], bits=r["bits"])
+    srv.drain()
+    wall = time.perf_counter() - t0
+    done = []
+    while True:
+        got = srv.poll()
+        if not got:
+            break
+        done += got
+    st = srv.stats()
+    srv.shutdown()
+    lat = np.array([d["t_done"] - d["t_submit"] for d in done])
+    total = sum(r["count"] for r in reqs)
+    return {"workload": "config 5: Poisson stream, depth U{12..20}, width U{64,80,96}, classes mod 31#, "
+                        "lengths log-uniform in [2^24, 2^34]",
+            "jobs": len(done), "load": load, "arrival_rate_per_s": lam, "capacity_candidates_per_s": capacity,
+            "aggregate_candidates_per_s": total / wall, "wall_s": wall,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_ms": float(lat.mean() * 1e3), "slices": st["slices"], "launches": st["launches"],
+            "tuning": tuned}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,7 +173,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="c2", choices=["c2", "reference", "c3"])
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default tuning)")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched and served measurements")
+    ap.add_argument("--served-jobs", type=int, default=10000)
     ap.add_argument("--tune-trials", type=int, default=40)
+    ap.add_argument("--tuning", default=None, help="JSON tuning to use instead of the autotuner")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this pipeline shards by disjoint tile ranges with no "
@@ -114,7 +213,9 @@ def main():
         shape.filter_launch(t0, tc, d_surv, cap, d_count)
         shape.prove_launch(t0, d_surv, d_count, pcap, d_res, bits=bits)
 
-    if not args.no_tune:
+    if args.tuning:
+        shape.set_tuning(**json.loads(args.tuning))
+    elif not args.no_tune:
         # autotuner on a 1/8 sample of the same range (setup work, not timed)
         t_tune = time.perf_counter()
         tuned, ms = shape.autotune(t0, max(6, (tc // 8) // 6 * 6), max_trials=args.tune_trials)
@@ -172,6 +273,11 @@ def main():
         "primes_per_step": int(((res["flags"] & ccs.F_PRIME) != 0).sum()),
         "analytic_survival": surv_frac,
     }
+    if not args.no_extras:
+        log("[bench] batched vs plain loop ...")
+        out["batched"] = measure_batched(ccs, WL, ctx)
+        log("[bench] served ...")
+        out["served"] = measure_served(ccs, WL, ctx, n_jobs=args.served_jobs)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(name, t0, args.cpu_seconds)

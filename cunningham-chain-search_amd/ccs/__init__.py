@@ -146,6 +146,16 @@ class Shape:
         self._refresh_info()
         return t.as_dict(), float(ms.value)
 
+    def autotune_jobs(self, jobs, max_trials=40):
+        """Pick the fastest tuning on the batched launch over sample jobs (ccs_autotune_jobs)."""
+        jarr = jobs_array(jobs)
+        t = L.Tuning()
+        ms = C.c_float(0)
+        L.check(self.lib.ccs_autotune_jobs(self.ctx.h, self.h, jarr.ctypes.data, len(jarr), max_trials,
+                                           C.byref(t), C.byref(ms)), "ccs_autotune_jobs")
+        self._refresh_info()
+        return t.as_dict(), float(ms.value)
+
     def bases(self):
         out = np.zeros(self.n_bases, np.uint64)
         L.check(self.lib.ccs_shape_bases(self.h, out.ctypes.data, self.n_bases), "bases")

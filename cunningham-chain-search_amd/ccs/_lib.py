@@ -28,7 +28,7 @@ EXPORTS = [
     "ccs_jobset_stats", "ccs_graph_create", "ccs_graph_launch", "ccs_graph_free",
     "ccs_prove_jobset_launch", "ccs_autotune", "ccs_jobset_update",
     "ccs_server_create", "ccs_server_submit", "ccs_server_poll", "ccs_server_job_results",
-    "ccs_server_drain", "ccs_server_stats", "ccs_server_shutdown",
+    "ccs_server_drain", "ccs_server_stats", "ccs_server_shutdown", "ccs_autotune_jobs",
 ]
 
 
@@ -196,6 +196,7 @@ def load():
     L.ccs_graph_free.restype = None
     L.ccs_prove_jobset_launch.argtypes = [vp, vp, vp, vp, C.c_uint32, C.POINTER(ProveParams), vp, vp]
     L.ccs_autotune.argtypes = [vp, vp, U128, C.c_uint32, C.c_uint32, C.POINTER(Tuning), C.POINTER(C.c_float)]
+    L.ccs_autotune_jobs.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, C.POINTER(Tuning), C.POINTER(C.c_float)]
     L.ccs_jobset_update.argtypes = [vp, vp, vp, C.c_uint32, vp]
     L.ccs_server_create.argtypes = [vp, vp, vp, C.c_uint32, C.POINTER(ServerParams), C.POINTER(vp)]
     L.ccs_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -734,6 +735,7 @@ static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, 
     da.mid_count = x.counters + 0;
     da.mid_cap = x.mid_cap;
     da.out_count = d_count;
+    da.stats = x.stats;
     prof_begin(c, 1, st);
     e = launch_deep(da, x.mid_cap, st);
     if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("deep launch: ") + hipGetErrorString(e));
@@ -803,12 +805,12 @@ static int ensure_buf(void** p, u64* have, u64 want) {
 }
 
 static int collect_stats(ccs_ctx* c, u32 count, const Scratch& x) {
-    unsigned long long st[2] = {0, 0};
+    unsigned long long st[3] = {0, 0, 0};
     u32 midc = 0;
     HIPCHK(hipMemcpy(st, x.stats, sizeof(st), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(&midc, x.counters, 4, hipMemcpyDeviceToHost));
     c->last.alive_dense = st[0];
-    c->last.alive_tierb = midc;
+    c->last.alive_tierb = st[2];
     c->last.survivors = count;
     float ms = 0;
     if (c->last_timed) {
@@ -1006,6 +1008,7 @@ struct ccs_jobset {
     u64* d_prefix = nullptr;
     std::vector<u64> prefix;
     u64 n_units = 0, candidates = 0;
+    u32 unit_j = 0;                 // the shape's unit size when the unit prefix was built
     u32 cap_jobs = 0;
     JobRec* staging = nullptr;      // pinned upload staging of ccs_jobset_update
     u64* staging_prefix = nullptr;
@@ -1055,6 +1058,7 @@ extern "C" int ccs_jobset_create(ccs_ctx* c, const ccs_shape* s, const ccs_job* 
     }
     js->prefix[n_jobs] = js->n_units;
     js->cap_jobs = n_jobs;
+    js->unit_j = s->tune.unit_j;
     u64 want = (u64)(js->candidates * s->est_survival_b * 2.0) + (1u << 20);
     if (want > (1u << 28)) want = 1u << 28;
     hipError_t e = hipMalloc((void**)&js->d_jobs, (size_t)n_jobs * sizeof(JobRec));
@@ -1108,6 +1112,7 @@ extern "C" int ccs_jobset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jo
         units += (h_jobs[i].count + s->tune.unit_j - 1) / s->tune.unit_j;
     }
     js->prefix[n_jobs] = js->staging_prefix[n_jobs] = units;
+    js->unit_j = s->tune.unit_j;
     js->n_units = units;
     js->candidates = cand;
     js->n_jobs = n_jobs;
@@ -1115,6 +1120,12 @@ extern "C" int ccs_jobset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jo
     HIPCHK(hipMemcpyAsync(js->d_jobs, js->staging, (size_t)n_jobs * sizeof(JobRec), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(js->d_prefix, js->staging_prefix, ((size_t)n_jobs + 1) * 8, hipMemcpyHostToDevice, st));
     HIPCHK(hipEventRecord(js->staged, st));
+    return CCS_OK;
+}
+
+static int jobset_check(const ccs_jobset* js) {
+    if (js->unit_j != js->shape->tune.unit_j)
+        return fail(CCS_ERR_INVALID, "the shape's tuning changed since the job set was built: rebuild or update it");
     return CCS_OK;
 }
 
@@ -1149,6 +1160,7 @@ extern "C" int ccs_jobset_launch(ccs_ctx* c, ccs_jobset* js, ccs_job_survivor* d
     int rc = ctx_check(c);
     if (rc) return rc;
     if (!js || !d_out || !d_count) return fail(CCS_ERR_INVALID, "null argument");
+    if ((rc = jobset_check(js))) return rc;
     SieveArgs sa;
     DeepArgs da;
     jobset_args(js, sa, da, d_out, cap);
@@ -1164,6 +1176,7 @@ extern "C" int ccs_jobset_launch_each(ccs_ctx* c, ccs_jobset* js, ccs_job_surviv
     int rc = ctx_check(c);
     if (rc) return rc;
     if (!js || !d_out || !d_count) return fail(CCS_ERR_INVALID, "null argument");
+    if ((rc = jobset_check(js))) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     jobset_stats_begin(c, js);
     if (launch_zero(d_count, 1, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
@@ -1214,6 +1227,7 @@ extern "C" int ccs_graph_create(ccs_ctx* c, ccs_jobset* js, ccs_job_survivor* d_
     int rc = ctx_check(c);
     if (rc) return rc;
     if (!js || !d_out || !d_count || !out) return fail(CCS_ERR_INVALID, "null argument");
+    if ((rc = jobset_check(js))) return rc;
     *out = nullptr;
     ccs_graph* g = new ccs_graph();
     g->device = c->device;
@@ -1405,14 +1419,61 @@ static int time_filter(ccs_ctx* c, ccs_shape* s, ccs_u128 ts, uint32_t tile_coun
     return CCS_OK;
 }
 
+// time the batched launch over a sample of jobs (a job set is rebuilt for every tuning,
+// since its unit prefix depends on unit_j)
+static int time_jobs(ccs_ctx* c, ccs_shape* s, const ccs_job* jobs, u32 n_jobs, int reps, float* ms_out) {
+    int rc = ensure_buf(&c->out, &c->out_bytes, (u64)(1u << 20) * sizeof(ccs_job_survivor) + 256);
+    if (rc) return rc;
+    u32* d_count = (u32*)((char*)c->out + (u64)(1u << 20) * sizeof(ccs_job_survivor));
+    ccs_jobset* js = nullptr;
+    rc = ccs_jobset_create(c, s, jobs, n_jobs, &js);
+    if (rc) return rc;
+    float best = 1e30f;
+    for (int r = 0; r < reps + 1 && !rc; ++r) {
+        c->timed = true;
+        rc = ccs_jobset_launch(c, js, (ccs_job_survivor*)c->out, 1u << 20, d_count, c->stream);
+        c->timed = false;
+        if (rc) break;
+        float a = 0, b = 0;
+        if (hipStreamSynchronize(c->stream) != hipSuccess || hipEventElapsedTime(&a, c->ev[0], c->ev[1]) != hipSuccess ||
+            hipEventElapsedTime(&b, c->ev[1], c->ev[2]) != hipSuccess) {
+            rc = fail(CCS_ERR_HIP, "timing");
+            break;
+        }
+        if (r > 0 && a + b < best) best = a + b;
+    }
+    ccs_jobset_free(js);
+    *ms_out = best;
+    return rc;
+}
+
+static int autotune_impl(ccs_ctx* c, ccs_shape* s, const std::function<int(float*)>& timer, uint32_t max_trials,
+                         ccs_tuning* best_out, float* best_ms_out);
+
 extern "C" int ccs_autotune(ccs_ctx* c, ccs_shape* s, ccs_u128 ts, uint32_t tile_count, uint32_t max_trials,
                             ccs_tuning* best_out, float* best_ms_out) {
     int rc = ctx_check(c);
     if (rc) return rc;
     if (!s || s->classes.empty()) return fail(CCS_ERR_INVALID, "autotune needs a shape with classes");
+    return autotune_impl(c, s, [&](float* ms) { return time_filter(c, s, ts, tile_count, 2, ms); }, max_trials,
+                         best_out, best_ms_out);
+}
+
+extern "C" int ccs_autotune_jobs(ccs_ctx* c, ccs_shape* s, const ccs_job* h_jobs, uint32_t n_jobs,
+                                 uint32_t max_trials, ccs_tuning* best_out, float* best_ms_out) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!s || !h_jobs || !n_jobs) return fail(CCS_ERR_INVALID, "autotune needs a shape and sample jobs");
+    return autotune_impl(c, s, [&](float* ms) { return time_jobs(c, s, h_jobs, n_jobs, 2, ms); }, max_trials,
+                         best_out, best_ms_out);
+}
+
+static int autotune_impl(ccs_ctx* c, ccs_shape* s, const std::function<int(float*)>& timer, uint32_t max_trials,
+                         ccs_tuning* best_out, float* best_ms_out) {
+    int rc;
     ccs_tuning best = s->tune;
     float best_ms = 0;
-    rc = time_filter(c, s, ts, tile_count, 2, &best_ms);
+    rc = timer(&best_ms);
     if (rc) return rc;
     u32 trials = 1;
     // candidate values per knob
@@ -1438,7 +1499,7 @@ extern "C" int ccs_autotune(ccs_ctx* c, ccs_shape* s, ccs_u128 ts, uint32_t tile
         int r2 = upload_tables(s);
         if (r2) return r2;
         float ms = 0;
-        r2 = time_filter(c, s, ts, tile_count, 2, &ms);
+        r2 = timer(&ms);
         if (r2) return r2;
         ++trials;
         if (ms < best_ms) { best_ms = ms; best = t; }
@@ -1461,7 +1522,7 @@ extern "C" int ccs_autotune(ccs_ctx* c, ccs_shape* s, ccs_u128 ts, uint32_t tile
         int r2 = upload_tables(s);
         if (r2) return r2;
         float ms = 0;
-        r2 = time_filter(c, s, ts, tile_count, 2, &ms);
+        r2 = timer(&ms);
         if (r2) return r2;
         ++trials;
         if (ms < best_ms) { best_ms = ms; best = t; }

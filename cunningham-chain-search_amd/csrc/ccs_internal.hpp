@@ -121,7 +121,8 @@ struct SieveArgs {
     MidRec* mid;
     u32 mid_cap;
     u32* mid_count;
-    unsigned long long* stats; // [0] alive after dense stage, [1] tested in stage B (may be null)
+    unsigned long long* stats; // [0] alive after dense stage, [1] tested in stage B, [2] stage-B
+                               // survivors accumulated by the deep kernel (may be null)
 };
 
 struct DeepArgs {
@@ -138,6 +139,7 @@ struct DeepArgs {
     void* out;                // plain: ccs_survivor[16 B]; jobs: ccs_job_survivor[8 B]
     u32 out_cap;
     u32* out_count;
+    unsigned long long* stats; // [2] += stage-B survivors of this launch (may be null)
 };
 
 // Prove: GMP confirmation engine semantics (v13.cu:1794-1924) per survivor.

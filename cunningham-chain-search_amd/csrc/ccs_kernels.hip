@@ -156,32 +156,50 @@ __device__ __forceinline__ bool resolve(const SieveArgs& A, u64 unit, u32 lane, 
 // K1: sieve kernel
 // ---------------------------------------------------------------------------
 
+// per-lane copies of the stage-B prime constants: lane l holds prime 64*i + l in slot i
+template <int NBV>
+struct TierBRegs {
+    u32 s[NBV];      // phase of the current unit
+    u32 q[NBV];
+    u32 off[NBV];    // LDS dword offset of the bitset
+    u32 rcp[NBV];    // f32 bits of 1/q
+};
+
 template <int NBV>
 __device__ __forceinline__ void stage_b(const SieveArgs& A, const u32* __restrict__ lds, u32* queue,
-                                        u32 cnt, const u32 (&sb)[NBV], u32 lane, const Unit& U,
+                                        u32 cnt, const TierBRegs<NBV>& R, u32 lane, const Unit& U,
                                         u64& cnt_b) {
     wave_sync();
     bool alive = lane < cnt;
-    u32 j = alive ? queue[lane] : 0u;
+    const u32 j = alive ? queue[lane] : 0u;
     cnt_b += alive ? 1 : 0;
 #pragma unroll
     for (int i = 0; i < NBV; ++i) {
         const u32 p0 = (u32)i * 64u;
         const u32 pend = A.nb > p0 ? min(64u, A.nb - p0) : 0u;
-        for (u32 pl = 0; pl < pend; ++pl) {
-            const u32 p = p0 + pl;
+        // four primes per step: their LDS probes are in flight together; the early exit is
+        // checked once per step (a dead candidate tested by a few more primes stays dead)
+        for (u32 pl = 0; pl < pend; pl += 4) {
             if (ballot(alive) == 0) break;
-            const u32 s = __builtin_amdgcn_readlane(sb[i], pl);
-            const TierBPrime tp = A.tierb[p];
-            if (alive) {
-                const u32 x = j + s;
-                const u32 qt = (u32)((float)x * tp.rcp);
-                u32 r = x - __umul24(qt, tp.q);
-                const u32 r1 = r - tp.q, r2 = r + tp.q;
-                r = min(r, min(r1, r2));
-                const u32 w = lds[tp.lds_off + (r >> 5)];
-                if ((w >> (r & 31)) & 1u) alive = false;
+            u32 w[4], sh[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const u32 l = min(pl + (u32)u, pend - 1u);
+                const u32 sv = __builtin_amdgcn_readlane(R.s[i], l);
+                const u32 q = __builtin_amdgcn_readlane(R.q[i], l);
+                const u32 off = __builtin_amdgcn_readlane(R.off[i], l);
+                const float rcp = __uint_as_float(__builtin_amdgcn_readlane(R.rcp[i], l));
+                const u32 x = j + sv;
+                const u32 qt = (u32)((float)x * rcp);
+                u32 r = x - __umul24(qt, q);
+                r = min(r, min(r - q, r + q));
+                w[u] = lds[off + (r >> 5)];
+                sh[u] = r & 31u;
             }
+            u32 kill = 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) kill |= (w[u] >> sh[u]) & 1u;
+            if (kill) alive = false;
         }
     }
     {
@@ -224,6 +242,18 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
         dstep = d.step;
     }
     u64 cnt_dense = 0, cnt_b = 0;
+    TierBRegs<NBV> R;
+#pragma unroll
+    for (int i = 0; i < NBV; ++i) {
+        const u32 p = (u32)i * 64u + lane;
+        TierBPrime tp;
+        tp.q = 1; tp.lds_off = 0; tp.rcp = 1.0f; tp.pad = 0;
+        if (p < A.nb) tp = A.tierb[p];
+        R.s[i] = 0;
+        R.q[i] = tp.q;
+        R.off[i] = tp.lds_off;
+        R.rcp[i] = __float_as_uint(tp.rcp);
+    }
 
     for (;;) {
         u64 unit = 0;
@@ -241,7 +271,7 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
         U.j0 = bcast32(U.j0);
 
         // phases: dense prime p held by lane p as an LDS bit address, stage-B prime
-        // (64*i + lane) held in sb[i]
+        // (64*i + lane) held in R.s[i]
         u32 sd = 0;
         if (lane < A.nd) {
             const DensePrime d = A.dense[lane];
@@ -253,11 +283,10 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
             }
             sd = doff * 32u + ph;
         }
-        u32 sb[NBV];
 #pragma unroll
         for (int i = 0; i < NBV; ++i) {
             const u32 p = (u32)i * 64u + lane;
-            sb[i] = p < A.nb ? phase(A.pmod[A.nd_primes + p], U.a, U.us_lo, U.us_hi) : 0u;
+            R.s[i] = p < A.nb ? phase(A.pmod[A.nd_primes + p], U.a, U.us_lo, U.us_hi) : 0u;
         }
 
         const u32 rows = (U.n + kRowBits - 1) / kRowBits;
@@ -266,7 +295,26 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
             u32 acc[RG];
 #pragma unroll
             for (int r = 0; r < RG; ++r) acc[r] = 0;
-            for (u32 p = 0; p < A.nd; ++p) {
+            // four patterns per step so their LDS reads are in flight together
+            u32 p = 0;
+            for (; p + 4 <= A.nd; p += 4) {
+                u32 ba[4], lo[4][RG], hi[4][RG];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    ba[u] = __builtin_amdgcn_readlane(sd, p + (u32)u);
+                    const u32* src = lds + (ba[u] >> 5) + lane;
+#pragma unroll
+                    for (int r = 0; r < RG; ++r) {
+                        lo[u][r] = src[64 * r];
+                        hi[u][r] = src[64 * r + 1];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int r = 0; r < RG; ++r) acc[r] |= __builtin_amdgcn_alignbit(hi[u][r], lo[u][r], ba[u] & 31u);
+            }
+            for (; p < A.nd; ++p) {
                 const u32 ba = __builtin_amdgcn_readlane(sd, p);
                 const u32* src = lds + (ba >> 5) + lane;
 #pragma unroll
@@ -314,7 +362,7 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                 }
                 qn += (u32)__popcll(m);
                 if (qn >= 64) {
-                    stage_b<NBV>(A, lds, queue, 64, sb, lane, U, cnt_b);
+                    stage_b<NBV>(A, lds, queue, 64, R, lane, U, cnt_b);
                     const u32 rest = qn - 64;
                     const u32 v = lane < rest ? queue[64 + lane] : 0u;
                     wave_sync();
@@ -323,7 +371,7 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                 }
             }
         }
-        if (qn) stage_b<NBV>(A, lds, queue, qn, sb, lane, U, cnt_b);
+        if (qn) stage_b<NBV>(A, lds, queue, qn, R, lane, U, cnt_b);
     }
 
     if (A.stats) {
@@ -614,8 +662,10 @@ __device__ __forceinline__ U2 cand_n(u64 W, u64 a, u64 u_lo, u64 u_hi) {
 // survivors are not serialised over thousands of dependent per-prime steps
 __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
     const u32 lane = lane_id();
-    u32 total = *A.mid_count;
+    const u32 all = *A.mid_count;
+    u32 total = all;
     if (total > A.mid_cap) total = A.mid_cap;
+    if (A.stats && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.stats[2], (unsigned long long)all);
     const u32 waves = gridDim.x * (blockDim.x >> 6);
     const u32 wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     for (u32 e = wid; e < total; e += waves) {

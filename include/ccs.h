@@ -169,6 +169,9 @@ int ccs_shape_get_tuning(const ccs_shape* shape, ccs_tuning* out);
  * range; leaves the fastest tuning set on the shape.  At most max_trials timed trials. */
 int ccs_autotune(ccs_ctx* ctx, ccs_shape* shape, ccs_u128 tile_start, uint32_t tile_count,
                  uint32_t max_trials, ccs_tuning* best, float* best_ms);
+/* The same search timed on the batched launch over a sample of class-form jobs. */
+int ccs_autotune_jobs(ccs_ctx* ctx, ccs_shape* shape, const ccs_job* h_jobs, uint32_t n_jobs,
+                      uint32_t max_trials, ccs_tuning* best, float* best_ms);
 uint64_t ccs_count_candidates(const ccs_shape* shape, ccs_u128 tile_start, uint32_t tile_count);
 
 /* ---- plain filter: K1 over tiles [tile_start, tile_start + tile_count), all bases

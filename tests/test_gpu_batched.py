@@ -95,3 +95,26 @@ def test_autotune_keeps_results(ctx, oracle):
     assert sorted(zip(g["tile_idx"].tolist(), g["base_idx"].tolist(), g["k_offset"].tolist())) == \
         sorted((int(x[2]), int(x[0]), int(x[1])) for x in o_surv)
     assert (st["candidates"], st["alive_dense"], st["alive_tierb"]) == tuple(counts)
+
+
+def test_autotune_jobs_keeps_results(ctx, oracle):
+    jobs, W = WL.batched_jobs(n_jobs=16, seed=0xB7, lo_exp=12, hi_exp=18)
+    s = WL.make_shape(ctx, "c4", classes=False, depth=9)
+    say("[autotune] job-set form, 6 trials")
+    tuning, ms = s.autotune_jobs(jobs, max_trials=6)
+    assert ms > 0 and tuning == s.tuning
+    g, st = s.batch(jobs)
+    want, alive = oracle_jobs(oracle, jobs, W, 9, [st["dense_end"], st["tierb_end"], len(s.filter_primes)])
+    assert sorted(zip(g["job_idx"].tolist(), g["u_off"].tolist())) == want
+    assert (st["candidates"], st["alive_dense"], st["alive_tierb"], st["survivors"]) == tuple(alive)
+
+
+def test_jobset_rejects_stale_tuning(ctx):
+    jobs, W = WL.batched_jobs(n_jobs=4, seed=0xB8, lo_exp=10, hi_exp=12)
+    s = WL.make_shape(ctx, "c4", classes=False)
+    js = ccs.JobSet(s, jobs)
+    s.set_tuning(unit_j=s.tuning["unit_j"] * 2)
+    d_out, d_cnt = ctx.dalloc(1 << 16), ctx.dalloc(64)
+    with pytest.raises(ccs.CcsError):
+        js.launch(d_out, 1 << 12, d_cnt)
+    js.close()
