@@ -171,6 +171,18 @@ class Shape:
                                            r.ctypes.data, n), "classes")
         return a, b, k, r
 
+    def tile_range(self, bits, prefix=None, lanes=0, lane_id=0):
+        """[tile_min, tile_max) for --bits, --prefix 0b..., --prefix-lanes (ccs_tile_range)."""
+        pv, pb = 0, 0
+        if prefix:
+            if not prefix.startswith("0b"):
+                raise ValueError("prefix must be binary (0b...)")
+            pv, pb = int(prefix[2:], 2), len(prefix) - 2
+        lo, hi = L.U128(), L.U128()
+        L.check(self.lib.ccs_tile_range(self.h, bits, L.U128.of(pv), pb, lanes, lane_id, C.byref(lo), C.byref(hi)),
+                "ccs_tile_range")
+        return lo.lo | (lo.hi << 64), hi.lo | (hi.hi << 64)
+
     def count_candidates(self, tile_start, tile_count):
         return int(self.lib.ccs_count_candidates(self.h, L.U128.of(tile_start), tile_count))
 

@@ -29,6 +29,7 @@ EXPORTS = [
     "ccs_prove_jobset_launch", "ccs_autotune", "ccs_jobset_update",
     "ccs_server_create", "ccs_server_submit", "ccs_server_poll", "ccs_server_job_results",
     "ccs_server_drain", "ccs_server_stats", "ccs_server_shutdown", "ccs_autotune_jobs",
+    "ccs_tile_range",
 ]
 
 
@@ -196,6 +197,8 @@ def load():
     L.ccs_graph_free.restype = None
     L.ccs_prove_jobset_launch.argtypes = [vp, vp, vp, vp, C.c_uint32, C.POINTER(ProveParams), vp, vp]
     L.ccs_autotune.argtypes = [vp, vp, U128, C.c_uint32, C.c_uint32, C.POINTER(Tuning), C.POINTER(C.c_float)]
+    L.ccs_tile_range.argtypes = [vp, C.c_int32, U128, C.c_int32, C.c_uint32, C.c_uint32, C.POINTER(U128),
+                                 C.POINTER(U128)]
     L.ccs_autotune_jobs.argtypes = [vp, vp, vp, C.c_uint32, C.c_uint32, C.POINTER(Tuning), C.POINTER(C.c_float)]
     L.ccs_jobset_update.argtypes = [vp, vp, vp, C.c_uint32, vp]
     L.ccs_server_create.argtypes = [vp, vp, vp, C.c_uint32, C.POINTER(ServerParams), C.POINTER(vp)]

@@ -1579,3 +1579,56 @@ static int autotune_impl(ccs_ctx* c, ccs_shape* s, const std::function<int(float
     if (best_ms_out) *best_ms_out = best_ms;
     return CCS_OK;
 }
+
+// ---------------------------------------------------------------------------
+// campaign ranges: the tiles whose candidates have `bits` bits, narrowed to a binary
+// prefix and split into lanes (v13.cu:450-518 compute_prefix_tile_range, :3474-3573)
+// ---------------------------------------------------------------------------
+
+extern "C" int ccs_tile_range(const ccs_shape* s, int32_t bits, ccs_u128 prefix_value, int32_t prefix_bits,
+                              uint32_t lanes, uint32_t lane_id, ccs_u128* tile_min, ccs_u128* tile_max) {
+    if (!s || !tile_min || !tile_max) return fail(CCS_ERR_INVALID, "null argument");
+    if (bits < 21 || bits > 127) return fail(CCS_ERR_INVALID, "bits must be in 21..127");
+    if (s->bases.empty()) return fail(CCS_ERR_INVALID, "shape has no bases");
+    const u128 M = s->M, WP = s->WP;
+    const u128 min_base = s->bases.front(), max_base = s->bases.back();
+    // full bit range: conservative k bounds with the largest / smallest base
+    const u128 n_min = (u128)1 << (bits - 1), n_max = ((u128)1 << bits) - 1;
+    u128 tmin = ((n_min - max_base) / M) / WP;
+    u128 tmax = ((n_max - min_base) / M) / WP + 1;
+    if (prefix_bits > 0) {
+        if (prefix_bits > bits) return fail(CCS_ERR_INVALID, "prefix longer than bits");
+        const u128 pv = mk128(prefix_value);
+        if (prefix_bits < 127 && (pv >> (prefix_bits - 1)) != 1)
+            return fail(CCS_ERR_INVALID, "binary prefix must start with 1 and have prefix_bits bits");
+        const int shift = bits - prefix_bits;
+        u128 cmin = pv << shift, cmax = ((pv + 1) << shift) - 1;
+        if (cmax < n_min || cmin > n_max) return fail(CCS_ERR_INVALID, "prefix does not overlap the bit range");
+        if (cmin < n_min) cmin = n_min;
+        if (cmax > n_max) cmax = n_max;
+        const u128 kmin = cmin > max_base ? (cmin - max_base + M - 1) / M : 0;
+        const u128 kmax = (cmax - min_base) / M;
+        if (kmin > kmax) return fail(CCS_ERR_INVALID, "prefix produces an empty k range");
+        u128 pmin = kmin / WP, pmax = kmax / WP + 1;
+        if (pmin < tmin) pmin = tmin;
+        if (pmax > tmax) pmax = tmax;
+        if (pmin >= pmax) return fail(CCS_ERR_INVALID, "prefix produces an empty tile range");
+        tmin = pmin;
+        tmax = pmax;
+        if (lanes > 0) {
+            if (lane_id >= lanes) return fail(CCS_ERR_INVALID, "lane_id must be < lanes");
+            const u128 total = tmax - tmin, size = total / lanes, rem = total - size * lanes;
+            u128 start = tmin + (u128)lane_id * size + (lane_id < rem ? lane_id : rem);
+            u128 end = start + size + (lane_id < rem ? 1 : 0);
+            tmin = start;
+            tmax = end;
+        }
+    } else if (lanes > 0) {
+        return fail(CCS_ERR_INVALID, "lanes need a prefix");
+    }
+    tile_min->lo = (u64)tmin;
+    tile_min->hi = (u64)(tmin >> 64);
+    tile_max->lo = (u64)tmax;
+    tile_max->hi = (u64)(tmax >> 64);
+    return CCS_OK;
+}

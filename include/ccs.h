@@ -174,6 +174,13 @@ int ccs_autotune_jobs(ccs_ctx* ctx, ccs_shape* shape, const ccs_job* h_jobs, uin
                       uint32_t max_trials, ccs_tuning* best, float* best_ms);
 uint64_t ccs_count_candidates(const ccs_shape* shape, ccs_u128 tile_start, uint32_t tile_count);
 
+/* Campaign range: tiles [tile_min, tile_max) whose candidates have `bits` bits, narrowed
+ * to the binary prefix (prefix_bits = 0: none) and split into `lanes` (0 = no split);
+ * replaces compute_prefix_tile_range (v13.cu:450-518) and the --bits / --prefix-lanes
+ * range setup (v13.cu:3474-3573). */
+int ccs_tile_range(const ccs_shape* shape, int32_t bits, ccs_u128 prefix_value, int32_t prefix_bits,
+                   uint32_t lanes, uint32_t lane_id, ccs_u128* tile_min, ccs_u128* tile_max);
+
 /* ---- plain filter: K1 over tiles [tile_start, tile_start + tile_count), all bases
  *      (replaces prepare_batch v13.cu:1232-1259 + cc18_filter_kernel :1551-1714 and its
  *      launch :3769-3796).  Survivors go to d_out in unspecified order; *d_count (device)

@@ -255,3 +255,44 @@ class Shape:
     def n_of(self, tile_start, surv_row):
         b, k, t = int(surv_row[0]), int(surv_row[1]), int(surv_row[2])
         return ((tile_start + t) * self.WP + k) * self.M + int(self.bases()[b])
+
+
+def tile_range(M, WP, min_base, max_base, bits, prefix=None, lanes=0, lane_id=0):
+    """[tile_min, tile_max) of a campaign: the --bits range with floor k bounds
+    (v13.cu:3480-3491), narrowed to a binary prefix "0b..." by compute_prefix_tile_range
+    (v13.cu:450-518) and clamped to the full range (v13.cu:3524-3530), then split into
+    prefix lanes, the first (total mod lanes) lanes one tile longer (v13.cu:3532-3550).
+    Raises ValueError where the reference prints an error and exits."""
+    if not 21 <= bits <= 127:
+        raise ValueError("bits")
+    n_min, n_max = 1 << (bits - 1), (1 << bits) - 1
+    tmin = ((n_min - max_base) // M) // WP
+    tmax = ((n_max - min_base) // M) // WP + 1
+    if prefix is None:
+        if lanes:
+            raise ValueError("--prefix-lanes requires --prefix")
+        return tmin, tmax
+    if not prefix.startswith("0b") or len(prefix) < 3 or set(prefix[2:]) - {"0", "1"} or prefix[2] != "1":
+        raise ValueError("binary prefix")
+    pv, pb = int(prefix[2:], 2), len(prefix) - 2
+    if pb > bits:
+        raise ValueError("prefix longer than bits")
+    shift = bits - pb
+    cmin, cmax = pv << shift, ((pv + 1) << shift) - 1
+    if cmax < n_min or cmin > n_max:
+        raise ValueError("prefix does not overlap the bit range")
+    cmin, cmax = max(cmin, n_min), min(cmax, n_max)
+    kmin = -(-(cmin - max_base) // M) if cmin > max_base else 0
+    kmax = (cmax - min_base) // M
+    if kmin > kmax:
+        raise ValueError("empty k range")
+    pmin, pmax = max(kmin // WP, tmin), min(kmax // WP + 1, tmax)
+    if pmin >= pmax:
+        raise ValueError("empty tile range")
+    if lanes:
+        if not 0 <= lane_id < lanes:
+            raise ValueError("lane id")
+        size, rem = divmod(pmax - pmin, lanes)
+        start = pmin + lane_id * size + min(lane_id, rem)
+        return start, start + size + (1 if lane_id < rem else 0)
+    return pmin, pmax

@@ -85,6 +85,34 @@ def test_ragged_tile_ranges(ctx, oracle, ref, oref, off, tile_count):
     check_plain(ctx, oracle, "reference", t0, tile_count, shape=ref, oshape=oref)
 
 
+def test_tile_range_equals_oracle(oracle, ref):
+    """ccs_tile_range (campaign --bits / --prefix / --prefix-lanes) equals the oracle's
+    restatement of compute_prefix_tile_range, errors included."""
+    say("[plain] tile ranges")
+    b = ref.bases()
+    args = (ref.M, ref.WP, int(b.min()), int(b.max()))
+    rnd = np.random.default_rng(0x7A)
+    cases = [(89, None, 0, 0), (89, "0b101", 0, 0), (89, "0b101", 4, 2), (127, "0b11", 3, 0),
+             (60, "0b1" + "0" * 59, 0, 0), (21, None, 0, 0)]
+    for _ in range(300):
+        bits = int(rnd.integers(40, 128))
+        pb = int(rnd.integers(1, min(bits, 20) + 1))
+        pv = int(rnd.integers(1 << (pb - 1), 1 << pb))
+        lanes = int(rnd.integers(0, 9))
+        cases.append((bits, "0b" + format(pv, "b"), lanes, int(rnd.integers(0, max(lanes, 1)))))
+    for bits, prefix, lanes, lane in cases:
+        try:
+            want = oracle.tile_range(*args, bits, prefix, lanes, lane)
+        except ValueError:
+            with pytest.raises(ccs.CcsError):
+                ref.tile_range(bits, prefix, lanes, lane)
+            continue
+        assert ref.tile_range(bits, prefix, lanes, lane) == want, (bits, prefix, lanes, lane)
+    for bad in [(89, None, 2, 0), (89, "0b1", 2, 2), (10, None, 0, 0)]:
+        with pytest.raises(ccs.CcsError):
+            ref.tile_range(*bad)
+
+
 def test_empty_range(ref):
     say("[plain] empty range")
     surv, st = ref.filter(12345, 0)

@@ -372,3 +372,53 @@ def test_bitvector_filter_equals_per_candidate(oracle, name):
         b, cand = s.filter_bv(t0, tc, cap=1 << 18)
         assert cand == c[0]
         assert np.array_equal(a, b)
+
+
+def _lattice_n(t, k, base, WP=20677, M=1616615):
+    return (t * WP + k) * M + base
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_tile_range_bounds_every_prefixed_candidate(oracle, ref_shape, seed):
+    """compute_prefix_tile_range is conservative and tight: no candidate with the bit
+    width and prefix lies in a tile outside the range, and the range is at most two tiles
+    wider than the prefix's interval."""
+    rnd = random.Random(seed)
+    b = ref_shape.bases()
+    lo_b, hi_b = int(b.min()), int(b.max())
+    M, WP = ref_shape.M, ref_shape.WP
+    for _ in range(200):
+        bits = rnd.randint(40, 127)
+        pb = rnd.randint(1, min(bits, 14))
+        pv = rnd.randrange(1 << (pb - 1), 1 << pb)
+        prefix = "0b" + format(pv, "b")
+        tmin, tmax = oracle.tile_range(M, WP, lo_b, hi_b, bits, prefix)
+        shift = bits - pb
+        cmin = max(pv << shift, 1 << (bits - 1))
+        cmax = min(((pv + 1) << shift) - 1, (1 << bits) - 1)
+        assert _lattice_n(tmin, 0, hi_b) - M < cmin or tmin == 0       # last n of tile tmin-1
+        assert _lattice_n(tmax, 0, lo_b) > cmax                         # first n of tile tmax
+        assert tmax - tmin <= (cmax - cmin) // (M * WP) + 3
+    full = oracle.tile_range(M, WP, lo_b, hi_b, 89)
+    halves = [oracle.tile_range(M, WP, lo_b, hi_b, 89, p) for p in ("0b10", "0b11")]
+    assert halves[0][0] == full[0] and halves[1][1] == full[1]
+    assert halves[0][1] >= halves[1][0] >= halves[0][1] - 1     # adjacent, one shared tile at most
+    assert oracle.tile_range(M, WP, lo_b, hi_b, 89, "0b1") == full
+
+
+def test_tile_range_lanes_partition(oracle, ref_shape):
+    b = ref_shape.bases()
+    M, WP = ref_shape.M, ref_shape.WP
+    lo, hi = oracle.tile_range(M, WP, int(b.min()), int(b.max()), 89, "0b101")
+    for lanes in (1, 3, 4, 7, 1000):
+        parts = [oracle.tile_range(M, WP, int(b.min()), int(b.max()), 89, "0b101", lanes, i)
+                 for i in range(lanes)]
+        assert parts[0][0] == lo and parts[-1][1] == hi
+        assert all(parts[i][1] == parts[i + 1][0] for i in range(lanes - 1))
+        sizes = [e - s for s, e in parts]
+        assert max(sizes) - min(sizes) <= 1 and sizes == sorted(sizes, reverse=True)
+    for bad in [dict(bits=89, prefix="0b0"), dict(bits=89, prefix="101"), dict(bits=10),
+                dict(bits=89, lanes=2), dict(bits=89, prefix="0b1", lanes=2, lane_id=2),
+                dict(bits=40, prefix="0b1" + "0" * 40)]:
+        with pytest.raises(ValueError):
+            oracle.tile_range(M, WP, int(b.min()), int(b.max()), **bad)
