@@ -216,9 +216,10 @@ def main():
     if args.tuning:
         shape.set_tuning(**json.loads(args.tuning))
     elif not args.no_tune:
-        # autotuner on a 1/8 sample of the same range (setup work, not timed)
+        # autotuner on the same range (setup work, not timed): the unit size it picks
+        # depends on the run length per class, so a shorter sample would mislead it
         t_tune = time.perf_counter()
-        tuned, ms = shape.autotune(t0, max(6, (tc // 8) // 6 * 6), max_trials=args.tune_trials)
+        tuned, ms = shape.autotune(t0, tc, max_trials=args.tune_trials)
         log(f"[bench] autotune {time.perf_counter() - t_tune:.1f}s -> {tuned} ({ms:.2f} ms on the sample)")
     log(f"[bench] {name}: tiles={tc} candidates/step={cand} tuning={shape.tuning}")
     for _ in range(args.warmup):

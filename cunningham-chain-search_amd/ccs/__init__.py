@@ -175,7 +175,7 @@ class Shape:
         """[tile_min, tile_max) for --bits, --prefix 0b..., --prefix-lanes (ccs_tile_range)."""
         pv, pb = 0, 0
         if prefix:
-            if not prefix.startswith("0b"):
+            if prefix[:2] not in ("0b", "0B") or not set(prefix[2:]) <= {"0", "1"}:
                 raise ValueError("prefix must be binary (0b...)")
             pv, pb = int(prefix[2:], 2), len(prefix) - 2
         lo, hi = L.U128(), L.U128()

@@ -48,6 +48,7 @@ extern "C" const char* ccs_last_error(void) { return g_err.c_str(); }
 struct ccs_ctx {
     int device = 0;
     int n_cu = 256;
+    u32 max_lds_words = 16384;   // LDS per workgroup, in dwords
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
     // scratch
@@ -115,10 +116,32 @@ static u64 mod_pow(u64 b, u64 e, u64 m) {
     return (u64)r;
 }
 
+// Smallest float >= (1/q)(1 + 2^-23).  For x < 2^21 the f32 product x * rcp, rounded
+// to nearest, then truncated, is exactly floor(x / q): the relative excess of rcp plus
+// the product's rounding (at most 2^-22 + 2^-24) stays below 1/x, too small to reach the
+// next integer, and it is never below x / q.  Checked for every prime < 4096 and every
+// x < 2^21 by the test suite.
+static float rcp_up(u32 q) {
+    const double d = (1.0 / (double)q) * (1.0 + std::ldexp(1.0, -23));
+    float f = (float)d;
+    while ((double)f < d) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+// a^-1 mod m for gcd(a, m) = 1 (extended Euclid)
+static u64 mod_inv(u64 a, u64 m) {
+    int64_t t = 0, nt = 1, r = (int64_t)m, nr = (int64_t)(a % m);
+    while (nr) {
+        const int64_t qq = r / nr, tt = t - qq * nt, rr = r - qq * nr;
+        t = nt; nt = tt; r = nr; nr = rr;
+    }
+    return (u64)(t < 0 ? t + (int64_t)m : t);
+}
+
 static PrimeMod make_primemod(u32 q, u64 W) {
     PrimeMod pm;
     pm.q = q;
-    pm.winv = W ? (u32)mod_pow(W % q, q - 2, q) : 0;
+    pm.winv = W ? (u32)mod_inv(W % q, q) : 0;
     pm.c32 = (u32)(((u128)1 << 32) % q);
     pm.c64 = (u32)(((u128)1 << 64) % q);
     pm.c96 = (u32)(((u128)1 << 96) % q);
@@ -152,6 +175,7 @@ extern "C" int ccs_ctx_create(int device, ccs_ctx** out) {
         return fail(CCS_ERR_HIP, hipGetErrorString(e));
     }
     c->n_cu = prop.multiProcessorCount;
+    c->max_lds_words = (u32)(prop.sharedMemPerBlock / 4);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(CCS_ERR_HIP, "stream create");
@@ -248,6 +272,7 @@ extern "C" int ccs_synchronize(ccs_ctx* c) {
 
 struct ccs_shape {
     int device = 0;
+    u32 max_lds_words = 16384;
     std::vector<u32> crt, wheel, fp, stage_end;
     int target = 0, depth = 0;
     u32 pos0_max_q = 97;
@@ -309,8 +334,6 @@ static void free_device_tables(ccs_shape* s) {
     s->d_hash = nullptr;
 }
 
-static constexpr u32 kMaxLdsWords = (64u * 1024u) / 4u - 16u * kQueueCap;
-
 static u32 dense_ext_words(u32 q, u32 rg) { return ((q - 1) >> 5) + 64u * rg + 1u; }
 
 // dense slots: consecutive dense primes folded in pairs while q1*q2 <= pair_max
@@ -341,7 +364,7 @@ static void default_tuning(const ccs_shape* s, ccs_tuning* t) {
     t->n_dense = nd;
     t->n_tierb = nb;
     t->rows_per_group = 2;
-    t->unit_j = 65536;
+    t->unit_j = 262144;
     t->blocks_per_cu = 4;
     t->pair_max = 0;
     t->waves_per_block = 4;
@@ -369,7 +392,7 @@ static int validate_tuning(const ccs_shape* s, ccs_tuning* t) {
     u32 words = 0;
     for (auto sl : slots) words += dense_ext_words(slot_period(s, sl), t->rows_per_group);
     for (u32 i = 0; i < t->n_tierb; ++i) words += (s->fp[t->n_dense + i] + 31) / 32;
-    if (words > kMaxLdsWords) return fail(CCS_ERR_INVALID, "dense + stage-B tables exceed LDS");
+    if (words > s->max_lds_words) return fail(CCS_ERR_INVALID, "dense + stage-B tables exceed LDS");
     return CCS_OK;
 }
 
@@ -382,7 +405,7 @@ static int upload_tables(ccs_shape* s) {
     std::vector<u32> image;
     std::vector<DensePrime> dense(slots.empty() ? 1 : slots.size());
     std::vector<TierBPrime> tierb(nb ? nb : 1);
-    std::vector<PrimeMod> pmod(nd + nb ? nd + nb : 1);
+    std::vector<PrimeMod> pmod(slots.size() + nb ? slots.size() + nb : 1);
     // u-space kill sets: u + (a * W^-1) = f * W^-1 (mod q)  for f in F_q
     auto uset = [&](u32 i) {
         const u32 q = s->fp[i];
@@ -391,7 +414,6 @@ static int upload_tables(ccs_shape* s) {
         for (u32 f : s->forb[i]) kill[(u64)f * winv % q] = 1;
         return kill;
     };
-    for (u32 i = 0; i < nd; ++i) pmod[i] = make_primemod(s->fp[i], s->W);
     for (size_t k = 0; k < slots.size(); ++k) {
         const int i1 = slots[k].first, i2 = slots[k].second;
         const u32 q1 = s->fp[i1], q2 = i2 < 0 ? 1 : s->fp[i2];
@@ -403,12 +425,9 @@ static int upload_tables(ccs_shape* s) {
         d.q = Q;
         d.lds_off = (u32)image.size();
         d.step = (u32)(((u64)kRowBits * rg) % Q);
-        d.q1 = q1;
-        d.q2 = q2;
-        d.inv = i2 < 0 ? 0 : (u32)mod_pow(q1 % q2, q2 - 2, q2);
-        d.pm1 = (u32)i1;
-        d.pm2 = (u32)(i2 < 0 ? i1 : i2);
+        d.pad = 0;
         dense[k] = d;
+        pmod[k] = make_primemod(Q, s->W);
         for (u32 w = 0; w < words; ++w) {
             u32 v = 0;
             for (u32 b = 0; b < 32; ++b) {
@@ -424,7 +443,7 @@ static int upload_tables(ccs_shape* s) {
         TierBPrime b;
         b.q = q;
         b.lds_off = (u32)image.size();
-        b.rcp = 1.0f / (float)q;
+        b.rcp = rcp_up(q);
         b.pad = 0;
         tierb[i] = b;
         for (u32 w = 0; w < (q + 31) / 32; ++w) {
@@ -435,7 +454,7 @@ static int upload_tables(ccs_shape* s) {
             }
             image.push_back(v);
         }
-        pmod[nd + i] = make_primemod(q, s->W);
+        pmod[slots.size() + i] = make_primemod(q, s->W);
     }
     std::vector<DeepPrime> deep(nc ? nc : 1);
     std::vector<u32> hash((size_t)(nc ? nc : 1) * kMaxHashSlots, 0xffffffffu);
@@ -493,6 +512,7 @@ extern "C" int ccs_shape_build(ccs_ctx* c, const ccs_shape_params* p, const ccs_
     if (p->n_crt > 16 || p->n_wheel > 8) return fail(CCS_ERR_INVALID, "too many CRT/wheel primes");
     ccs_shape* s = new ccs_shape();
     s->device = c->device;
+    s->max_lds_words = c->max_lds_words;
     s->target = p->target;
     s->depth = p->depth;
     s->pos0_max_q = p->pos0_max_q;
@@ -694,7 +714,6 @@ static void fill_sieve_common(const ccs_shape* s, const Scratch& x, SieveArgs& a
     a.lds_image = s->d_lds_image;
     a.lds_words = s->lds_words;
     a.nd = s->nd;
-    a.nd_primes = s->nd_primes;
     a.nb = s->nb;
     a.unit_j = s->tune.unit_j;
     a.dense = s->d_dense;
@@ -720,7 +739,7 @@ static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, 
     const int nbv = s->nb ? (int)((s->nb + 63) / 64) : 1;
     const u32 wpb = s->tune.waves_per_block;
     const int block = 64 * (int)wpb;
-    const size_t lds = ((size_t)s->lds_words + (size_t)wpb * kQueueCap) * 4;
+    const size_t lds = (size_t)s->lds_words * 4;
     u64 want_grid = (u64)c->n_cu * s->tune.blocks_per_cu;
     const u64 blocks_needed = (n_units + wpb - 1) / wpb;
     if (want_grid > blocks_needed) want_grid = blocks_needed ? blocks_needed : 1;
@@ -1548,7 +1567,7 @@ static int autotune_impl(ccs_ctx* c, ccs_shape* s, const std::function<int(float
             t.rows_per_group = rg;
             if ((rc = try_tuning(t))) return rc;
         }
-        for (u32 uj : {16384u, 32768u, 65536u, 131072u, 262144u}) {
+        for (u32 uj : {32768u, 65536u, 131072u, 262144u, 524288u, 1048576u}) {
             ccs_tuning t = best;
             if (uj == t.unit_j) continue;
             t.unit_j = uj;
@@ -1594,10 +1613,11 @@ extern "C" int ccs_tile_range(const ccs_shape* s, int32_t bits, ccs_u128 prefix_
     const u128 min_base = s->bases.front(), max_base = s->bases.back();
     // full bit range: conservative k bounds with the largest / smallest base
     const u128 n_min = (u128)1 << (bits - 1), n_max = ((u128)1 << bits) - 1;
+    if (n_min <= max_base) return fail(CCS_ERR_INVALID, "bits too small for the lattice bases");
     u128 tmin = ((n_min - max_base) / M) / WP;
     u128 tmax = ((n_max - min_base) / M) / WP + 1;
     if (prefix_bits > 0) {
-        if (prefix_bits > bits) return fail(CCS_ERR_INVALID, "prefix longer than bits");
+        if (prefix_bits >= bits) return fail(CCS_ERR_INVALID, "prefix_bits must be < bits");
         const u128 pv = mk128(prefix_value);
         if (prefix_bits < 127 && (pv >> (prefix_bits - 1)) != 1)
             return fail(CCS_ERR_INVALID, "binary prefix must start with 1 and have prefix_bits bits");

@@ -16,16 +16,15 @@ typedef uint8_t u8;
 // (base, k, rho) with tile = 6u + rho, W = 6*WP*M and a = rho*WP*M + k*M + base.
 
 constexpr int kRowBits = 2048;      // one row = 64 lanes x 32 candidates
-constexpr int kQueueCap = 128;      // per-wave stage-B candidate queue (LDS)
 constexpr int kMaxDense = 64;       // dense primes held one per lane
 constexpr int kMaxHashSlots = 64;   // forbidden-residue hash per deep prime
 
-// Per-prime constants for exact residues of u64/u128 values: y = sum limb_k*2^(32k)
+// Per-modulus constants for exact residues of u64/u128 values: y = sum limb_k*2^(32k)
 // folded with c_k = 2^(32k) mod q (< 2^52 for q < 2^18), then reduced with a double
-// reciprocal.
+// reciprocal.  q is a prime, or the product of two primes for a paired dense slot.
 struct PrimeMod {
     u32 q;
-    u32 winv;       // W^-1 mod q
+    u32 winv;       // W^-1 mod q (W is prime to q)
     u32 c32, c64, c96;
     u32 pad;
     double rcp;     // 1.0 / q
@@ -34,16 +33,14 @@ struct PrimeMod {
 // Dense stage: kill pattern over u, periodic with period q, stored as an extended bit
 // string in LDS so that a row (64 lanes x 32 bits) starting at any phase s reads the two
 // dwords (s>>5)+lane and (s>>5)+lane+1 with no wrap-around.
-// A dense slot is one prime, or two primes q1*q2 folded into one pattern of period q1*q2
-// (bit x killed iff x mod q1 or x mod q2 is killed); its phase is the CRT combination of
-// the two primes' phases.
+// A dense slot is one prime, or two primes folded into one pattern of period q = q1*q2
+// (bit x killed iff x mod q1 or x mod q2 is killed).  The slot's phase is taken modulo q
+// directly (W is prime to q), with the slot's own PrimeMod record: pmod[slot].
 struct DensePrime {
     u32 q;          // period of the slot's pattern (q1 or q1*q2)
     u32 lds_off;    // dword offset of the extended pattern in LDS
     u32 step;       // (kRowBits * RG) mod q: phase advance per row group
-    u32 q1, q2;     // q2 = 1 for a single prime
-    u32 inv;        // q1^-1 mod q2
-    u32 pm1, pm2;   // indices of the primes' PrimeMod records
+    u32 pad;
 };
 
 // Stage B: per-candidate test with a q-bit forbidden bitset in LDS.
@@ -104,12 +101,11 @@ struct SieveArgs {
     const u32* lds_image;     // dense patterns + tier-B bitsets, copied to LDS at start
     u32 lds_words;
     u32 nd;                   // dense slots (one or two primes each)
-    u32 nd_primes;            // dense primes (PrimeMod records before the stage-B ones)
     u32 nb;                   // tier-B primes
     u32 unit_j;               // candidates (u values) per work unit
     const DensePrime* dense;
     const TierBPrime* tierb;
-    const PrimeMod* pmod;     // [nd + nb] residue constants for dense then tier-B primes
+    const PrimeMod* pmod;     // [nd + nb] residue constants: dense slot moduli, then tier-B primes
     // work
     u64 n_units;
     u64 unit_base;            // added to the cursor value: first unit of this launch

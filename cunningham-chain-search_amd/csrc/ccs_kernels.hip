@@ -160,66 +160,29 @@ __device__ __forceinline__ bool resolve(const SieveArgs& A, u64 unit, u32 lane, 
 template <int NBV>
 struct TierBRegs {
     u32 s[NBV];      // phase of the current unit
-    u32 q[NBV];
+    int nq[NBV];     // -q
     u32 off[NBV];    // LDS dword offset of the bitset
-    u32 rcp[NBV];    // f32 bits of 1/q
+    u32 rcp[NBV];    // f32 bits of the rounded-up 1/q
 };
 
+// Stage B for one candidate j (uniform across the wave): the stage-B primes are held one
+// per lane (prime 64*i + lane in slot i), so one probe of every lane tests 64 primes at
+// once.  x = j + s < 2^21 and the host's reciprocal is rounded up so that the truncated
+// f32 product is exactly floor(x / q); the probe reads bit r of the prime's
+// forbidden-residue bitset.  Most candidates die in the first group of 64 primes, so the
+// later groups are rarely evaluated.
 template <int NBV>
-__device__ __forceinline__ void stage_b(const SieveArgs& A, const u32* __restrict__ lds, u32* queue,
-                                        u32 cnt, const TierBRegs<NBV>& R, u32 lane, const Unit& U,
-                                        u64& cnt_b) {
-    wave_sync();
-    bool alive = lane < cnt;
-    const u32 j = alive ? queue[lane] : 0u;
-    cnt_b += alive ? 1 : 0;
+__device__ __forceinline__ bool stage_b_pass(const u32* __restrict__ lds, const TierBRegs<NBV>& R,
+                                             const u64 (&valid)[NBV], u32 j) {
 #pragma unroll
     for (int i = 0; i < NBV; ++i) {
-        const u32 p0 = (u32)i * 64u;
-        const u32 pend = A.nb > p0 ? min(64u, A.nb - p0) : 0u;
-        // four primes per step: their LDS probes are in flight together; the early exit is
-        // checked once per step (a dead candidate tested by a few more primes stays dead)
-        for (u32 pl = 0; pl < pend; pl += 4) {
-            if (ballot(alive) == 0) break;
-            u32 w[4], sh[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const u32 l = min(pl + (u32)u, pend - 1u);
-                const u32 sv = __builtin_amdgcn_readlane(R.s[i], l);
-                const u32 q = __builtin_amdgcn_readlane(R.q[i], l);
-                const u32 off = __builtin_amdgcn_readlane(R.off[i], l);
-                const float rcp = __uint_as_float(__builtin_amdgcn_readlane(R.rcp[i], l));
-                const u32 x = j + sv;
-                const u32 qt = (u32)((float)x * rcp);
-                u32 r = x - __umul24(qt, q);
-                r = min(r, min(r - q, r + q));
-                w[u] = lds[off + (r >> 5)];
-                sh[u] = r & 31u;
-            }
-            u32 kill = 0;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) kill |= (w[u] >> sh[u]) & 1u;
-            if (kill) alive = false;
-        }
+        const u32 x = j + R.s[i];
+        const int qt = (int)((float)x * __uint_as_float(R.rcp[i]));
+        const u32 r = (u32)(__mul24(qt, R.nq[i]) + (int)x);                 // x mod q
+        const u32 w = lds[R.off[i] + __builtin_amdgcn_ubfe(r, 5u, 11u)];
+        if (ballot(__builtin_amdgcn_ubfe(w, r, 1u) != 0u) & valid[i]) return false;
     }
-    {
-        const u64 m = ballot(alive);
-        if (m) {
-            u32 base = 0;
-            if (lane == 0) base = atomicAdd(A.mid_count, (u32)__popcll(m));
-            base = bcast32(base);
-            if (alive) {
-                const u32 slot = base + lanes_below(m);
-                if (slot < A.mid_cap) {
-                    MidRec rec;
-                    rec.owner = U.owner;
-                    rec.j = U.j0 + j;
-                    A.mid[slot] = rec;
-                }
-            }
-        }
-    }
-    wave_sync();
+    return true;
 }
 
 template <int RG, int NBV>
@@ -230,8 +193,6 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
     __syncthreads();
 
     const u32 lane = lane_id();
-    const u32 wave = tid >> 6;
-    u32* queue = lds + A.lds_words + wave * kQueueCap;
 
     // dense prime `lane` (lane < nd): modulus, pattern offset, per-row-group advance
     u32 dq = 1, doff = 0, dstep = 0;
@@ -241,8 +202,9 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
         doff = d.lds_off;
         dstep = d.step;
     }
-    u64 cnt_dense = 0, cnt_b = 0;
+    u64 cnt_dense = 0;   // candidates alive after the dense stage (= tested in stage B)
     TierBRegs<NBV> R;
+    u64 valid[NBV];   // lanes holding a stage-B prime, per slot
 #pragma unroll
     for (int i = 0; i < NBV; ++i) {
         const u32 p = (u32)i * 64u + lane;
@@ -250,16 +212,19 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
         tp.q = 1; tp.lds_off = 0; tp.rcp = 1.0f; tp.pad = 0;
         if (p < A.nb) tp = A.tierb[p];
         R.s[i] = 0;
-        R.q[i] = tp.q;
+        R.nq[i] = -(int)tp.q;
         R.off[i] = tp.lds_off;
         R.rcp[i] = __float_as_uint(tp.rcp);
+        valid[i] = ballot(p < A.nb);
     }
 
+    // work stealing; the next unit's index is fetched while the current one is sieved
+    u64 next = 0;
+    if (lane == 0) next = atomicAdd(A.unit_counter, 1ull);
     for (;;) {
-        u64 unit = 0;
-        if (lane == 0) unit = atomicAdd(A.unit_counter, 1ull);
-        unit = bcast64(unit);
+        u64 unit = bcast64(next);
         if (unit >= A.n_units) break;
+        if (lane == 0) next = atomicAdd(A.unit_counter, 1ull);
         unit += A.unit_base;
         Unit U;
         if (!resolve(A, unit, lane, U)) continue;
@@ -273,24 +238,14 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
         // phases: dense prime p held by lane p as an LDS bit address, stage-B prime
         // (64*i + lane) held in R.s[i]
         u32 sd = 0;
-        if (lane < A.nd) {
-            const DensePrime d = A.dense[lane];
-            u32 ph = phase(A.pmod[d.pm1], U.a, U.us_lo, U.us_hi);
-            if (d.q2 > 1) {   // CRT: the slot phase is s1 (mod q1) and s2 (mod q2)
-                const u32 s2 = phase(A.pmod[d.pm2], U.a, U.us_lo, U.us_hi);
-                const u32 t = ((s2 + d.q2 - ph % d.q2) % d.q2) * d.inv % d.q2;
-                ph += d.q1 * t;
-            }
-            sd = doff * 32u + ph;
-        }
+        if (lane < A.nd) sd = doff * 32u + phase(A.pmod[lane], U.a, U.us_lo, U.us_hi);
 #pragma unroll
         for (int i = 0; i < NBV; ++i) {
             const u32 p = (u32)i * 64u + lane;
-            R.s[i] = p < A.nb ? phase(A.pmod[A.nd_primes + p], U.a, U.us_lo, U.us_hi) : 0u;
+            R.s[i] = p < A.nb ? phase(A.pmod[A.nd + p], U.a, U.us_lo, U.us_hi) : 0u;
         }
 
         const u32 rows = (U.n + kRowBits - 1) / kRowBits;
-        u32 qn = 0;
         for (u32 row0 = 0; row0 < rows; row0 += RG) {
             u32 acc[RG];
 #pragma unroll
@@ -332,56 +287,48 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
             u32 alive[RG];
 #pragma unroll
             for (int r = 0; r < RG; ++r) {
-                const u32 jb = (row0 + (u32)r) * (u32)kRowBits + lane * 32u;
+                const u32 rowbase = (row0 + (u32)r) * (u32)kRowBits;
                 u32 v = ~acc[r];
-                if (jb >= U.n) v = 0;
-                else if (U.n - jb < 32u) v &= (1u << (U.n - jb)) - 1u;
+                if (rowbase + (u32)kRowBits > U.n) {   // the unit's last, partial row (uniform)
+                    const u32 jb = rowbase + lane * 32u;
+                    if (jb >= U.n) v = 0;
+                    else if (U.n - jb < 32u) v &= (1u << (U.n - jb)) - 1u;
+                }
                 alive[r] = v;
-                cnt_dense += __popc(v);
             }
-            // compaction: each round every lane with a pending candidate pushes its lowest
-            // one (first non-empty row) into the wave's LDS queue at its ballot rank
-            for (;;) {
-                u32 any = 0;
+            // survivors of the dense stage, one at a time in uniform control flow: the
+            // lanes with a non-empty word (ballot), then that word's bits (readlane)
 #pragma unroll
-                for (int r = 0; r < RG; ++r) any |= alive[r];
-                const u64 m = ballot(any != 0);
-                if (m == 0) break;
-                if (any) {
-                    u32 jv = 0;
-                    bool taken = false;
-#pragma unroll
-                    for (int r = 0; r < RG; ++r) {
-                        if (!taken && alive[r]) {
-                            taken = true;
-                            jv = (row0 + (u32)r) * (u32)kRowBits + lane * 32u + (u32)__builtin_ctz(alive[r]);
-                            alive[r] &= alive[r] - 1u;
+            for (int r = 0; r < RG; ++r) {
+                u64 m = ballot(alive[r] != 0u);
+                while (m) {
+                    const u32 l = (u32)__builtin_ctzll(m);
+                    m &= m - 1u;
+                    u32 w = __builtin_amdgcn_readlane(alive[r], l);
+                    cnt_dense += (u32)__builtin_popcount(w);
+                    const u32 jb = (row0 + (u32)r) * (u32)kRowBits + l * 32u;
+                    while (w) {
+                        const u32 j = jb + (u32)__builtin_ctz(w);
+                        w &= w - 1u;
+                        if (stage_b_pass<NBV>(lds, R, valid, j) && lane == 0) {
+                            const u32 slot = atomicAdd(A.mid_count, 1u);
+                            if (slot < A.mid_cap) {
+                                MidRec rec;
+                                rec.owner = U.owner;
+                                rec.j = U.j0 + j;
+                                A.mid[slot] = rec;
+                            }
                         }
                     }
-                    queue[qn + lanes_below(m)] = jv;
-                }
-                qn += (u32)__popcll(m);
-                if (qn >= 64) {
-                    stage_b<NBV>(A, lds, queue, 64, R, lane, U, cnt_b);
-                    const u32 rest = qn - 64;
-                    const u32 v = lane < rest ? queue[64 + lane] : 0u;
-                    wave_sync();
-                    if (lane < rest) queue[lane] = v;
-                    qn = rest;
                 }
             }
         }
-        if (qn) stage_b<NBV>(A, lds, queue, qn, R, lane, U, cnt_b);
     }
 
-    if (A.stats) {
-        for (int o = 32; o > 0; o >>= 1) {
-            cnt_dense += __shfl_xor(cnt_dense, o);
-            cnt_b += __shfl_xor(cnt_b, o);
-        }
+    if (A.stats) {   // cnt_dense is uniform (counted from readlane values)
         if (lane == 0) {
             atomicAdd(&A.stats[0], (unsigned long long)cnt_dense);
-            atomicAdd(&A.stats[1], (unsigned long long)cnt_b);
+            atomicAdd(&A.stats[1], (unsigned long long)cnt_dense);
         }
     }
 }

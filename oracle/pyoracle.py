@@ -266,17 +266,19 @@ def tile_range(M, WP, min_base, max_base, bits, prefix=None, lanes=0, lane_id=0)
     if not 21 <= bits <= 127:
         raise ValueError("bits")
     n_min, n_max = 1 << (bits - 1), (1 << bits) - 1
+    if n_min <= max_base:    # the reference's unsigned subtraction would wrap here
+        raise ValueError("bits too small for the lattice bases")
     tmin = ((n_min - max_base) // M) // WP
     tmax = ((n_max - min_base) // M) // WP + 1
     if prefix is None:
         if lanes:
             raise ValueError("--prefix-lanes requires --prefix")
         return tmin, tmax
-    if not prefix.startswith("0b") or len(prefix) < 3 or set(prefix[2:]) - {"0", "1"} or prefix[2] != "1":
-        raise ValueError("binary prefix")
+    if prefix[:2] not in ("0b", "0B") or len(prefix) < 3 or set(prefix[2:]) - {"0", "1"} or prefix[2] != "1":
+        raise ValueError("binary prefix")          # parse_prefix (v13.cu:408-448)
     pv, pb = int(prefix[2:], 2), len(prefix) - 2
-    if pb > bits:
-        raise ValueError("prefix longer than bits")
+    if pb >= bits:
+        raise ValueError("prefix_bits must be < bits")   # v13.cu:3181-3185
     shift = bits - pb
     cmin, cmax = pv << shift, ((pv + 1) << shift) - 1
     if cmax < n_min or cmin > n_max:

@@ -93,7 +93,7 @@ def test_tile_range_equals_oracle(oracle, ref):
     args = (ref.M, ref.WP, int(b.min()), int(b.max()))
     rnd = np.random.default_rng(0x7A)
     cases = [(89, None, 0, 0), (89, "0b101", 0, 0), (89, "0b101", 4, 2), (127, "0b11", 3, 0),
-             (60, "0b1" + "0" * 59, 0, 0), (21, None, 0, 0)]
+             (60, "0b1" + "0" * 58, 0, 0), (60, "0b1" + "0" * 59, 0, 0), (21, None, 0, 0)]
     for _ in range(300):
         bits = int(rnd.integers(40, 128))
         pb = int(rnd.integers(1, min(bits, 20) + 1))

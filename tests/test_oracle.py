@@ -389,7 +389,7 @@ def test_tile_range_bounds_every_prefixed_candidate(oracle, ref_shape, seed):
     M, WP = ref_shape.M, ref_shape.WP
     for _ in range(200):
         bits = rnd.randint(40, 127)
-        pb = rnd.randint(1, min(bits, 14))
+        pb = rnd.randint(1, min(bits - 1, 14))
         pv = rnd.randrange(1 << (pb - 1), 1 << pb)
         prefix = "0b" + format(pv, "b")
         tmin, tmax = oracle.tile_range(M, WP, lo_b, hi_b, bits, prefix)
@@ -419,6 +419,6 @@ def test_tile_range_lanes_partition(oracle, ref_shape):
         assert max(sizes) - min(sizes) <= 1 and sizes == sorted(sizes, reverse=True)
     for bad in [dict(bits=89, prefix="0b0"), dict(bits=89, prefix="101"), dict(bits=10),
                 dict(bits=89, lanes=2), dict(bits=89, prefix="0b1", lanes=2, lane_id=2),
-                dict(bits=40, prefix="0b1" + "0" * 40)]:
+                dict(bits=40, prefix="0b1" + "0" * 39), dict(bits=21)]:
         with pytest.raises(ValueError):
             oracle.tile_range(M, WP, int(b.min()), int(b.max()), **bad)
