@@ -177,7 +177,18 @@ def main():
     ap.add_argument("--served-jobs", type=int, default=10000)
     ap.add_argument("--tune-trials", type=int, default=40)
     ap.add_argument("--tuning", default=None, help="JSON tuning to use instead of the autotuner")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device (scales rocprofv3's FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 28, device="cuda")      # 1 GiB of float32
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this pipeline shards by disjoint tile ranges with no "
               "collective, and multi-GPU runs are not part of this build (use --gpus 1).", file=sys.stderr)
@@ -205,13 +216,28 @@ def main():
     bits = n_start.bit_length()
     cand = shape.count_candidates(t0, tc)
     cap, pcap = 1 << 20, 1 << 16
-    d_surv = ctx.dalloc(cap * 16)
-    d_count = ctx.dalloc(64)
-    d_res = ctx.dalloc(pcap * 32)
+    # two survivor buffers: the prove of step k (stream B) overlaps the filter of step k+1
+    # (stream A), as the reference's prover threads consume one batch while the GPU
+    # filters the next
+    d_surv = [ctx.dalloc(cap * 16) for _ in range(2)]
+    d_count = [ctx.dalloc(64) for _ in range(2)]
+    d_res = [ctx.dalloc(pcap * 32) for _ in range(2)]
+    s_filter, s_prove = ctx.stream(), ctx.stream()
+    ev_filtered = [ctx.event() for _ in range(2)]
+    ev_proved = [ctx.event() for _ in range(2)]
+    n_step = [0]
 
     def step():
-        shape.filter_launch(t0, tc, d_surv, cap, d_count)
-        shape.prove_launch(t0, d_surv, d_count, pcap, d_res, bits=bits)
+        k = n_step[0]
+        b = k % 2
+        if k >= 2:
+            ctx.wait(s_filter, ev_proved[b])      # prove k-2 has read buffer b
+        shape.filter_launch(t0, tc, d_surv[b], cap, d_count[b], stream=s_filter)
+        ctx.record(ev_filtered[b], s_filter)
+        ctx.wait(s_prove, ev_filtered[b])
+        shape.prove_launch(t0, d_surv[b], d_count[b], pcap, d_res[b], bits=bits, stream=s_prove)
+        ctx.record(ev_proved[b], s_prove)
+        n_step[0] = k + 1
 
     if args.tuning:
         shape.set_tuning(**json.loads(args.tuning))
@@ -222,6 +248,7 @@ def main():
         tuned, ms = shape.autotune(t0, tc, max_trials=args.tune_trials)
         log(f"[bench] autotune {time.perf_counter() - t_tune:.1f}s -> {tuned} ({ms:.2f} ms on the sample)")
     log(f"[bench] {name}: tiles={tc} candidates/step={cand} tuning={shape.tuning}")
+    # ctx.synchronize() is a device-wide hipDeviceSynchronize: both streams are drained
     for _ in range(args.warmup):
         step()
     ctx.synchronize()
@@ -232,11 +259,12 @@ def main():
     ctx.synchronize()
     dt = time.perf_counter() - t
     kt = ctx.profile_stop()
+    last = (n_step[0] - 1) % 2
     n_surv = np.zeros(1, np.uint32)
-    ctx.d2h(n_surv, d_count)
+    ctx.d2h(n_surv, d_count[last])
     res = np.zeros(min(int(n_surv[0]), pcap), ccs.CHAIN_RESULT_DTYPE)
     if len(res):
-        ctx.d2h(res, d_res)
+        ctx.d2h(res, d_res[last])
     value = cand * args.steps / dt
     ms_sieve = kt["ms_sieve"] / max(kt["n_sieve"], 1)
     # algorithmic work: E[residue tests] per candidate under ascending-prime early exit,

@@ -47,7 +47,31 @@ class Context:
             pass
 
     def synchronize(self):
+        """wait for all work of the device (every stream)"""
         L.check(self.lib.ccs_synchronize(self.h), "ccs_synchronize")
+
+    # streams and events for overlapping launches (default priority)
+    def stream(self):
+        p = C.c_void_p()
+        L.check(self.lib.ccs_stream_create(self.h, C.byref(p)), "ccs_stream_create")
+        return p
+
+    def stream_destroy(self, st):
+        L.check(self.lib.ccs_stream_destroy(self.h, st), "ccs_stream_destroy")
+
+    def event(self):
+        p = C.c_void_p()
+        L.check(self.lib.ccs_event_create(self.h, C.byref(p)), "ccs_event_create")
+        return p
+
+    def event_destroy(self, ev):
+        L.check(self.lib.ccs_event_destroy(self.h, ev), "ccs_event_destroy")
+
+    def record(self, ev, stream=None):
+        L.check(self.lib.ccs_event_record(self.h, ev, stream), "ccs_event_record")
+
+    def wait(self, stream, ev):
+        L.check(self.lib.ccs_stream_wait_event(self.h, stream, ev), "ccs_stream_wait_event")
 
     def profile_start(self):
         L.check(self.lib.ccs_profile_start(self.h), "ccs_profile_start")

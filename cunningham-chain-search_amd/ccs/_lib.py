@@ -29,7 +29,8 @@ EXPORTS = [
     "ccs_prove_jobset_launch", "ccs_autotune", "ccs_jobset_update",
     "ccs_server_create", "ccs_server_submit", "ccs_server_poll", "ccs_server_job_results",
     "ccs_server_drain", "ccs_server_stats", "ccs_server_shutdown", "ccs_autotune_jobs",
-    "ccs_tile_range",
+    "ccs_tile_range", "ccs_stream_create", "ccs_stream_destroy", "ccs_event_create",
+    "ccs_event_destroy", "ccs_event_record", "ccs_stream_wait_event",
 ]
 
 
@@ -163,6 +164,12 @@ def load():
     L.ccs_memcpy_d2h.argtypes = [vp, vp, vp, C.c_uint64]
     L.ccs_memset_d.argtypes = [vp, vp, C.c_int, C.c_uint64]
     L.ccs_synchronize.argtypes = [vp]
+    L.ccs_stream_create.argtypes = [vp, C.POINTER(vp)]
+    L.ccs_stream_destroy.argtypes = [vp, vp]
+    L.ccs_event_create.argtypes = [vp, C.POINTER(vp)]
+    L.ccs_event_destroy.argtypes = [vp, vp]
+    L.ccs_event_record.argtypes = [vp, vp, vp]
+    L.ccs_stream_wait_event.argtypes = [vp, vp, vp]
     L.ccs_shape_build.argtypes = [vp, C.POINTER(ShapeParams), C.POINTER(Tuning), C.POINTER(vp)]
     L.ccs_shape_free.argtypes = [vp]
     L.ccs_shape_free.restype = None

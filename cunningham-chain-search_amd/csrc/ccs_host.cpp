@@ -266,6 +266,59 @@ extern "C" int ccs_synchronize(ccs_ctx* c) {
     return CCS_OK;
 }
 
+extern "C" int ccs_stream_create(ccs_ctx* c, void** stream) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!stream) return fail(CCS_ERR_INVALID, "null argument");
+    hipStream_t st = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *stream = (void*)st;
+    return CCS_OK;
+}
+
+extern "C" int ccs_stream_destroy(ccs_ctx* c, void* stream) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!stream) return fail(CCS_ERR_INVALID, "null stream");
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    HIPCHK(hipStreamDestroy((hipStream_t)stream));
+    return CCS_OK;
+}
+
+extern "C" int ccs_event_create(ccs_ctx* c, void** event) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!event) return fail(CCS_ERR_INVALID, "null argument");
+    hipEvent_t e = nullptr;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    *event = (void*)e;
+    return CCS_OK;
+}
+
+extern "C" int ccs_event_destroy(ccs_ctx* c, void* event) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!event) return fail(CCS_ERR_INVALID, "null event");
+    HIPCHK(hipEventDestroy((hipEvent_t)event));
+    return CCS_OK;
+}
+
+extern "C" int ccs_event_record(ccs_ctx* c, void* event, void* stream) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!event) return fail(CCS_ERR_INVALID, "null event");
+    HIPCHK(hipEventRecord((hipEvent_t)event, stream ? (hipStream_t)stream : c->stream));
+    return CCS_OK;
+}
+
+extern "C" int ccs_stream_wait_event(ccs_ctx* c, void* stream, void* event) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!event) return fail(CCS_ERR_INVALID, "null event");
+    HIPCHK(hipStreamWaitEvent(stream ? (hipStream_t)stream : c->stream, (hipEvent_t)event, 0));
+    return CCS_OK;
+}
+
 // ---------------------------------------------------------------------------
 // shape
 // ---------------------------------------------------------------------------
@@ -356,23 +409,38 @@ static u32 slot_period(const ccs_shape* s, std::pair<int, int> sl) {
 }
 
 // default split: dense primes below 128 (at most 64), stage B up to 256 primes below 2^12
+static int validate_tuning(const ccs_shape* s, ccs_tuning* t);
+
+// Starting point measured on the 31# depth-18 workload: dense primes below 200 folded in
+// pairs up to period 2^16, eight rows per lane, stage B over the next 256 primes below
+// 4096, 2^20-candidate units, one block of 16 waves per CU.  Shrunk until the tables fit
+// the device's LDS.
 static void default_tuning(const ccs_shape* s, ccs_tuning* t) {
     u32 nd = 0;
-    while (nd < s->fp.size() && nd < (u32)kMaxDense && s->fp[nd] < 128) ++nd;
+    while (nd < s->fp.size() && nd < 2u * (u32)kMaxDense && s->fp[nd] < 200) ++nd;
     u32 nb = 0;
     while (nd + nb < s->fp.size() && nb < 256 && s->fp[nd + nb] < 4096) ++nb;
     t->n_dense = nd;
     t->n_tierb = nb;
-    t->rows_per_group = 2;
-    t->unit_j = 262144;
-    t->blocks_per_cu = 4;
-    t->pair_max = 0;
-    t->waves_per_block = 4;
+    t->rows_per_group = 8;
+    t->unit_j = 1u << 20;
+    t->blocks_per_cu = 1;
+    t->pair_max = 1u << 16;
+    t->waves_per_block = 16;
+    for (;;) {
+        ccs_tuning tt = *t;
+        if (validate_tuning(s, &tt) == CCS_OK) return;
+        if (t->rows_per_group > 2) t->rows_per_group >>= 1;
+        else if (t->pair_max > 4096) t->pair_max >>= 2;
+        else if (t->n_tierb > 16) t->n_tierb -= 16;
+        else if (t->n_dense > 8) --t->n_dense;
+        else return;
+    }
 }
 
 static int validate_tuning(const ccs_shape* s, ccs_tuning* t) {
-    if (t->rows_per_group != 1 && t->rows_per_group != 2 && t->rows_per_group != 4)
-        return fail(CCS_ERR_INVALID, "rows_per_group must be 1, 2 or 4");
+    if (t->rows_per_group != 1 && t->rows_per_group != 2 && t->rows_per_group != 4 && t->rows_per_group != 8)
+        return fail(CCS_ERR_INVALID, "rows_per_group must be 1, 2, 4 or 8");
     if (t->n_dense > 2u * kMaxDense) return fail(CCS_ERR_INVALID, "n_dense > 128");
     if (t->pair_max > (1u << 16)) return fail(CCS_ERR_INVALID, "pair_max > 65536");
     const auto slots = dense_slots(s, t);
@@ -1561,7 +1629,7 @@ static int autotune_impl(ccs_ctx* c, ccs_shape* s, const std::function<int(float
             t.pair_max = pm;
             if ((rc = try_tuning(t))) return rc;
         }
-        for (u32 rg : {1u, 2u, 4u}) {
+        for (u32 rg : {1u, 2u, 4u, 8u}) {
             ccs_tuning t = best;
             if (rg == t.rows_per_group) continue;
             t.rows_per_group = rg;
@@ -1578,16 +1646,14 @@ static int autotune_impl(ccs_ctx* c, ccs_shape* s, const std::function<int(float
             t.n_tierb = nb;
             if ((rc = try_tuning_nb(t))) return rc;
         }
-        for (u32 wpb : {2u, 4u, 8u, 16u}) {
+        // waves per block and blocks per CU together: the LDS tables are per block, so
+        // more waves per block raise occupancy without more LDS
+        static const u32 occ[][2] = {{16, 1}, {16, 2}, {8, 2}, {8, 3}, {8, 4}, {4, 4}, {4, 8}};
+        for (const auto& o : occ) {
             ccs_tuning t = best;
-            if (wpb == t.waves_per_block) continue;
-            t.waves_per_block = wpb;
-            if ((rc = try_tuning(t))) return rc;
-        }
-        for (u32 bpc : {1u, 2u, 3u, 4u, 6u, 8u}) {
-            ccs_tuning t = best;
-            if (bpc == t.blocks_per_cu) continue;
-            t.blocks_per_cu = bpc;
+            if (o[0] == t.waves_per_block && o[1] == t.blocks_per_cu) continue;
+            t.waves_per_block = o[0];
+            t.blocks_per_cu = o[1];
             if ((rc = try_tuning(t))) return rc;
         }
     }

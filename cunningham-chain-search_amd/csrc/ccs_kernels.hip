@@ -250,12 +250,13 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
             u32 acc[RG];
 #pragma unroll
             for (int r = 0; r < RG; ++r) acc[r] = 0;
-            // four patterns per step so their LDS reads are in flight together
+            // SPS patterns per step so their LDS reads (2*RG per pattern) are in flight together
+            constexpr int SPS = RG >= 8 ? 2 : 4;
             u32 p = 0;
-            for (; p + 4 <= A.nd; p += 4) {
-                u32 ba[4], lo[4][RG], hi[4][RG];
+            for (; p + SPS <= A.nd; p += SPS) {
+                u32 ba[SPS], lo[SPS][RG], hi[SPS][RG];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < SPS; ++u) {
                     ba[u] = __builtin_amdgcn_readlane(sd, p + (u32)u);
                     const u32* src = lds + (ba[u] >> 5) + lane;
 #pragma unroll
@@ -265,7 +266,7 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
+                for (int u = 0; u < SPS; ++u)
 #pragma unroll
                     for (int r = 0; r < RG; ++r) acc[r] |= __builtin_amdgcn_alignbit(hi[u][r], lo[u][r], ba[u] & 31u);
             }
@@ -837,6 +838,7 @@ hipError_t launch_sieve(const SieveArgs& a, int rg, int nbv, int grid, int block
     CCS_SIEVE_CASE(1, 1) CCS_SIEVE_CASE(1, 2) CCS_SIEVE_CASE(1, 3) CCS_SIEVE_CASE(1, 4)
     CCS_SIEVE_CASE(2, 1) CCS_SIEVE_CASE(2, 2) CCS_SIEVE_CASE(2, 3) CCS_SIEVE_CASE(2, 4)
     CCS_SIEVE_CASE(4, 1) CCS_SIEVE_CASE(4, 2) CCS_SIEVE_CASE(4, 3) CCS_SIEVE_CASE(4, 4)
+    CCS_SIEVE_CASE(8, 1) CCS_SIEVE_CASE(8, 2) CCS_SIEVE_CASE(8, 3) CCS_SIEVE_CASE(8, 4)
 #undef CCS_SIEVE_CASE
     return hipErrorInvalidValue;
 }

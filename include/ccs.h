@@ -130,8 +130,8 @@ typedef struct {
 typedef struct {
     uint32_t n_dense;          /* filter primes handled by the dense pattern stage (<= 128, at most 64 patterns) */
     uint32_t n_tierb;          /* next primes handled by stage B (<= 256)           */
-    uint32_t rows_per_group;   /* 1, 2 or 4                                         */
-    uint32_t unit_j;           /* candidates per work unit, multiple of 2048        */
+    uint32_t rows_per_group;   /* 1, 2, 4 or 8 rows of 2048 candidates per lane step */
+    uint32_t unit_j;           /* candidates per work unit, multiple of 2048, <= 2^20 */
     uint32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu             */
     uint32_t pair_max;         /* fold adjacent dense primes into one pattern when  */
                                /*   q1*q2 <= pair_max (0 = one prime per pattern)   */
@@ -150,7 +150,19 @@ int ccs_dfree(ccs_ctx* ctx, void* d_ptr);
 int ccs_memcpy_h2d(ccs_ctx* ctx, void* d_dst, const void* h_src, uint64_t bytes);
 int ccs_memcpy_d2h(ccs_ctx* ctx, void* h_dst, const void* d_src, uint64_t bytes);
 int ccs_memset_d(ccs_ctx* ctx, void* d_dst, int value, uint64_t bytes);
-int ccs_synchronize(ccs_ctx* ctx);
+int ccs_synchronize(ccs_ctx* ctx);   /* waits for every stream of the device */
+
+/* ---- streams and events of the context's device (default priority), for callers with
+ *      no HIP runtime of their own: the `stream` argument of the launch entry points takes
+ *      these (NULL = the context's stream).  Lets the filter of one batch overlap the
+ *      prove of the previous one, as the reference overlaps its GPU filter with the CPU
+ *      prover threads (v13.cu:1967-2068, :3769-3960). ---- */
+int ccs_stream_create(ccs_ctx* ctx, void** stream);
+int ccs_stream_destroy(ccs_ctx* ctx, void* stream);
+int ccs_event_create(ccs_ctx* ctx, void** event);
+int ccs_event_destroy(ccs_ctx* ctx, void* event);
+int ccs_event_record(ccs_ctx* ctx, void* event, void* stream);
+int ccs_stream_wait_event(ccs_ctx* ctx, void* stream, void* event);
 
 /* ---- shape (replaces init_filter_tables v13.cu:793-901, generate_bases :907-1059,
  *      gpu_upload_tables :1065-1222) ---- */

@@ -113,7 +113,8 @@ def test_jobset_rejects_stale_tuning(ctx):
     jobs, W = WL.batched_jobs(n_jobs=4, seed=0xB8, lo_exp=10, hi_exp=12)
     s = WL.make_shape(ctx, "c4", classes=False)
     js = ccs.JobSet(s, jobs)
-    s.set_tuning(unit_j=s.tuning["unit_j"] * 2)
+    uj = s.tuning["unit_j"]
+    s.set_tuning(unit_j=uj // 2 if uj > 2048 else uj * 2)
     d_out, d_cnt = ctx.dalloc(1 << 16), ctx.dalloc(64)
     with pytest.raises(ccs.CcsError):
         js.launch(d_out, 1 << 12, d_cnt)

@@ -327,3 +327,55 @@ def test_config2_full_launch_properties(ctx, oracle):
         lo = 0 if q <= 97 else 1
         pd *= 1 - len({(pow(2, -i, q) - 1) % q for i in range(lo, 18)}) / q
     assert abs(st1["alive_dense"] / st1["candidates"] - pd) < 0.01 * pd
+
+
+def test_pipelined_streams_match_sequential(ctx, ref):
+    """Filter of batch k+1 on one stream overlapping the prove of batch k on another
+    (context streams and events) gives each batch the results of a sequential run."""
+    say("[plain] two-stream pipeline")
+    cap, pcap = 1 << 18, 1 << 14
+    ranges = [WL.reference_range(seed=0x700 + i, bits=89)[0] for i in range(4)]
+    d_s, d_c, d_r = ctx.dalloc(cap * 16), ctx.dalloc(64), ctx.dalloc(pcap * 32)
+
+    def read(d_surv, d_count, d_res):
+        n = np.zeros(1, np.uint32)
+        ctx.d2h(n, d_count)
+        surv = np.zeros(int(n[0]), ccs.SURVIVOR_DTYPE)
+        res = np.zeros(min(int(n[0]), pcap), ccs.CHAIN_RESULT_DTYPE)
+        if len(surv):
+            ctx.d2h(surv, d_surv)
+            ctx.d2h(res, d_res)
+        order = np.lexsort((surv["k_offset"], surv["base_idx"], surv["tile_idx"]))
+        by_idx = {int(r["survivor_idx"]): r for r in res}
+        return [(int(surv[i]["tile_idx"]), int(surv[i]["base_idx"]), int(surv[i]["k_offset"]),
+                 int(by_idx[i]["flags"]), int(by_idx[i]["chain_len"])) for i in order]
+
+    want = []
+    for t0 in ranges:
+        ref.filter_launch(t0, 1800, d_s, cap, d_c)
+        ref.prove_launch(t0, d_s, d_c, pcap, d_r, bits=89)
+        ctx.synchronize()
+        want.append(read(d_s, d_c, d_r))
+    sf, sp = ctx.stream(), ctx.stream()
+    bufs = [(ctx.dalloc(cap * 16), ctx.dalloc(64), ctx.dalloc(pcap * 32)) for _ in range(2)]
+    ev_f, ev_p = [ctx.event() for _ in range(2)], [ctx.event() for _ in range(2)]
+    got = [None] * len(ranges)
+    for k, t0 in enumerate(ranges):
+        b = k % 2
+        if k >= 2:
+            ctx.wait(sf, ev_p[b])
+            ctx.synchronize()
+            got[k - 2] = read(*bufs[b])
+        ref.filter_launch(t0, 1800, bufs[b][0], cap, bufs[b][1], stream=sf)
+        ctx.record(ev_f[b], sf)
+        ctx.wait(sp, ev_f[b])
+        ref.prove_launch(t0, bufs[b][0], bufs[b][1], pcap, bufs[b][2], bits=89, stream=sp)
+        ctx.record(ev_p[b], sp)
+    ctx.synchronize()
+    for k in range(len(ranges) - 2, len(ranges)):
+        got[k] = read(*bufs[k % 2])
+    assert got == want and all(len(w) > 0 for w in want)
+    for e in ev_f + ev_p:
+        ctx.event_destroy(e)
+    ctx.stream_destroy(sf)
+    ctx.stream_destroy(sp)
