@@ -342,6 +342,7 @@ struct ccs_shape {
     u32* d_lds_image = nullptr;
     DensePrime* d_dense = nullptr;
     TierBPrime* d_tierb = nullptr;
+    u32* d_tierb_far = nullptr;          // stage-B bitsets of the primes after the first 64
     PrimeMod* d_pmod = nullptr;
     DeepPrime* d_deep = nullptr;
     u32* d_hash = nullptr;
@@ -374,6 +375,7 @@ static void free_device_tables(ccs_shape* s) {
     (void)hipFree(s->d_lds_image);
     (void)hipFree(s->d_dense);
     (void)hipFree(s->d_tierb);
+    (void)hipFree(s->d_tierb_far);
     (void)hipFree(s->d_pmod);
     (void)hipFree(s->d_deep);
     (void)hipFree(s->d_hash);
@@ -382,6 +384,7 @@ static void free_device_tables(ccs_shape* s) {
     s->d_lds_image = nullptr;
     s->d_dense = nullptr;
     s->d_tierb = nullptr;
+    s->d_tierb_far = nullptr;
     s->d_pmod = nullptr;
     s->d_deep = nullptr;
     s->d_hash = nullptr;
@@ -459,7 +462,7 @@ static int validate_tuning(const ccs_shape* s, ccs_tuning* t) {
         return fail(CCS_ERR_INVALID, "waves_per_block must be 1, 2, 4, 8 or 16");
     u32 words = 0;
     for (auto sl : slots) words += dense_ext_words(slot_period(s, sl), t->rows_per_group);
-    for (u32 i = 0; i < t->n_tierb; ++i) words += (s->fp[t->n_dense + i] + 31) / 32;
+    for (u32 i = 0; i < t->n_tierb && i < (u32)kTierBLds; ++i) words += (s->fp[t->n_dense + i] + 31) / 32;
     if (words > s->max_lds_words) return fail(CCS_ERR_INVALID, "dense + stage-B tables exceed LDS");
     return CCS_OK;
 }
@@ -505,12 +508,16 @@ static int upload_tables(ccs_shape* s) {
             image.push_back(v);
         }
     }
+    // stage-B bitsets: the first group of 64 primes (tested for every dense survivor) in
+    // LDS, the later groups (reached by a few percent of them) in device memory
+    std::vector<u32> far_bits;
     for (u32 i = 0; i < nb; ++i) {
         const u32 q = s->fp[nd + i];
         std::vector<u8> kill = uset(nd + i);
+        std::vector<u32>& dst = i < (u32)kTierBLds ? image : far_bits;
         TierBPrime b;
         b.q = q;
-        b.lds_off = (u32)image.size();
+        b.lds_off = (u32)dst.size();
         b.rcp = rcp_up(q);
         b.pad = 0;
         tierb[i] = b;
@@ -520,10 +527,11 @@ static int upload_tables(ccs_shape* s) {
                 u32 x = w * 32 + bb;
                 if (x < q && kill[x]) v |= 1u << bb;
             }
-            image.push_back(v);
+            dst.push_back(v);
         }
         pmod[slots.size() + i] = make_primemod(q, s->W);
     }
+    if (far_bits.empty()) far_bits.push_back(0);
     std::vector<DeepPrime> deep(nc ? nc : 1);
     std::vector<u32> hash((size_t)(nc ? nc : 1) * kMaxHashSlots, 0xffffffffu);
     for (u32 i = 0; i < nc; ++i) {
@@ -556,6 +564,7 @@ static int upload_tables(ccs_shape* s) {
     HIPCHK(up((void**)&s->d_lds_image, image.data(), image.size() * 4));
     HIPCHK(up((void**)&s->d_dense, dense.data(), dense.size() * sizeof(DensePrime)));
     HIPCHK(up((void**)&s->d_tierb, tierb.data(), tierb.size() * sizeof(TierBPrime)));
+    HIPCHK(up((void**)&s->d_tierb_far, far_bits.data(), far_bits.size() * 4));
     HIPCHK(up((void**)&s->d_pmod, pmod.data(), pmod.size() * sizeof(PrimeMod)));
     HIPCHK(up((void**)&s->d_deep, deep.data(), deep.size() * sizeof(DeepPrime)));
     HIPCHK(up((void**)&s->d_hash, hash.data(), hash.size() * 4));
@@ -783,9 +792,11 @@ static void fill_sieve_common(const ccs_shape* s, const Scratch& x, SieveArgs& a
     a.lds_words = s->lds_words;
     a.nd = s->nd;
     a.nb = s->nb;
+    a.one = 1;
     a.unit_j = s->tune.unit_j;
     a.dense = s->d_dense;
     a.tierb = s->d_tierb;
+    a.tierb_far = s->d_tierb_far;
     a.pmod = s->d_pmod;
     a.unit_counter = (u64*)(x.counters + 2);
     a.mid = x.mid;

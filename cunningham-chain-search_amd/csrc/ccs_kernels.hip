@@ -169,17 +169,17 @@ struct TierBRegs {
 // per lane (prime 64*i + lane in slot i), so one probe of every lane tests 64 primes at
 // once.  x = j + s < 2^21 and the host's reciprocal is rounded up so that the truncated
 // f32 product is exactly floor(x / q); the probe reads bit r of the prime's
-// forbidden-residue bitset.  Most candidates die in the first group of 64 primes, so the
-// later groups are rarely evaluated.
+// forbidden-residue bitset.  Most candidates die in the first group of 64 primes (bitsets
+// in LDS), so the later groups (bitsets in device memory, L2-resident) are rarely read.
 template <int NBV>
-__device__ __forceinline__ bool stage_b_pass(const u32* __restrict__ lds, const TierBRegs<NBV>& R,
-                                             const u64 (&valid)[NBV], u32 j) {
+__device__ __forceinline__ bool stage_b_pass(const u32* __restrict__ lds, const u32* __restrict__ far,
+                                             const TierBRegs<NBV>& R, const u64 (&valid)[NBV], u32 j) {
 #pragma unroll
     for (int i = 0; i < NBV; ++i) {
         const u32 x = j + R.s[i];
         const int qt = (int)((float)x * __uint_as_float(R.rcp[i]));
         const u32 r = (u32)(__mul24(qt, R.nq[i]) + (int)x);                 // x mod q
-        const u32 w = lds[R.off[i] + __builtin_amdgcn_ubfe(r, 5u, 11u)];
+        const u32 w = (i == 0 ? lds : far)[R.off[i] + __builtin_amdgcn_ubfe(r, 5u, 11u)];
         if (ballot(__builtin_amdgcn_ubfe(w, r, 1u) != 0u) & valid[i]) return false;
     }
     return true;
@@ -259,10 +259,13 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                 for (int u = 0; u < SPS; ++u) {
                     ba[u] = __builtin_amdgcn_readlane(sd, p + (u32)u);
                     const u32* src = lds + (ba[u] >> 5) + lane;
+                    // A.one == 1: an offset the compiler cannot fold, so the lo words of
+                    // row pairs (and the hi words) load with one stride-64 instruction each
+                    const u32* srch = src + A.one;
 #pragma unroll
                     for (int r = 0; r < RG; ++r) {
                         lo[u][r] = src[64 * r];
-                        hi[u][r] = src[64 * r + 1];
+                        hi[u][r] = srch[64 * r];
                     }
                 }
 #pragma unroll
@@ -311,7 +314,7 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                     while (w) {
                         const u32 j = jb + (u32)__builtin_ctz(w);
                         w &= w - 1u;
-                        if (stage_b_pass<NBV>(lds, R, valid, j) && lane == 0) {
+                        if (stage_b_pass<NBV>(lds, A.tierb_far, R, valid, j) && lane == 0) {
                             const u32 slot = atomicAdd(A.mid_count, 1u);
                             if (slot < A.mid_cap) {
                                 MidRec rec;
