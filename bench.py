@@ -21,6 +21,13 @@ for p in (ROOT, PKG):
 # published campaign baseline (RTX 4090/5090, v13 3-stage filter, depth 18-20): 57-65 G/s
 PUBLISHED_GPU_CAND_PER_S = 65e9
 
+# HBM bytes per sieve_kernel launch of the config-2 step at the default tuning, from two
+# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch), each scaled by what the
+# counter reads for a 1 GiB device copy (bench.py --copy-calibration): FETCH_SIZE reads
+# 524308 KiB for 1 GiB fetched (x2.0), WRITE_SIZE 1048576 KiB for 1 GiB written (x1.0).
+SIEVE_TRAFFIC = {"c2": {"fetch_kib": 6415.97, "write_kib": 10742.23, "fetch_scale": 2.0, "write_scale": 1.0,
+                        "profile": "profiles/sieve_kernel_cc18_31primorial_pmc.md"}}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -43,12 +50,16 @@ def cpu_baseline(name, t0, seconds):
     p = WL.shape_params(name)
     o = pyoracle.Shape(p["crt"], p["wheel"], p["target"], p["depth"], p["filter_primes"], p["pos0_max_q"])
     nt = cpu_threads()
-    tiles = 6 * nt * 50
     o.filter_bv(t0, 6, cap=1 << 16, nthreads=1)        # one-time table build (startup work)
-    t = time.perf_counter()
-    o.filter_bv(t0, tiles, cap=1 << 20, nthreads=nt)
-    dt = time.perf_counter() - t
-    tiles = int(max(tiles, min(6 * 10 ** 7, tiles * seconds / max(dt, 1e-3))))
+    tiles = 6 * nt * 50
+    while True:                                         # calibrate on >= 1 s of work
+        t = time.perf_counter()
+        o.filter_bv(t0, tiles, cap=1 << 22, nthreads=nt)
+        dt = time.perf_counter() - t
+        if dt >= 1.0 or tiles >= 6 * 10 ** 8:
+            break
+        tiles *= 4
+    tiles = int(min(6 * 10 ** 8, tiles * seconds / max(dt, 1e-3)))
     tiles -= tiles % 6
     t = time.perf_counter()
     surv, cand = o.filter_bv(t0, tiles, cap=1 << 22, nthreads=nt)
@@ -172,7 +183,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="c2", choices=["c2", "reference", "c3"])
-    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default tuning)")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the autotuner on the same range first (default: the shape's default tuning)")
+    ap.add_argument("--no-tune", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and served measurements")
     ap.add_argument("--served-jobs", type=int, default=10000)
     ap.add_argument("--tune-trials", type=int, default=40)
@@ -241,7 +254,7 @@ def main():
 
     if args.tuning:
         shape.set_tuning(**json.loads(args.tuning))
-    elif not args.no_tune:
+    elif args.tune:
         # autotuner on the same range (setup work, not timed): the unit size it picks
         # depends on the run length per class, so a shorter sample would mislead it
         t_tune = time.perf_counter()
@@ -274,6 +287,10 @@ def main():
         lo = 0 if q <= shape.pos0_max_q else 1
         fs.append((q, len(WL.forbidden(q, lo, shape.depth))))
     e_tests, surv_frac = WL.expected_tests(fs)
+    tr = SIEVE_TRAFFIC.get(name)
+    traffic = None
+    if tr and tr["fetch_kib"] is not None:
+        traffic = (tr["fetch_kib"] * tr["fetch_scale"] + tr["write_kib"] * tr["write_scale"]) * 1024
     ops = cand * e_tests * WL.INT32_OPS_PER_TEST
     achieved = ops / (ms_sieve / 1e3) / 1e12
     out = {
@@ -293,7 +310,7 @@ def main():
                    "candidates_per_step": cand, "tuning": shape.tuning},
         "roofline": {"bound": "valu_int32", "achieved": achieved, "peak": WL.INT32_PEAK_TOPS,
                      "unit": "Tops/s (int32 VALU)", "frac": achieved / WL.INT32_PEAK_TOPS,
-                     "traffic": None, "kernel": "sieve_kernel",
+                     "traffic": traffic, "kernel": "sieve_kernel",
                      "kernel_ms": ms_sieve, "e_tests_per_candidate": e_tests,
                      "ops_per_launch": ops},
         "kernels_ms": {"sieve": ms_sieve, "deep": kt["ms_deep"] / max(kt["n_deep"], 1),
