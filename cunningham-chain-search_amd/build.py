@@ -6,8 +6,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libccs.so")
-SOURCES = ["ccs_kernels.hip", "ccs_host.cpp", "ccs_server.cpp"]
-HEADERS = ["ccs_internal.hpp", os.path.join("..", "..", "include", "ccs.h")]
+SOURCES = ["ccs_kernels.hip", "ccs_host.cpp", "ccs_server.cpp", "ccs_confirm.cpp"]
+HEADERS = ["ccs_internal.hpp", "ccs_arith.hpp", os.path.join("..", "..", "include", "ccs.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CCS_ARCH", "gfx950")

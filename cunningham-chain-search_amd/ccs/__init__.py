@@ -243,6 +243,18 @@ class Shape:
                                    len(surv), C.byref(pp), out.ctypes.data), "ccs_prove")
         return out
 
+    def confirm_host(self, tile_start, surv, bits=0, test="bpsw", bases=(2,), nthreads=0):
+        """Host confirmation engine over plain survivors (ccs_confirm_host): the GPU prove's
+        semantics on CPU threads.  Returns (results, stats dict)."""
+        surv = np.ascontiguousarray(surv, SURVIVOR_DTYPE)
+        out = np.zeros(len(surv), CHAIN_RESULT_DTYPE)
+        pp = _prove_params(bits, test, bases)
+        st = L.ConfirmStats()
+        L.check(self.lib.ccs_confirm_host(self.h, L.U128.of(tile_start), surv.ctypes.data, len(surv),
+                                          C.byref(pp), nthreads, out.ctypes.data, C.byref(st)),
+                "ccs_confirm_host")
+        return out, st.as_dict()
+
     def batch(self, jobs, cap=1 << 20):
         """Batched form: one persistent launch over class-form jobs (a, u0, count)."""
         jarr = jobs_array(jobs)
@@ -402,6 +414,30 @@ def _prove_params(bits, test, bases):
     for i, b in enumerate(bl):
         pp.bases[i] = b
     return pp
+
+
+def confirm_values(values, bits=0, test="bpsw", bases=(2,), nthreads=0):
+    """Host confirmation engine over arbitrary n < 2^128 (ccs_confirm_values); needs no GPU.
+    Returns (results, stats dict)."""
+    lib = L.load()
+    vals = np.zeros(len(values), np.dtype([("lo", "<u8"), ("hi", "<u8")]))
+    for i, v in enumerate(values):
+        vals[i] = (int(v) & ((1 << 64) - 1), int(v) >> 64)
+    out = np.zeros(len(vals), CHAIN_RESULT_DTYPE)
+    pp = _prove_params(bits, test, bases)
+    st = L.ConfirmStats()
+    L.check(lib.ccs_confirm_values(vals.ctypes.data, len(vals), C.byref(pp), nthreads, out.ctypes.data,
+                                   C.byref(st)), "ccs_confirm_values")
+    return out, st.as_dict()
+
+
+def chain_stats(res):
+    """Stats record of prove results (ccs_confirm_stats_add)."""
+    lib = L.load()
+    res = np.ascontiguousarray(res, CHAIN_RESULT_DTYPE)
+    st = L.ConfirmStats()
+    L.check(lib.ccs_confirm_stats_add(res.ctypes.data, len(res), C.byref(st)), "ccs_confirm_stats_add")
+    return st.as_dict()
 
 
 def format_results(res, log_threshold=8, target=18):

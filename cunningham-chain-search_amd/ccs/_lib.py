@@ -31,6 +31,7 @@ EXPORTS = [
     "ccs_server_drain", "ccs_server_stats", "ccs_server_shutdown", "ccs_autotune_jobs",
     "ccs_tile_range", "ccs_stream_create", "ccs_stream_destroy", "ccs_event_create",
     "ccs_event_destroy", "ccs_event_record", "ccs_stream_wait_event",
+    "ccs_confirm_values", "ccs_confirm_host", "ccs_confirm_stats_add",
 ]
 
 
@@ -88,6 +89,15 @@ class FilterStats(C.Structure):
 class ProveParams(C.Structure):
     _fields_ = [("bits", C.c_int32), ("test_kind", C.c_int32), ("n_bases", C.c_uint32),
                 ("bases", C.c_uint32 * 16)]
+
+
+class ConfirmStats(C.Structure):
+    _fields_ = [("primes", C.c_uint64), ("chains", C.c_uint64), ("non_roots", C.c_uint64),
+                ("best_chain", C.c_uint32), ("_pad", C.c_uint32), ("chain_dist", C.c_uint64 * 65)]
+
+    def as_dict(self):
+        return {"primes": int(self.primes), "chains": int(self.chains), "non_roots": int(self.non_roots),
+                "best_chain": int(self.best_chain), "chain_dist": [int(x) for x in self.chain_dist]}
 
 
 class KernelTimes(C.Structure):
@@ -186,6 +196,11 @@ def load():
     L.ccs_last_stats.argtypes = [vp, C.POINTER(FilterStats)]
     L.ccs_prove_launch.argtypes = [vp, vp, U128, vp, vp, C.c_uint32, C.POINTER(ProveParams), vp, vp]
     L.ccs_prove.argtypes = [vp, vp, U128, vp, C.c_uint32, C.POINTER(ProveParams), vp]
+    L.ccs_confirm_values.argtypes = [vp, C.c_uint32, C.POINTER(ProveParams), C.c_uint32, vp,
+                                     C.POINTER(ConfirmStats)]
+    L.ccs_confirm_host.argtypes = [vp, U128, vp, C.c_uint32, C.POINTER(ProveParams), C.c_uint32, vp,
+                                   C.POINTER(ConfirmStats)]
+    L.ccs_confirm_stats_add.argtypes = [vp, C.c_uint32, C.POINTER(ConfirmStats)]
     L.ccs_format_results.argtypes = [vp, C.c_uint32, C.c_int32, C.c_int32, C.c_char_p, C.c_uint64,
                                      C.POINTER(C.c_uint64)]
     L.ccs_batch_launch.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp, vp]

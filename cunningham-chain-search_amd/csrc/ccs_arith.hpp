@@ -1,0 +1,299 @@
+// Exact modular and u128 arithmetic shared by the GPU kernels (ccs_kernels.hip) and the
+// host confirmation engine (ccs_confirm.cpp): the same code decides primality on both
+// sides.  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ccs_internal.hpp"
+
+#define CCS_HD __host__ __device__ __forceinline__
+#define CCS_HDN __host__ __device__ inline
+
+namespace ccs {
+
+// ---------------------------------------------------------------------------
+// exact residues of small moduli
+// ---------------------------------------------------------------------------
+
+// y mod q for y < 2^52, q < 2^18: double reciprocal estimate, corrected by +-q
+CCS_HD u32 mod52(u64 y, u32 q, double rcp) {
+    u64 qt = (u64)((double)y * rcp);
+    int64_t r = (int64_t)(y - qt * (u64)q);
+    if (r < 0) r += q;
+    if (r < 0) r += q;
+    if (r >= (int64_t)q) r -= q;
+    if (r >= (int64_t)q) r -= q;
+    return (u32)r;
+}
+
+// (hi:lo) mod q via limb folding with c_k = 2^(32k) mod q
+CCS_HD u32 mod128(u64 lo, u64 hi, const PrimeMod& pm) {
+    u64 y = (lo & 0xffffffffull) + (lo >> 32) * (u64)pm.c32 + (hi & 0xffffffffull) * (u64)pm.c64 +
+            (hi >> 32) * (u64)pm.c96;
+    return mod52(y, pm.q, pm.rcp);
+}
+
+// high 64 bits of a 64x64-bit product
+CCS_HD u64 umulhi64(u64 a, u64 b) { return (u64)(((unsigned __int128)a * b) >> 64); }
+
+// ---------------------------------------------------------------------------
+// u128 helpers
+// ---------------------------------------------------------------------------
+
+struct U2 {
+    u64 lo, hi;
+};
+
+CCS_HD U2 mk(u64 lo, u64 hi) { U2 r; r.lo = lo; r.hi = hi; return r; }
+CCS_HD bool ge(U2 a, U2 b) { return a.hi > b.hi || (a.hi == b.hi && a.lo >= b.lo); }
+CCS_HD bool eq(U2 a, U2 b) { return a.hi == b.hi && a.lo == b.lo; }
+CCS_HD bool iszero(U2 a) { return (a.lo | a.hi) == 0; }
+CCS_HD U2 add(U2 a, U2 b) { U2 r; r.lo = a.lo + b.lo; r.hi = a.hi + b.hi + (r.lo < a.lo); return r; }
+CCS_HD U2 sub(U2 a, U2 b) { U2 r; r.lo = a.lo - b.lo; r.hi = a.hi - b.hi - (a.lo < b.lo); return r; }
+CCS_HD U2 shr1(U2 a) { return mk((a.lo >> 1) | (a.hi << 63), a.hi >> 1); }
+CCS_HD U2 shl1(U2 a) { return mk(a.lo << 1, (a.hi << 1) | (a.lo >> 63)); }
+CCS_HD int bitlen(U2 a) {
+    return a.hi ? 128 - __builtin_clzll(a.hi) : (a.lo ? 64 - __builtin_clzll(a.lo) : 0);
+}
+CCS_HD bool bit(U2 a, int i) { return i >= 64 ? ((a.hi >> (i - 64)) & 1) : ((a.lo >> i) & 1); }
+// low 128 bits of a*b
+CCS_HD U2 mullo(U2 a, U2 b) {
+    return mk(a.lo * b.lo, umulhi64(a.lo, b.lo) + a.lo * b.hi + a.hi * b.lo);
+}
+// a (u128) * b (u64), low 128 bits
+CCS_HD U2 mul_u64(U2 a, u64 b) { return mk(a.lo * b, umulhi64(a.lo, b) + a.hi * b); }
+
+// ---------------------------------------------------------------------------
+// u128 Montgomery arithmetic, R = 2^128, n odd, n < 2^127
+// ---------------------------------------------------------------------------
+
+struct Mont {
+    U2 n, ni;   // ni = -n^-1 mod 2^128
+    U2 one;     // R mod n
+};
+
+CCS_HD void mont_init(Mont& c, U2 n) {
+    c.n = n;
+    U2 x = n;   // n*n = 1 (mod 8)
+    const U2 two = mk(2, 0);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) x = mullo(x, sub(two, mullo(n, x)));
+    c.ni = sub(mk(0, 0), x);
+    // R mod n: start from 2^(b-1) < n and double up to 2^128
+    const int b = bitlen(n);
+    U2 r = b - 1 >= 64 ? mk(0, 1ull << (b - 1 - 64)) : mk(1ull << (b - 1), 0);
+    for (int i = b - 1; i < 128; ++i) {
+        r = shl1(r);
+        if (ge(r, n)) r = sub(r, n);
+    }
+    c.one = r;
+}
+
+CCS_HD U2 mont_mul(const Mont& c, U2 a, U2 b) {
+    // t = a*b, four 64-bit limbs
+    u64 l00 = a.lo * b.lo, h00 = umulhi64(a.lo, b.lo);
+    u64 l01 = a.lo * b.hi, h01 = umulhi64(a.lo, b.hi);
+    u64 l10 = a.hi * b.lo, h10 = umulhi64(a.hi, b.lo);
+    u64 l11 = a.hi * b.hi, h11 = umulhi64(a.hi, b.hi);
+    u64 t0 = l00;
+    u64 t1 = h00 + l01;
+    u64 c1 = t1 < l01;
+    t1 += l10;
+    c1 += t1 < l10;
+    u64 t2 = h01 + h10;
+    u64 c2 = t2 < h10;
+    t2 += l11;
+    c2 += t2 < l11;
+    t2 += c1;
+    c2 += t2 < c1;
+    u64 t3 = h11 + c2;
+    // m = t * ni mod 2^128
+    u64 m0 = t0 * c.ni.lo;
+    u64 m1 = umulhi64(t0, c.ni.lo) + t0 * c.ni.hi + t1 * c.ni.lo;
+    // s = m * n, limbs 1..3 (limb 0 cancels t0)
+    l00 = m0 * c.n.lo; h00 = umulhi64(m0, c.n.lo);
+    l01 = m0 * c.n.hi; h01 = umulhi64(m0, c.n.hi);
+    l10 = m1 * c.n.lo; h10 = umulhi64(m1, c.n.lo);
+    l11 = m1 * c.n.hi; h11 = umulhi64(m1, c.n.hi);
+    u64 s1 = h00 + l01;
+    u64 d1 = s1 < l01;
+    s1 += l10;
+    d1 += s1 < l10;
+    u64 s2 = h01 + h10;
+    u64 d2 = s2 < h10;
+    s2 += l11;
+    d2 += s2 < l11;
+    s2 += d1;
+    d2 += s2 < d1;
+    u64 s3 = h11 + d2;
+    (void)s1;
+    // (t + s) / 2^128: the low 128 bits sum to 0 mod 2^128 with a carry iff t_low != 0
+    const u64 k = (t0 | t1) != 0;
+    u64 r0 = t2 + s2;
+    u64 k2 = r0 < s2;
+    r0 += k;
+    k2 += r0 < k;
+    u64 r1 = t3 + s3 + k2;
+    U2 r = mk(r0, r1);
+    if (ge(r, c.n)) r = sub(r, c.n);
+    return r;
+}
+
+CCS_HD U2 madd(const Mont& c, U2 a, U2 b) { U2 r = add(a, b); if (ge(r, c.n)) r = sub(r, c.n); return r; }
+CCS_HD U2 msub(const Mont& c, U2 a, U2 b) { return ge(a, b) ? sub(a, b) : add(sub(a, b), c.n); }
+CCS_HD U2 mhalf(const Mont& c, U2 a) { if (a.lo & 1) a = add(a, c.n); return shr1(a); }
+
+// Montgomery form of a small constant x: x*R mod n by double-and-add of R mod n
+CCS_HD U2 mont_small(const Mont& c, u32 x) {
+    U2 r = mk(0, 0);
+    for (int i = 31; i >= 0; --i) {
+        r = madd(c, r, r);
+        if ((x >> i) & 1) r = madd(c, r, c.one);
+    }
+    return r;
+}
+
+CCS_HD U2 mont_pow(const Mont& c, U2 bm, U2 e) {
+    U2 r = c.one;
+    for (int i = bitlen(e) - 1; i >= 0; --i) {
+        r = mont_mul(c, r, r);
+        if (bit(e, i)) r = mont_mul(c, r, bm);
+    }
+    return r;
+}
+
+CCS_HDN bool strong_mr(const Mont& c, u32 a) {
+    U2 d = sub(c.n, mk(1, 0));
+    int s = 0;
+    while (!(d.lo & 1)) { d = shr1(d); ++s; }
+    const U2 one = c.one, mone = sub(c.n, c.one);
+    U2 x = mont_pow(c, mont_small(c, a), d);
+    if (eq(x, one) || eq(x, mone)) return true;
+    for (int i = 1; i < s; ++i) {
+        x = mont_mul(c, x, x);
+        if (eq(x, mone)) return true;
+        if (eq(x, one)) return false;
+    }
+    return false;
+}
+
+// n mod m for a small odd m (< 2^16)
+CCS_HD u32 mod_small(U2 n, u32 m) {
+    PrimeMod pm;
+    pm.q = m;
+    u64 c32 = (0xffffffffull % m + 1) % m;
+    pm.c32 = (u32)c32;
+    pm.c64 = (u32)(c32 * c32 % m);
+    pm.c96 = (u32)((u64)pm.c64 * c32 % m);
+    pm.rcp = 1.0 / (double)m;
+    return mod128(n.lo, n.hi, pm);
+}
+
+// Jacobi symbol (a/m), m odd positive, both small
+CCS_HD int jacobi_small(u32 a, u32 m) {
+    int res = 1;
+    a %= m;
+    while (a) {
+        while (!(a & 1)) {
+            a >>= 1;
+            const u32 r8 = m & 7;
+            if (r8 == 3 || r8 == 5) res = -res;
+        }
+        const u32 t = a; a = m; m = t;
+        if ((a & 3) == 3 && (m & 3) == 3) res = -res;
+        a %= m;
+    }
+    return m == 1 ? res : 0;
+}
+
+// (D/n) for odd D, |D| small, n odd large
+CCS_HD int jacobi_D(int D, U2 n) {
+    const u32 ad = (u32)(D < 0 ? -D : D);
+    int res = 1;
+    if (D < 0 && (n.lo & 3) == 3) res = -res;          // (-1/n)
+    // (|D|/n) = (n/|D|) * (-1)^(((|D|-1)/2) * ((n-1)/2))
+    if ((ad & 3) == 3 && (n.lo & 3) == 3) res = -res;
+    return res * jacobi_small(mod_small(n, ad), ad);
+}
+
+// strong Lucas-Selfridge test (P = 1, Q = (1-D)/4, D = 5, -7, 9, ... first with (D/n) = -1).
+// `ok` is cleared when no parameter was found in range (never happens for the chain
+// values tested here: they are 2 mod 3, so never perfect squares).
+CCS_HDN bool strong_lucas(const Mont& c, bool& ok) {
+    int D = 5;
+    int tries = 0;
+    for (;;) {
+        const int j = jacobi_D(D, c.n);
+        if (j == -1) break;
+        if (j == 0) {
+            const u32 ad = (u32)(D < 0 ? -D : D);
+            if (!(c.n.hi == 0 && c.n.lo == ad)) return false;
+        }
+        D = D > 0 ? -(D + 2) : -(D - 2);
+        if (++tries > 200) { ok = false; return false; }
+    }
+    const int Q = (1 - D) / 4;
+    U2 Dm = mont_small(c, (u32)(D < 0 ? -D : D));
+    if (D < 0) Dm = msub(c, mk(0, 0), Dm);
+    U2 Qm = mont_small(c, (u32)(Q < 0 ? -Q : Q));
+    if (Q < 0) Qm = msub(c, mk(0, 0), Qm);
+    U2 d = add(c.n, mk(1, 0));
+    int s = 0;
+    while (!(d.lo & 1)) { d = shr1(d); ++s; }
+    U2 U = c.one, V = c.one, Qk = Qm;
+    for (int i = bitlen(d) - 2; i >= 0; --i) {
+        U = mont_mul(c, U, V);
+        V = msub(c, mont_mul(c, V, V), madd(c, Qk, Qk));
+        Qk = mont_mul(c, Qk, Qk);
+        if (bit(d, i)) {
+            const U2 nu = mhalf(c, madd(c, U, V));
+            const U2 nv = mhalf(c, madd(c, mont_mul(c, Dm, U), V));
+            U = nu;
+            V = nv;
+            Qk = mont_mul(c, Qk, Qm);
+        }
+    }
+    if (iszero(U) || iszero(V)) return true;
+    for (int r = 1; r < s; ++r) {
+        V = msub(c, mont_mul(c, V, V), madd(c, Qk, Qk));
+        if (iszero(V)) return true;
+        Qk = mont_mul(c, Qk, Qk);
+    }
+    return false;
+}
+
+// Baillie-PSW: the decision of GMP's mpz_probab_prime_p(n, 1) (v13.cu:1841)
+// small18: PrimeMod records of the 18 primes 2..61
+CCS_HDN bool is_prime_bpsw(U2 n, const PrimeMod* small18, bool& ok) {
+    if (n.hi == 0 && n.lo < 2) return false;
+#pragma unroll 1
+    for (int i = 0; i < 18; ++i) {
+        const u32 p = small18[i].q;
+        if (n.hi == 0 && n.lo == p) return true;
+        if (mod128(n.lo, n.hi, small18[i]) == 0) return false;
+    }
+    if (n.hi == 0 && n.lo < 61ull * 61ull) return true;
+    Mont c;
+    mont_init(c, n);
+    if (!strong_mr(c, 2)) return false;
+    return strong_lucas(c, ok);
+}
+
+// strong Miller-Rabin over the given bases
+CCS_HDN bool is_prime_mr(U2 n, const u32* bases, int n_bases) {
+    if (n.hi == 0 && n.lo < 2) return false;
+    if (n.hi == 0 && n.lo < 4) return true;
+    if (!(n.lo & 1)) return false;
+    Mont c;
+    mont_init(c, n);
+    for (int i = 0; i < n_bases; ++i) {
+        const u32 b = bases[i];
+        if (n.hi == 0 && (u64)b % n.lo == 0) continue;
+        if (!strong_mr(c, b)) return false;
+    }
+    return true;
+}
+
+
+}  // namespace ccs

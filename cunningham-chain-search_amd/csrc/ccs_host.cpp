@@ -32,6 +32,11 @@ static int fail(int code, const std::string& msg) {
     return code;
 }
 
+namespace ccs {
+// error reporting for the other translation units of the library
+int set_error(int code, const char* msg) { return fail(code, msg); }
+}  // namespace ccs
+
 #define HIPCHK(call)                                                                         \
     do {                                                                                     \
         hipError_t e_ = (call);                                                              \
@@ -1680,6 +1685,23 @@ static int autotune_impl(ccs_ctx* c, ccs_shape* s, const std::function<int(float
 // campaign ranges: the tiles whose candidates have `bits` bits, narrowed to a binary
 // prefix and split into lanes (v13.cu:450-518 compute_prefix_tile_range, :3474-3573)
 // ---------------------------------------------------------------------------
+
+extern "C" int ccs_confirm_host(const ccs_shape* s, ccs_u128 tile_start, const ccs_survivor* h_surv,
+                                uint32_t count, const ccs_prove_params* params, uint32_t nthreads,
+                                ccs_chain_result* h_out, ccs_confirm_stats* stats) {
+    if (!s || (count && (!h_surv || !h_out))) return fail(CCS_ERR_INVALID, "null argument");
+    if (s->bases.empty()) return fail(CCS_ERR_INVALID, "shape has no bases");
+    std::vector<ccs_u128> n(count);
+    const u128 T0 = mk128(tile_start);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (h_surv[i].base_idx >= s->bases.size()) return fail(CCS_ERR_INVALID, "survivor base_idx out of range");
+        const u128 v = ((T0 + h_surv[i].tile_idx) * (u128)s->WP + h_surv[i].k_offset) * (u128)s->M +
+                       s->bases[h_surv[i].base_idx];
+        n[i].lo = (u64)v;
+        n[i].hi = (u64)(v >> 64);
+    }
+    return ccs_confirm_values(n.data(), count, params, nthreads, h_out, stats);
+}
 
 extern "C" int ccs_tile_range(const ccs_shape* s, int32_t bits, ccs_u128 prefix_value, int32_t prefix_bits,
                               uint32_t lanes, uint32_t lane_id, ccs_u128* tile_min, ccs_u128* tile_max) {

@@ -164,6 +164,9 @@ struct ProveArgs {
     u32* chain_count;          // queue length (scratch, cleared by the launcher)
 };
 
+// sets the text returned by ccs_last_error() and returns code (ccs_host.cpp)
+int set_error(int code, const char* msg);
+
 // kernel launchers (ccs_kernels.hip)
 hipError_t launch_sieve(const SieveArgs& a, int rg, int nbv, int grid, int block, size_t lds_bytes,
                         hipStream_t s);

@@ -220,6 +220,29 @@ int ccs_prove(ccs_ctx* ctx, const ccs_shape* shape, ccs_u128 tile_start,
 int ccs_format_results(const ccs_chain_result* res, uint32_t n, int32_t log_threshold,
                        int32_t target, char* buf, uint64_t buf_size, uint64_t* needed);
 
+/* ---- host confirmation engine (replaces the CPU prover pool prover_worker v13.cu:1794-1924
+ *      and its merge prove_survivors_mt :1967-2068): the same semantics on host threads with
+ *      the library's own u128 Baillie-PSW instead of GMP.  Pure CPU work; usable without a
+ *      GPU, e.g. to re-confirm the GPU's chains ---- */
+typedef struct {
+    uint64_t primes;            /* prime survivors (ProverStats.primes)               */
+    uint64_t chains;            /* chains followed from a true root                   */
+    uint64_t non_roots;         /* primes whose (n-1)/2 is prime (walked back)        */
+    uint32_t best_chain, _pad;
+    uint64_t chain_dist[65];    /* chains by length (v13 g_chain_dist)                */
+} ccs_confirm_stats;
+
+/* one result per value, same fields as the GPU prove; nthreads 0 = hardware threads
+ * (at most 16) */
+int ccs_confirm_values(const ccs_u128* h_n, uint32_t count, const ccs_prove_params* params,
+                       uint32_t nthreads, ccs_chain_result* h_out, ccs_confirm_stats* stats);
+/* survivors of a plain launch, n = ((T0 + tile_idx)*WP + k)*M + base (v13.cu:1819-1826) */
+int ccs_confirm_host(const ccs_shape* shape, ccs_u128 tile_start, const ccs_survivor* h_surv,
+                     uint32_t count, const ccs_prove_params* params, uint32_t nthreads,
+                     ccs_chain_result* h_out, ccs_confirm_stats* stats);
+/* add prove results (GPU or host) to a stats record */
+int ccs_confirm_stats_add(const ccs_chain_result* res, uint32_t count, ccs_confirm_stats* stats);
+
 /* ---- kernel timing: while collecting, every sieve / deep / prove launch on the context
  *      records HIP events on the stream it runs on (the reference times K1 with CUDA events,
  *      v13.cu:3786-3796) ---- */

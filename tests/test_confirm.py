@@ -1,0 +1,88 @@
+"""Host confirmation engine (ccs_confirm_values / ccs_confirm_stats_add): the prover
+semantics of v13.cu:1794-1924 on CPU threads with the library's own Baillie-PSW.  Pure host
+code in libccs.so, so these run without a GPU; each result is compared field by field with
+the oracle's prover (pinned by tests/test_oracle.py) and with the reference's chain vectors."""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import ccs
+from conftest import GOLDEN
+
+VEC = json.load(open(os.path.join(GOLDEN, "reference_vectors.json")))
+
+
+def known_roots():
+    roots = [(int(h, 16), L) for h, L in VEC["v16_qvectors"]]
+    roots += [(int(r), L) for r, L in VEC["classic_chains"]]
+    roots.append((int(VEC["cc13_hex"][0], 16), 13))
+    with open(os.path.join(GOLDEN, "cc10plus_roots_sample_1000.txt")) as f:
+        for line in f:
+            tag, hx, _ = line.split()
+            roots.append((int(hx, 16), int(tag[2:])))
+    return roots
+
+
+def compare(values, oracle, bits=0, kind=0, bases=(2,), nthreads=0):
+    test = "bpsw" if kind == 0 else "mr"
+    res, st = ccs.confirm_values(values, bits=bits, test=test, bases=bases, nthreads=nthreads)
+    for v, r in zip(values, res):
+        f, root, ln, back = oracle.prove_one(v, bits, kind, bases)
+        got = (int(r["flags"]), int(r["root_lo"]) | (int(r["root_hi"]) << 64), int(r["chain_len"]),
+               int(r["back_steps"]))
+        assert got == (f, root, ln, back), hex(v)
+    return res, st
+
+
+def test_reference_chains_and_members(oracle):
+    """Every chain of the reference's vectors and data sample, proven from its root and from
+    a later member (backward walk to the true root), has the published length."""
+    roots = known_roots()
+    vals = [r for r, _ in roots]
+    res, st = compare(vals, oracle)
+    # the chain length counts only primes >= 2; from 2 (and 89: 89 is a trial prime) the
+    # v13 gates stop earlier, so check those through the oracle comparison only
+    for (r, L), x in zip(roots, res):
+        if r > 409:
+            assert int(x["chain_len"]) == L and int(x["back_steps"]) == 0, (hex(r), L)
+    members = []
+    for r, L in roots[:300]:
+        if r > 409 and L >= 3:
+            members.append(2 * (2 * r + 1) + 1)        # third member: two steps back to r
+    res2, _ = compare(members, oracle)
+    assert all(int(x["back_steps"]) >= 2 and (x["flags"] & ccs.F_NONROOT) for x in res2)
+
+
+def test_random_values_and_gates(oracle):
+    rnd = random.Random(0xC0F)
+    vals = [rnd.getrandbits(rnd.randint(2, 127)) | 1 for _ in range(3000)]
+    vals += [int(x) for x in VEC["v33_mr_primes"][:50]] if "v33_mr_primes" in VEC else []
+    vals += [3474749660383, 3825123056546413051, 2 ** 127 - 1, 2 ** 127 + 1, 2 ** 128 - 1, 0, 1, 2, 3, 4]
+    compare(vals, oracle)
+    compare(vals[:500], oracle, bits=89)
+    compare(vals[:500], oracle, kind=1, bases=(2,))
+    compare(vals[:500], oracle, kind=1, bases=(2, 3, 5, 7, 11, 13))
+
+
+def test_threads_and_stats(oracle):
+    rnd = random.Random(5)
+    vals = [r for r, _ in known_roots()][:400] + [rnd.getrandbits(100) | 1 for _ in range(400)]
+    r1, s1 = ccs.confirm_values(vals, nthreads=1)
+    r8, s8 = ccs.confirm_values(vals, nthreads=8)
+    assert np.array_equal(r1, r8) and s1 == s8
+    primes = r1[(r1["flags"] & ccs.F_PRIME) != 0]
+    assert s1["primes"] == len(primes) == s1["chains"]
+    assert s1["non_roots"] == int(((primes["flags"] & ccs.F_NONROOT) != 0).sum())
+    assert s1["best_chain"] == int(primes["chain_len"].max())
+    assert s1["chain_dist"] == [int((primes["chain_len"] == L).sum()) for L in range(65)]
+    assert ccs.chain_stats(r1) == s1
+
+
+def test_bad_parameters():
+    with pytest.raises(ccs.CcsError):
+        ccs.confirm_values([7], test="mr", bases=())
+    with pytest.raises(ccs.CcsError):
+        ccs.confirm_values([7], bits=200)

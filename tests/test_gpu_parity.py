@@ -379,3 +379,17 @@ def test_pipelined_streams_match_sequential(ctx, ref):
         ctx.event_destroy(e)
     ctx.stream_destroy(sf)
     ctx.stream_destroy(sp)
+
+
+def test_host_confirm_equals_gpu_prove(ctx, ref):
+    """The host confirmation engine (CPU threads) and the GPU prove kernels give the same
+    result for every survivor of a reference-shape range, and the same stats."""
+    say("[prove] host confirm vs GPU prove")
+    t0, _ = WL.reference_range(seed=0x3C, bits=89)
+    surv, _ = ref.filter(t0, 6000)
+    assert len(surv) > 500
+    for bits in (89, 0):
+        g = ref.prove(t0, surv, bits=bits)
+        h, st = ref.confirm_host(t0, surv, bits=bits)
+        assert np.array_equal(g, h)
+        assert st == ccs.chain_stats(g) and st["primes"] > 0
