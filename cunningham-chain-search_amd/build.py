@@ -1,4 +1,5 @@
-"""Build libccs.so (HIP kernels for gfx950 + the C ABI) in-tree with hipcc."""
+"""Build libccs.so (HIP kernels for gfx950 + the C ABI) in-tree with hipcc, and the
+ccs_search campaign CLI on top of it."""
 import os
 import subprocess
 import sys
@@ -6,7 +7,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libccs.so")
-SOURCES = ["ccs_kernels.hip", "ccs_host.cpp", "ccs_server.cpp", "ccs_confirm.cpp"]
+CLI = os.path.join(HERE, "ccs_search")
+SOURCES = ["ccs_kernels.hip", "ccs_host.cpp", "ccs_server.cpp", "ccs_confirm.cpp", "ccs_campaign.cpp"]
 HEADERS = ["ccs_internal.hpp", "ccs_arith.hpp", os.path.join("..", "..", "include", "ccs.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -40,6 +42,14 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    # the campaign CLI: host code on the C ABI only, so the host compiler builds it
+    cli_tmp = CLI + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(HERE, "..", "include"),
+           os.path.join(CSRC, "ccs_search.cpp"), "-L" + HERE, "-lccs", "-Wl,-rpath,$ORIGIN", "-o", cli_tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(cli_tmp, CLI)
     return LIB
 
 

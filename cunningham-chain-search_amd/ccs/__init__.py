@@ -207,6 +207,12 @@ class Shape:
                 "ccs_tile_range")
         return lo.lo | (lo.hi << 64), hi.lo | (hi.hi << 64)
 
+    def decompose(self, n):
+        """ccs_shape_decompose: where n sits on the lattice (the --verify-root diagnostic)."""
+        d = L.Decomposition()
+        L.check(self.lib.ccs_shape_decompose(self.h, L.U128.of(n), C.byref(d)), "ccs_shape_decompose")
+        return d.as_dict()
+
     def count_candidates(self, tile_start, tile_count):
         return int(self.lib.ccs_count_candidates(self.h, L.U128.of(tile_start), tile_count))
 
@@ -414,6 +420,81 @@ def _prove_params(bits, test, bases):
     for i, b in enumerate(bl):
         pp.bases[i] = b
     return pp
+
+
+class Campaign:
+    """Campaign driver state (ccs_campaign_*): the v13 main loop's tile walk, stop rules,
+    statistics and CC18_CUDA_v10 checkpoint.  Built from a shape (range from --bits /
+    --prefix / --prefix-lanes) or from an explicit tile range."""
+
+    WALKS = {"inherit": L.WALK_INHERIT, "sequential": L.WALK_SEQUENTIAL, "random": L.WALK_RANDOM}
+
+    def __init__(self, shape=None, tile_range=None, bits=89, target=18, depth=18, prefix=None,
+                 walk="inherit", lanes=0, lane_id=0, start=None, seed=None, batch_tiles=16384, max_tiles=0):
+        self.lib = L.load()
+        p = L.CampaignParams()
+        p.bits, p.target, p.depth = bits, target, depth
+        self._prefix = prefix.encode() if prefix else None
+        p.prefix = self._prefix
+        p.walk = self.WALKS[walk]
+        p.lanes, p.lane_id = lanes, lane_id
+        p.has_start = start is not None
+        p.start = L.U128.of(start or 0)
+        p.has_seed = seed is not None
+        p.seed = seed or 0
+        p.batch_tiles, p.max_tiles = batch_tiles, max_tiles
+        h = C.c_void_p()
+        if shape is not None:
+            L.check(self.lib.ccs_campaign_create(shape.h, C.byref(p), C.byref(h)), "ccs_campaign_create")
+        else:
+            lo, hi = tile_range
+            L.check(self.lib.ccs_campaign_create_range(L.U128.of(lo), L.U128.of(hi), C.byref(p), C.byref(h)),
+                    "ccs_campaign_create_range")
+        self.h = h
+
+    def next(self):
+        """(tile_start, tile_count), or None when the campaign is done."""
+        ts, n = L.U128(), C.c_uint32(0)
+        rc = self.lib.ccs_campaign_next(self.h, C.byref(ts), C.byref(n))
+        if rc == L.CCS_DONE:
+            return None
+        L.check(rc, "ccs_campaign_next")
+        return ts.lo | (ts.hi << 64), n.value
+
+    def record(self, tile_start, tile_count, candidates=0, survivors=0, res=None):
+        res = np.zeros(0, CHAIN_RESULT_DTYPE) if res is None else np.ascontiguousarray(res, CHAIN_RESULT_DTYPE)
+        L.check(self.lib.ccs_campaign_record(self.h, L.U128.of(tile_start), tile_count, candidates, survivors,
+                                             res.ctypes.data, len(res)), "ccs_campaign_record")
+
+    def state(self):
+        st = L.CampaignState()
+        L.check(self.lib.ccs_campaign_get_state(self.h, C.byref(st)), "ccs_campaign_get_state")
+        return st.as_dict()
+
+    def save(self, path):
+        L.check(self.lib.ccs_campaign_save(self.h, str(path).encode()), "ccs_campaign_save")
+
+    def resume(self, path):
+        L.check(self.lib.ccs_campaign_resume(self.h, str(path).encode()), "ccs_campaign_resume")
+
+    def close(self):
+        if self.h:
+            self.lib.ccs_campaign_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def parse_prefix(text):
+    """ccs_parse_prefix: '0b101' -> (5, 3); raises CcsError on the reference's errors."""
+    lib = L.load()
+    v, b = L.U128(), C.c_int32(0)
+    L.check(lib.ccs_parse_prefix(text.encode(), C.byref(v), C.byref(b)), "ccs_parse_prefix")
+    return v.lo | (v.hi << 64), b.value
 
 
 def confirm_values(values, bits=0, test="bpsw", bases=(2,), nthreads=0):

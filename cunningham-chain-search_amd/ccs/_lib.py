@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(PKG_DIR, "libccs.so")
 
 CCS_OK, CCS_OVERFLOW = 0, 1
 CCS_TEST_BPSW, CCS_TEST_MR = 0, 1
+CCS_DONE = 2
+WALK_INHERIT, WALK_SEQUENTIAL, WALK_RANDOM = 0, 1, 2
 CCS_SHAPE_NO_CLASSES = 1
 F_BITS_OK, F_TRIAL_OK, F_PRIME, F_NONROOT, F_OVERFLOW, F_UNSUPPORTED = 1, 2, 4, 8, 16, 32
 
@@ -32,6 +34,9 @@ EXPORTS = [
     "ccs_tile_range", "ccs_stream_create", "ccs_stream_destroy", "ccs_event_create",
     "ccs_event_destroy", "ccs_event_record", "ccs_stream_wait_event",
     "ccs_confirm_values", "ccs_confirm_host", "ccs_confirm_stats_add",
+    "ccs_event_synchronize", "ccs_parse_prefix", "ccs_shape_decompose", "ccs_campaign_create", "ccs_campaign_create_range",
+    "ccs_campaign_free", "ccs_campaign_next", "ccs_campaign_record", "ccs_campaign_get_state",
+    "ccs_campaign_save", "ccs_campaign_resume",
 ]
 
 
@@ -98,6 +103,43 @@ class ConfirmStats(C.Structure):
     def as_dict(self):
         return {"primes": int(self.primes), "chains": int(self.chains), "non_roots": int(self.non_roots),
                 "best_chain": int(self.best_chain), "chain_dist": [int(x) for x in self.chain_dist]}
+
+
+class Decomposition(C.Structure):
+    _fields_ = [("on_lattice", C.c_uint32), ("base_found", C.c_uint32), ("class_found", C.c_uint32),
+                ("bucket_ok", C.c_uint32), ("n_mod6", C.c_uint32), ("base_idx", C.c_uint32),
+                ("k_offset", C.c_uint32), ("_pad", C.c_uint32), ("base", C.c_uint64), ("tile", U128)]
+
+    def as_dict(self):
+        d = {f: int(getattr(self, f)) for f, _ in self._fields_ if f not in ("_pad", "tile")}
+        d["tile"] = self.tile.lo | (self.tile.hi << 64)
+        return d
+
+
+class CampaignParams(C.Structure):
+    _fields_ = [("bits", C.c_int32), ("target", C.c_int32), ("depth", C.c_int32), ("prefix", C.c_char_p),
+                ("walk", C.c_int32), ("lanes", C.c_uint32), ("lane_id", C.c_uint32), ("has_start", C.c_int32),
+                ("has_seed", C.c_int32), ("seed", C.c_uint64), ("start", U128), ("batch_tiles", C.c_uint32),
+                ("_pad", C.c_uint32), ("max_tiles", C.c_uint64)]
+
+
+class CampaignState(C.Structure):
+    _fields_ = [("tile_min", U128), ("tile_max", U128), ("current", U128), ("seed", C.c_uint64),
+                ("batches", C.c_uint64), ("total_tiles", C.c_uint64), ("total_candidates", C.c_uint64),
+                ("total_survivors", C.c_uint64), ("random_mode", C.c_int32), ("done", C.c_int32),
+                ("chains", ConfirmStats)]
+
+    def as_dict(self):
+        d = {}
+        for f, t in self._fields_:
+            v = getattr(self, f)
+            if t is U128:
+                d[f] = v.lo | (v.hi << 64)
+            elif t is ConfirmStats:
+                d[f] = v.as_dict()
+            else:
+                d[f] = int(v)
+        return d
 
 
 class KernelTimes(C.Structure):
@@ -180,6 +222,7 @@ def load():
     L.ccs_event_destroy.argtypes = [vp, vp]
     L.ccs_event_record.argtypes = [vp, vp, vp]
     L.ccs_stream_wait_event.argtypes = [vp, vp, vp]
+    L.ccs_event_synchronize.argtypes = [vp, vp]
     L.ccs_shape_build.argtypes = [vp, C.POINTER(ShapeParams), C.POINTER(Tuning), C.POINTER(vp)]
     L.ccs_shape_free.argtypes = [vp]
     L.ccs_shape_free.restype = None
@@ -201,6 +244,17 @@ def load():
     L.ccs_confirm_host.argtypes = [vp, U128, vp, C.c_uint32, C.POINTER(ProveParams), C.c_uint32, vp,
                                    C.POINTER(ConfirmStats)]
     L.ccs_confirm_stats_add.argtypes = [vp, C.c_uint32, C.POINTER(ConfirmStats)]
+    L.ccs_parse_prefix.argtypes = [C.c_char_p, C.POINTER(U128), C.POINTER(C.c_int32)]
+    L.ccs_shape_decompose.argtypes = [vp, U128, C.POINTER(Decomposition)]
+    L.ccs_campaign_create.argtypes = [vp, C.POINTER(CampaignParams), C.POINTER(vp)]
+    L.ccs_campaign_create_range.argtypes = [U128, U128, C.POINTER(CampaignParams), C.POINTER(vp)]
+    L.ccs_campaign_free.argtypes = [vp]
+    L.ccs_campaign_free.restype = None
+    L.ccs_campaign_next.argtypes = [vp, C.POINTER(U128), C.POINTER(C.c_uint32)]
+    L.ccs_campaign_record.argtypes = [vp, U128, C.c_uint32, C.c_uint64, C.c_uint64, vp, C.c_uint32]
+    L.ccs_campaign_get_state.argtypes = [vp, C.POINTER(CampaignState)]
+    L.ccs_campaign_save.argtypes = [vp, C.c_char_p]
+    L.ccs_campaign_resume.argtypes = [vp, C.c_char_p]
     L.ccs_format_results.argtypes = [vp, C.c_uint32, C.c_int32, C.c_int32, C.c_char_p, C.c_uint64,
                                      C.POINTER(C.c_uint64)]
     L.ccs_batch_launch.argtypes = [vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp, vp]

@@ -316,6 +316,14 @@ extern "C" int ccs_event_record(ccs_ctx* c, void* event, void* stream) {
     return CCS_OK;
 }
 
+extern "C" int ccs_event_synchronize(ccs_ctx* c, void* event) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!event) return fail(CCS_ERR_INVALID, "null event");
+    HIPCHK(hipEventSynchronize((hipEvent_t)event));
+    return CCS_OK;
+}
+
 extern "C" int ccs_stream_wait_event(ccs_ctx* c, void* stream, void* event) {
     int rc = ctx_check(c);
     if (rc) return rc;
@@ -1701,6 +1709,33 @@ extern "C" int ccs_confirm_host(const ccs_shape* s, ccs_u128 tile_start, const c
         n[i].hi = (u64)(v >> 64);
     }
     return ccs_confirm_values(n.data(), count, params, nthreads, h_out, stats);
+}
+
+extern "C" int ccs_shape_decompose(const ccs_shape* s, ccs_u128 nv, ccs_decomposition* out) {
+    if (!s || !out) return fail(CCS_ERR_INVALID, "null argument");
+    if (s->classes.empty()) return fail(CCS_ERR_INVALID, "shape was built without its class table");
+    std::memset(out, 0, sizeof(*out));
+    const u128 n = mk128(nv);
+    out->n_mod6 = (u32)(n % 6);
+    const u64 base = (u64)(n % s->M);
+    out->base = base;
+    const u128 k = (n - base) / s->M;
+    const u128 tile = k / s->WP;
+    out->k_offset = (u32)(k % s->WP);
+    out->tile.lo = (u64)tile;
+    out->tile.hi = (u64)(tile >> 64);
+    const auto it = std::lower_bound(s->bases.begin(), s->bases.end(), base);
+    if (it == s->bases.end() || *it != base) return CCS_OK;
+    out->base_found = 1;
+    out->base_idx = (u32)(it - s->bases.begin());
+    for (const ClassRec& c : s->classes) {
+        if (c.base_idx != out->base_idx || c.k != out->k_offset) continue;
+        out->class_found = 1;
+        out->bucket_ok = (u32)(tile % 6) == c.rho;
+        break;
+    }
+    out->on_lattice = out->base_found && out->class_found && out->bucket_ok && out->n_mod6 == 5;
+    return CCS_OK;
 }
 
 extern "C" int ccs_tile_range(const ccs_shape* s, int32_t bits, ccs_u128 prefix_value, int32_t prefix_bits,

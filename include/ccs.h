@@ -29,6 +29,7 @@ extern "C" {
 #define CCS_ERR_NOMEM (-3)
 #define CCS_ERR_UNSUPPORTED (-4)
 #define CCS_ERR_BUSY (-5)
+#define CCS_DONE 2                  /* a campaign has no further batch (sweep complete, --tiles reached) */
 
 typedef struct ccs_ctx ccs_ctx;       /* one GPU: stream, scratch buffers, timing events   */
 typedef struct ccs_shape ccs_shape;   /* CRT shape + filter profile, device-resident tables */
@@ -99,6 +100,15 @@ typedef struct {
     uint64_t root_lo, root_hi;
 } ccs_chain_result;
 
+/* chain statistics of prove results (v13 ProverStats, v13.cu:1740-1768) */
+typedef struct {
+    uint64_t primes;            /* prime survivors (ProverStats.primes)               */
+    uint64_t chains;            /* chains followed from a true root                   */
+    uint64_t non_roots;         /* primes whose (n-1)/2 is prime (walked back)        */
+    uint32_t best_chain, _pad;
+    uint64_t chain_dist[65];    /* chains by length (v13 g_chain_dist)                */
+} ccs_confirm_stats;
+
 #define CCS_F_BITS_OK 1u       /* passed the bit-size gate (v13.cu:1828)          */
 #define CCS_F_TRIAL_OK 2u      /* no factor among 67..409 (v13.cu:1832-1838)      */
 #define CCS_F_PRIME 4u         /* probable prime (Baillie-PSW or MR bases)        */
@@ -163,6 +173,7 @@ int ccs_event_create(ccs_ctx* ctx, void** event);
 int ccs_event_destroy(ccs_ctx* ctx, void* event);
 int ccs_event_record(ccs_ctx* ctx, void* event, void* stream);
 int ccs_stream_wait_event(ccs_ctx* ctx, void* stream, void* event);
+int ccs_event_synchronize(ccs_ctx* ctx, void* event);   /* host waits for the event */
 
 /* ---- shape (replaces init_filter_tables v13.cu:793-901, generate_bases :907-1059,
  *      gpu_upload_tables :1065-1222) ---- */
@@ -192,6 +203,68 @@ uint64_t ccs_count_candidates(const ccs_shape* shape, ccs_u128 tile_start, uint3
  * range setup (v13.cu:3474-3573). */
 int ccs_tile_range(const ccs_shape* shape, int32_t bits, ccs_u128 prefix_value, int32_t prefix_bits,
                    uint32_t lanes, uint32_t lane_id, ccs_u128* tile_min, ccs_u128* tile_max);
+
+/* "0b1..." -> value and bit count, with the reference's errors (parse_prefix, v13.cu:408-448) */
+int ccs_parse_prefix(const char* str, ccs_u128* value, int32_t* bits);
+
+/* Where n sits on the shape's lattice (the --verify-root diagnostic, v13.cu:2921-3066):
+ * base = n mod M, k = (n - base)/M, tile = k / WP, k_offset = k mod WP; on_lattice when the
+ * base is admissible, (base, k_offset) is an admissible class and tile = rho (mod 6).
+ * Needs a shape built with its class table. */
+typedef struct {
+    uint32_t on_lattice;       /* all checks below hold                              */
+    uint32_t base_found;       /* n mod M is one of the shape's bases                */
+    uint32_t class_found;      /* (base, k_offset) is an admissible class            */
+    uint32_t bucket_ok;        /* tile = rho(base, k_offset) (mod 6)                 */
+    uint32_t n_mod6;           /* 5 for the first kind                               */
+    uint32_t base_idx, k_offset, _pad;
+    uint64_t base;
+    ccs_u128 tile;
+} ccs_decomposition;
+int ccs_shape_decompose(const ccs_shape* shape, ccs_u128 n, ccs_decomposition* out);
+
+/* ---- campaign driver state (replaces the v13 main loop's tile walk, stop rules and
+ *      checkpoint file: v13.cu:3584-3667, :3719-3765, :3960-3970, :520-664).  Host-only;
+ *      the caller runs the filter / prove launches for the batches it hands out. ---- */
+typedef struct ccs_campaign ccs_campaign;
+#define CCS_WALK_INHERIT 0          /* --prefix-mode not given: sequential with --start, else random */
+#define CCS_WALK_SEQUENTIAL 1
+#define CCS_WALK_RANDOM 2
+typedef struct {
+    int32_t bits, target, depth;   /* --bits, --target, --depth (checkpoint compatibility) */
+    const char* prefix;            /* --prefix "0b..." or NULL                              */
+    int32_t walk;                  /* --prefix-mode: CCS_WALK_*                            */
+    uint32_t lanes, lane_id;       /* --prefix-lanes, --prefix-lane-id                      */
+    int32_t has_start;             /* --start given                                        */
+    int32_t has_seed;              /* --seed given (else /dev/urandom; a given seed makes   */
+    uint64_t seed;                 /*   the random walk reproducible)                       */
+    ccs_u128 start;
+    uint32_t batch_tiles, _pad;    /* --batch                                               */
+    uint64_t max_tiles;            /* --tiles (0 = unlimited)                               */
+} ccs_campaign_params;
+typedef struct {
+    ccs_u128 tile_min, tile_max, current;
+    uint64_t seed, batches, total_tiles, total_candidates, total_survivors;
+    int32_t random_mode, done;
+    ccs_confirm_stats chains;
+} ccs_campaign_state;
+
+int ccs_campaign_create(const ccs_shape* shape, const ccs_campaign_params* params, ccs_campaign** out);
+/* the same over an explicit tile range (no shape needed) */
+int ccs_campaign_create_range(ccs_u128 tile_min, ccs_u128 tile_max, const ccs_campaign_params* params,
+                              ccs_campaign** out);
+void ccs_campaign_free(ccs_campaign* c);
+/* next batch [*tile_start, *tile_start + *tile_count): CCS_OK, or CCS_DONE */
+int ccs_campaign_next(ccs_campaign* c, ccs_u128* tile_start, uint32_t* tile_count);
+/* account a finished batch, in the order next() handed them out */
+int ccs_campaign_record(ccs_campaign* c, ccs_u128 tile_start, uint32_t tile_count, uint64_t candidates,
+                        uint64_t survivors, const ccs_chain_result* res, uint32_t n_res);
+int ccs_campaign_get_state(const ccs_campaign* c, ccs_campaign_state* out);
+/* CC18_CUDA_v10 checkpoint, written to path.tmp then renamed over path; the tile saved is
+ * the first one whose batch is not recorded yet */
+int ccs_campaign_save(const ccs_campaign* c, const char* path);
+/* continue sequentially from a checkpoint (bits, target, depth, prefix and lanes must match) */
+int ccs_campaign_resume(ccs_campaign* c, const char* path);
 
 /* ---- plain filter: K1 over tiles [tile_start, tile_start + tile_count), all bases
  *      (replaces prepare_batch v13.cu:1232-1259 + cc18_filter_kernel :1551-1714 and its
@@ -224,13 +297,6 @@ int ccs_format_results(const ccs_chain_result* res, uint32_t n, int32_t log_thre
  *      and its merge prove_survivors_mt :1967-2068): the same semantics on host threads with
  *      the library's own u128 Baillie-PSW instead of GMP.  Pure CPU work; usable without a
  *      GPU, e.g. to re-confirm the GPU's chains ---- */
-typedef struct {
-    uint64_t primes;            /* prime survivors (ProverStats.primes)               */
-    uint64_t chains;            /* chains followed from a true root                   */
-    uint64_t non_roots;         /* primes whose (n-1)/2 is prime (walked back)        */
-    uint32_t best_chain, _pad;
-    uint64_t chain_dist[65];    /* chains by length (v13 g_chain_dist)                */
-} ccs_confirm_stats;
 
 /* one result per value, same fields as the GPU prove; nthreads 0 = hardware threads
  * (at most 16) */
