@@ -75,7 +75,7 @@ def test_prefix_sweep_visits_every_tile_once(tmp_path, ctx):
 def test_host_prove_and_confirm_agree_with_gpu(tmp_path):
     t0, _ = WL.reference_range(seed=0x77, bits=89)
     a, b = tmp_path / "gpu.txt", tmp_path / "cpu.txt"
-    common = ["--bits", "89", "--log", "4", "--batch", "3000", "--start", str(t0), "--tiles", "6000",
+    common = ["--bits", "89", "--log", "3", "--batch", "3000", "--start", str(t0), "--tiles", "9000",
               "--report", "1000"]
     out = run(*common, "--output", str(a), "--confirm")
     assert "all logged chains agree" in out
