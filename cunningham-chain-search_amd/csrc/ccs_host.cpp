@@ -360,6 +360,7 @@ struct ccs_shape {
     DeepPrime* d_deep = nullptr;
     u32* d_hash = nullptr;
     u32 lds_words = 0, nd = 0, nd_primes = 0, nb = 0, nc = 0;   // nd: dense slots
+    u32 nb_lds = 0;                          // stage-B primes with their bitsets in LDS
     double est_survival_b = 1.0;             // expected fraction alive after stage B
 };
 
@@ -427,6 +428,27 @@ static u32 slot_period(const ccs_shape* s, std::pair<int, int> sl) {
 // default split: dense primes below 128 (at most 64), stage B up to 256 primes below 2^12
 static int validate_tuning(const ccs_shape* s, ccs_tuning* t);
 
+// Stage-B primes whose bitsets go to LDS: whole groups of 64, as many as fit beside the
+// dense patterns in the block's share of the CU's LDS (the rest stay in device memory);
+// ~0u when even the dense patterns and the first group do not fit.
+static u32 tierb_in_lds(const ccs_shape* s, const ccs_tuning* t) {
+    u32 words = 0;
+    for (auto sl : dense_slots(s, t)) words += dense_ext_words(slot_period(s, sl), t->rows_per_group);
+    const u32 bpc = t->blocks_per_cu ? t->blocks_per_cu : 1;
+    const u32 budget = std::max(s->max_lds_words / bpc, std::min(s->max_lds_words, 16384u));
+    u32 n = 0;
+    while (n < t->n_tierb) {
+        const u32 end = std::min(n + (u32)kTierBGroup, t->n_tierb);
+        u32 w = 0;
+        for (u32 i = n; i < end; ++i) w += (s->fp[t->n_dense + i] + 31) / 32;
+        if (words + w > budget) break;
+        words += w;
+        n = end;
+    }
+    if (words > s->max_lds_words || (t->n_tierb && n == 0)) return ~0u;
+    return n;
+}
+
 // Starting point measured on the 31# depth-18 workload: dense primes below 200 folded in
 // pairs up to period 2^16, eight rows per lane, stage B over the next 256 primes below
 // 4096, 2^20-candidate units, one block of 16 waves per CU.  Shrunk until the tables fit
@@ -435,7 +457,7 @@ static void default_tuning(const ccs_shape* s, ccs_tuning* t) {
     u32 nd = 0;
     while (nd < s->fp.size() && nd < 2u * (u32)kMaxDense && s->fp[nd] < 200) ++nd;
     u32 nb = 0;
-    while (nd + nb < s->fp.size() && nb < 256 && s->fp[nd + nb] < 4096) ++nb;
+    while (nd + nb < s->fp.size() && nb < (u32)kMaxTierB && s->fp[nd + nb] < 4096) ++nb;
     t->n_dense = nd;
     t->n_tierb = nb;
     t->rows_per_group = 8;
@@ -461,7 +483,7 @@ static int validate_tuning(const ccs_shape* s, ccs_tuning* t) {
     if (t->pair_max > (1u << 16)) return fail(CCS_ERR_INVALID, "pair_max > 65536");
     const auto slots = dense_slots(s, t);
     if (slots.size() > (size_t)kMaxDense) return fail(CCS_ERR_INVALID, "more than 64 dense patterns");
-    if (t->n_tierb > 256) return fail(CCS_ERR_INVALID, "n_tierb > 256");
+    if (t->n_tierb > (u32)kMaxTierB) return fail(CCS_ERR_INVALID, "n_tierb > 512");
     if (t->n_dense + t->n_tierb > s->fp.size())
         return fail(CCS_ERR_INVALID, "n_dense + n_tierb exceeds the filter primes");
     if (t->unit_j == 0 || t->unit_j % kRowBits || t->unit_j > (1u << 20))
@@ -473,10 +495,7 @@ static int validate_tuning(const ccs_shape* s, ccs_tuning* t) {
     if (t->waves_per_block != 1 && t->waves_per_block != 2 && t->waves_per_block != 4 &&
         t->waves_per_block != 8 && t->waves_per_block != 16)
         return fail(CCS_ERR_INVALID, "waves_per_block must be 1, 2, 4, 8 or 16");
-    u32 words = 0;
-    for (auto sl : slots) words += dense_ext_words(slot_period(s, sl), t->rows_per_group);
-    for (u32 i = 0; i < t->n_tierb && i < (u32)kTierBLds; ++i) words += (s->fp[t->n_dense + i] + 31) / 32;
-    if (words > s->max_lds_words) return fail(CCS_ERR_INVALID, "dense + stage-B tables exceed LDS");
+    if (tierb_in_lds(s, t) == ~0u) return fail(CCS_ERR_INVALID, "dense + stage-B tables exceed LDS");
     return CCS_OK;
 }
 
@@ -486,6 +505,8 @@ static int upload_tables(ccs_shape* s) {
     const u32 nd = t.n_dense, nb = t.n_tierb, nfp = (u32)s->fp.size(), nc = nfp - nd - nb;
     const u32 rg = t.rows_per_group;
     const auto slots = dense_slots(s, &t);
+    const u32 nb_lds = tierb_in_lds(s, &t);
+    if (nb_lds == ~0u) return fail(CCS_ERR_INVALID, "dense + stage-B tables exceed LDS");
     std::vector<u32> image;
     std::vector<DensePrime> dense(slots.empty() ? 1 : slots.size());
     std::vector<TierBPrime> tierb(nb ? nb : 1);
@@ -521,13 +542,13 @@ static int upload_tables(ccs_shape* s) {
             image.push_back(v);
         }
     }
-    // stage-B bitsets: the first group of 64 primes (tested for every dense survivor) in
-    // LDS, the later groups (reached by a few percent of them) in device memory
+    // stage-B bitsets: the first groups of 64 primes (the first is tested for every dense
+    // survivor) in LDS as far as it holds them, the later groups in device memory
     std::vector<u32> far_bits;
     for (u32 i = 0; i < nb; ++i) {
         const u32 q = s->fp[nd + i];
         std::vector<u8> kill = uset(nd + i);
-        std::vector<u32>& dst = i < (u32)kTierBLds ? image : far_bits;
+        std::vector<u32>& dst = i < nb_lds ? image : far_bits;
         TierBPrime b;
         b.q = q;
         b.lds_off = (u32)dst.size();
@@ -587,6 +608,7 @@ static int upload_tables(ccs_shape* s) {
     s->nd = (u32)slots.size();
     s->nd_primes = nd;
     s->nb = nb;
+    s->nb_lds = nb_lds;
     s->nc = nc;
     return CCS_OK;
 }
@@ -805,6 +827,7 @@ static void fill_sieve_common(const ccs_shape* s, const Scratch& x, SieveArgs& a
     a.lds_words = s->lds_words;
     a.nd = s->nd;
     a.nb = s->nb;
+    a.nb_lds = s->nb_lds;
     a.one = 1;
     a.unit_j = s->tune.unit_j;
     a.dense = s->d_dense;
@@ -1598,7 +1621,7 @@ static int autotune_impl(ccs_ctx* c, ccs_shape* s, const std::function<int(float
     auto try_tuning_nb = [&](ccs_tuning t) -> int {
         if (trials >= max_trials) return CCS_OK;
         u32 avail = 0;
-        while (t.n_dense + avail < s->fp.size() && avail < 256 && s->fp[t.n_dense + avail] < 4096) ++avail;
+        while (t.n_dense + avail < s->fp.size() && avail < (u32)kMaxTierB && s->fp[t.n_dense + avail] < 4096) ++avail;
         if (t.n_tierb > avail) t.n_tierb = avail;
         for (;;) {
             ccs_tuning tt = t;
@@ -1665,7 +1688,7 @@ static int autotune_impl(ccs_ctx* c, ccs_shape* s, const std::function<int(float
             t.unit_j = uj;
             if ((rc = try_tuning(t))) return rc;
         }
-        for (u32 nb : {32u, 64u, 128u, 192u, 256u}) {
+        for (u32 nb : {64u, 128u, 256u, 384u, 512u}) {
             ccs_tuning t = best;
             t.n_tierb = nb;
             if ((rc = try_tuning_nb(t))) return rc;

@@ -18,7 +18,8 @@ typedef uint8_t u8;
 constexpr int kRowBits = 2048;      // one row = 64 lanes x 32 candidates
 constexpr int kMaxDense = 64;       // dense primes held one per lane
 constexpr int kMaxHashSlots = 64;   // forbidden-residue hash per deep prime
-constexpr int kTierBLds = 64;       // stage-B primes whose bitsets sit in LDS (the first group)
+constexpr int kTierBGroup = 64;     // stage-B primes tested per wave step (one per lane)
+constexpr int kMaxTierB = 512;      // stage-B primes: at most 8 groups of 64
 
 // Per-modulus constants for exact residues of u64/u128 values: y = sum limb_k*2^(32k)
 // folded with c_k = 2^(32k) mod q (< 2^52 for q < 2^18), then reduced with a double
@@ -47,8 +48,8 @@ struct DensePrime {
 // Stage B: per-candidate test with a q-bit forbidden bitset in LDS.
 struct TierBPrime {
     u32 q;
-    u32 lds_off;    // dword offset of the q-bit bitset: in LDS for the first kTierBLds
-                    // primes, in SieveArgs::tierb_far for the others
+    u32 lds_off;    // dword offset of the q-bit bitset: in LDS for the first
+                    // SieveArgs::nb_lds primes, in SieveArgs::tierb_far for the others
     float rcp;      // 1.0f / q, for x mod q with x < 2^18
     u32 pad;
 };
@@ -104,11 +105,12 @@ struct SieveArgs {
     u32 lds_words;
     u32 nd;                   // dense slots (one or two primes each)
     u32 nb;                   // tier-B primes
+    u32 nb_lds;               // tier-B primes with bitsets in LDS (whole groups of 64)
     u32 unit_j;               // candidates (u values) per work unit
     u32 one;                  // 1 (see the dense loop)
     const DensePrime* dense;
     const TierBPrime* tierb;
-    const u32* tierb_far;     // bitsets of stage-B primes >= kTierBLds (lds_off indexes here)
+    const u32* tierb_far;     // bitsets of stage-B primes >= nb_lds (lds_off indexes here)
     const PrimeMod* pmod;     // [nd + nb] residue constants: dense slot moduli, then tier-B primes
     // work
     u64 n_units;

@@ -151,20 +151,48 @@ struct TierBRegs {
 // per lane (prime 64*i + lane in slot i), so one probe of every lane tests 64 primes at
 // once.  x = j + s < 2^21 and the host's reciprocal is rounded up so that the truncated
 // f32 product is exactly floor(x / q); the probe reads bit r of the prime's
-// forbidden-residue bitset.  Most candidates die in the first group of 64 primes (bitsets
-// in LDS), so the later groups (bitsets in device memory, L2-resident) are rarely read.
+// forbidden-residue bitset.  Most candidates die in the first group of 64 primes, so the
+// later groups are rarely read; their bitsets sit in LDS as far as it holds them, the rest
+// in device memory (L2-resident).
+// the probe of stage-B group i for candidate j: this lane's prime kills j
+template <int NBV>
+__device__ __forceinline__ bool stage_b_bit(const u32* __restrict__ mem, const TierBRegs<NBV>& R, int i, u32 j) {
+    const u32 x = j + R.s[i];
+    const int qt = (int)((float)x * __uint_as_float(R.rcp[i]));
+    const u32 r = (u32)(__mul24(qt, R.nq[i]) + (int)x);                       // x mod q
+    const u32 w = mem[R.off[i] + __builtin_amdgcn_ubfe(r, 5u, 11u)];
+    return __builtin_amdgcn_ubfe(w, r, 1u) != 0u;
+}
+
+// groups from..NBV-1 for one candidate (groups past the first live in device memory)
+template <int NBV>
+__device__ __forceinline__ bool stage_b_rest(const u32* __restrict__ lds, const u32* __restrict__ far,
+                                             u32 nb_lds, const TierBRegs<NBV>& R, const u64 (&valid)[NBV],
+                                             u32 j) {
+#pragma unroll
+    for (int i = 1; i < NBV; ++i)
+        if (ballot(stage_b_bit<NBV>((u32)i * 64u < nb_lds ? lds : far, R, i, j)) & valid[i]) return false;
+    return true;
+}
+
+// Two candidates at once: their first-group probes are independent, so both LDS reads are
+// in flight together; the rare candidates that pass the first group continue alone.
+template <int NBV>
+__device__ __forceinline__ void stage_b_pair(const u32* __restrict__ lds, const u32* __restrict__ far,
+                                             u32 nb_lds, const TierBRegs<NBV>& R, const u64 (&valid)[NBV],
+                                             u32 ja, u32 jb, bool& pa, bool& pb) {
+    const bool ba = stage_b_bit<NBV>(lds, R, 0, ja);
+    const bool bb = stage_b_bit<NBV>(lds, R, 0, jb);
+    pa = !(ballot(ba) & valid[0]) && stage_b_rest<NBV>(lds, far, nb_lds, R, valid, ja);
+    pb = !(ballot(bb) & valid[0]) && stage_b_rest<NBV>(lds, far, nb_lds, R, valid, jb);
+}
+
 template <int NBV>
 __device__ __forceinline__ bool stage_b_pass(const u32* __restrict__ lds, const u32* __restrict__ far,
-                                             const TierBRegs<NBV>& R, const u64 (&valid)[NBV], u32 j) {
-#pragma unroll
-    for (int i = 0; i < NBV; ++i) {
-        const u32 x = j + R.s[i];
-        const int qt = (int)((float)x * __uint_as_float(R.rcp[i]));
-        const u32 r = (u32)(__mul24(qt, R.nq[i]) + (int)x);                 // x mod q
-        const u32 w = (i == 0 ? lds : far)[R.off[i] + __builtin_amdgcn_ubfe(r, 5u, 11u)];
-        if (ballot(__builtin_amdgcn_ubfe(w, r, 1u) != 0u) & valid[i]) return false;
-    }
-    return true;
+                                             u32 nb_lds, const TierBRegs<NBV>& R, const u64 (&valid)[NBV],
+                                             u32 j) {
+    return !(ballot(stage_b_bit<NBV>(lds, R, 0, j)) & valid[0]) &&
+           stage_b_rest<NBV>(lds, far, nb_lds, R, valid, j);
 }
 
 template <int RG, int NBV>
@@ -282,8 +310,21 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                 }
                 alive[r] = v;
             }
-            // survivors of the dense stage, one at a time in uniform control flow: the
-            // lanes with a non-empty word (ballot), then that word's bits (readlane)
+            // survivors of the dense stage in uniform control flow: the lanes with a
+            // non-empty word (ballot), then that word's bits (readlane); stage B takes them
+            // two at a time
+            auto emit = [&](u32 j) {
+                if (lane == 0) {
+                    const u32 slot = atomicAdd(A.mid_count, 1u);
+                    if (slot < A.mid_cap) {
+                        MidRec rec;
+                        rec.owner = U.owner;
+                        rec.j = U.j0 + j;
+                        A.mid[slot] = rec;
+                    }
+                }
+            };
+            u32 pend = 0xffffffffu;
 #pragma unroll
             for (int r = 0; r < RG; ++r) {
                 u64 m = ballot(alive[r] != 0u);
@@ -296,18 +337,19 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                     while (w) {
                         const u32 j = jb + (u32)__builtin_ctz(w);
                         w &= w - 1u;
-                        if (stage_b_pass<NBV>(lds, A.tierb_far, R, valid, j) && lane == 0) {
-                            const u32 slot = atomicAdd(A.mid_count, 1u);
-                            if (slot < A.mid_cap) {
-                                MidRec rec;
-                                rec.owner = U.owner;
-                                rec.j = U.j0 + j;
-                                A.mid[slot] = rec;
-                            }
+                        if (pend == 0xffffffffu) {
+                            pend = j;
+                            continue;
                         }
+                        bool pa, pb;
+                        stage_b_pair<NBV>(lds, A.tierb_far, A.nb_lds, R, valid, pend, j, pa, pb);
+                        if (pa) emit(pend);
+                        if (pb) emit(j);
+                        pend = 0xffffffffu;
                     }
                 }
             }
+            if (pend != 0xffffffffu && stage_b_pass<NBV>(lds, A.tierb_far, A.nb_lds, R, valid, pend)) emit(pend);
         }
     }
 
@@ -564,10 +606,11 @@ hipError_t launch_sieve(const SieveArgs& a, int rg, int nbv, int grid, int block
                         hipStream_t s) {
 #define CCS_SIEVE_CASE(R, B) \
     if (rg == R && nbv == B) return launch_sieve_t<R, B>(a, grid, block, lds_bytes, s);
-    CCS_SIEVE_CASE(1, 1) CCS_SIEVE_CASE(1, 2) CCS_SIEVE_CASE(1, 3) CCS_SIEVE_CASE(1, 4)
-    CCS_SIEVE_CASE(2, 1) CCS_SIEVE_CASE(2, 2) CCS_SIEVE_CASE(2, 3) CCS_SIEVE_CASE(2, 4)
-    CCS_SIEVE_CASE(4, 1) CCS_SIEVE_CASE(4, 2) CCS_SIEVE_CASE(4, 3) CCS_SIEVE_CASE(4, 4)
-    CCS_SIEVE_CASE(8, 1) CCS_SIEVE_CASE(8, 2) CCS_SIEVE_CASE(8, 3) CCS_SIEVE_CASE(8, 4)
+#define CCS_SIEVE_ROW(R)                                                                      \
+    CCS_SIEVE_CASE(R, 1) CCS_SIEVE_CASE(R, 2) CCS_SIEVE_CASE(R, 3) CCS_SIEVE_CASE(R, 4)      \
+    CCS_SIEVE_CASE(R, 5) CCS_SIEVE_CASE(R, 6) CCS_SIEVE_CASE(R, 7) CCS_SIEVE_CASE(R, 8)
+    CCS_SIEVE_ROW(1) CCS_SIEVE_ROW(2) CCS_SIEVE_ROW(4) CCS_SIEVE_ROW(8)
+#undef CCS_SIEVE_ROW
 #undef CCS_SIEVE_CASE
     return hipErrorInvalidValue;
 }
