@@ -121,20 +121,21 @@ def measure_batched(ccs, WL, ctx, n_jobs=4096, tune_trials=40):
             "tuning": tuned}
 
 
-def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=30):
+def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_candidates=1 << 36):
     """Config 5: a Poisson stream of search jobs (depth U{12..20}, width U{64,80,96}, random
     admissible class mod 31#, length log-uniform in [2^24, 2^34]) at `load` times the server's
-    measured capacity; p50 / p99 latency from submission to completion."""
+    measured capacity; p50 / p99 latency from submission to completion.  Each depth's shape
+    is autotuned on its own sample jobs (the stage cutoffs that suit depth 12 and depth 20
+    differ)."""
     import numpy as np
     shapes = {d: WL.make_shape(ctx, f"c5d{d}", classes=False) for d in range(12, 21)}
-    tune_reqs = [r for r in WL.served_requests(n_jobs=600, seed=0x7E) if r["depth"] == 16][:256]
-    tuned, _ = shapes[16].autotune_jobs([(r["a"], r["u0"], min(r["count"], 1 << 26)) for r in tune_reqs],
-                                        max_trials=tune_trials)
+    tune_pool = WL.served_requests(n_jobs=2000, seed=0x7E)
+    tuned = {}
     for d, sh in shapes.items():
-        if d != 16:
-            sh.set_tuning(**tuned)
+        sample = [(r["a"], r["u0"], min(r["count"], 1 << 27)) for r in tune_pool if r["depth"] == d][:192]
+        tuned[d], _ = sh.autotune_jobs(sample, max_trials=tune_trials)
     cal = WL.served_requests(n_jobs=400, seed=0xCA1)
-    srv = ccs.Server(ctx, shapes)
+    srv = ccs.Server(ctx, shapes, slice_candidates=slice_candidates)
     t = time.perf_counter()
     for r in cal:
         srv.submit(r["depth"], r["a"], r["u0"], r["count"])
@@ -142,6 +143,8 @@ def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=30):
     capacity = sum(r["count"] for r in cal) / (time.perf_counter() - t)
     while srv.poll():
         pass
+    srv.shutdown()
+    srv = ccs.Server(ctx, shapes, slice_candidates=slice_candidates)
     reqs = WL.served_requests(n_jobs=n_jobs)
     mean = sum(r["count"] for r in reqs) / len(reqs)
     lam = load * capacity / mean
@@ -172,7 +175,7 @@ def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=30):
             "aggregate_candidates_per_s": total / wall, "wall_s": wall,
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_ms": float(lat.mean() * 1e3), "slices": st["slices"], "launches": st["launches"],
-            "tuning": tuned}
+            "slice_candidates": slice_candidates, "tuning": {str(d): t for d, t in tuned.items()}}
 
 
 def main():

@@ -315,6 +315,13 @@ class JobSet:
         except Exception:
             pass
 
+    def update(self, jobs, stream=None):
+        """Replace the jobs (at most as many as the set was built with): ccs_jobset_update."""
+        arr = jobs_array(jobs)
+        L.check(self.lib.ccs_jobset_update(self.ctx.h, self.h, arr.ctypes.data, len(arr), stream), "jobset_update")
+        self.jobs = arr
+        self.candidates = int(arr["count"].sum())
+
     def launch(self, d_out, cap, d_count, stream=None):
         L.check(self.lib.ccs_jobset_launch(self.ctx.h, self.h, d_out, cap, d_count, stream), "jobset_launch")
 

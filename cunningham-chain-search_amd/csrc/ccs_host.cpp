@@ -858,6 +858,7 @@ static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, 
     u64 want_grid = (u64)c->n_cu * s->tune.blocks_per_cu;
     const u64 blocks_needed = (n_units + wpb - 1) / wpb;
     if (want_grid > blocks_needed) want_grid = blocks_needed ? blocks_needed : 1;
+    sa.sched_div = 4 * want_grid * wpb;   // claims of a quarter of each wave's fair share
     c->last_timed = c->timed;
     if (c->timed) HIPCHK(hipEventRecord(c->ev[0], st));
     prof_begin(c, 0, st);
@@ -1142,11 +1143,15 @@ struct ccs_jobset {
     u64* d_prefix = nullptr;
     std::vector<u64> prefix;
     u64 n_units = 0, candidates = 0;
-    u32 unit_j = 0;                 // the shape's unit size when the unit prefix was built
+    u32 unit_j = 0;                 // candidates per work unit of this set (set_unit)
     u32 cap_jobs = 0;
     JobRec* staging = nullptr;      // pinned upload staging of ccs_jobset_update
     u64* staging_prefix = nullptr;
     hipEvent_t staged = nullptr;    // the last staging upload has been consumed
+    std::vector<u64> counts;        // candidates of each job (host copy)
+    u64* d_each = nullptr;          // per-job {0, units} pairs of the one-job-per-launch loop
+    std::vector<u32> each_unit;     // unit size of each job launched on its own
+    bool each_valid = false;
     void* chain = nullptr;          // prove chain queue of launches on this set
     u64 chain_bytes = 0;
     Scratch x = {nullptr, 0, nullptr, nullptr};
@@ -1161,10 +1166,37 @@ extern "C" void ccs_jobset_free(ccs_jobset* js) {
     (void)hipFree(js->x.counters);
     (void)hipFree(js->x.stats);
     (void)hipFree(js->chain);
+    (void)hipFree(js->d_each);
     if (js->staging) (void)hipHostFree(js->staging);
     if (js->staging_prefix) (void)hipHostFree(js->staging_prefix);
     if (js->staged) (void)hipEventDestroy(js->staged);
     delete js;
+}
+
+// Work-unit size of a job set: the shape's unit_j, halved (down to kMinSetUnit) while the
+// set would give a full-chip launch fewer than kUnitsPerWave units per wave.  A small set
+// (a served slice) then still spreads over every wave and the launch's tail stays short;
+// a large one keeps the tuned size.  Units only divide the work: any size gives the same
+// survivors.
+template <class Count>
+static u32 set_unit(const ccs_ctx* c, const ccs_shape* s, u32 n_jobs, Count count) {
+    constexpr u32 kMinSetUnit = 32768;
+    constexpr u64 kUnitsPerWave = 8;
+    const u64 waves = (u64)c->n_cu * s->tune.blocks_per_cu * s->tune.waves_per_block;
+    u32 unit = s->tune.unit_j;
+    while (unit > kMinSetUnit) {
+        u64 units = 0;
+        for (u32 i = 0; i < n_jobs && units < kUnitsPerWave * waves; ++i) units += (count(i) + unit - 1) / unit;
+        if (units >= kUnitsPerWave * waves) break;
+        const u32 half = (unit / 2) / kRowBits * kRowBits;
+        if (half < kMinSetUnit) break;
+        unit = half;
+    }
+    return unit;
+}
+
+static u32 set_unit(const ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, u32 n_jobs) {
+    return set_unit(c, s, n_jobs, [&](u32 i) { return jobs[i].count; });
 }
 
 extern "C" int ccs_jobset_create(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, uint32_t n_jobs,
@@ -1182,17 +1214,20 @@ extern "C" int ccs_jobset_create(ccs_ctx* c, const ccs_shape* s, const ccs_job* 
     for (u32 i = 0; i < n_jobs; ++i) {
         if (jobs[i].a >= s->W) { delete js; return fail(CCS_ERR_INVALID, "job residue a must be < W"); }
         if (jobs[i].count > (1ull << 32)) { delete js; return fail(CCS_ERR_INVALID, "job count must be <= 2^32"); }
+    }
+    js->unit_j = set_unit(c, s, jobs, n_jobs);
+    for (u32 i = 0; i < n_jobs; ++i) {
         jr[i].a = jobs[i].a;
         jr[i].u0_lo = jobs[i].u0.lo;
         jr[i].u0_hi = jobs[i].u0.hi;
         jr[i].count = jobs[i].count;
         js->prefix[i] = js->n_units;
-        js->n_units += (jobs[i].count + s->tune.unit_j - 1) / s->tune.unit_j;
+        js->n_units += (jobs[i].count + js->unit_j - 1) / js->unit_j;
         js->candidates += jobs[i].count;
+        js->counts.push_back(jobs[i].count);
     }
     js->prefix[n_jobs] = js->n_units;
     js->cap_jobs = n_jobs;
-    js->unit_j = s->tune.unit_j;
     u64 want = (u64)(js->candidates * s->est_survival_b * 2.0) + (1u << 20);
     if (want > (1u << 28)) want = 1u << 28;
     hipError_t e = hipMalloc((void**)&js->d_jobs, (size_t)n_jobs * sizeof(JobRec));
@@ -1236,17 +1271,20 @@ extern "C" int ccs_jobset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jo
     if ((u64)(cand * s->est_survival_b * 2.0) + (1u << 16) > js->x.mid_cap)
         return fail(CCS_ERR_INVALID, "jobs exceed the stage-B buffer of the set");
     js->prefix.resize((size_t)n_jobs + 1);
+    js->unit_j = set_unit(c, s, h_jobs, n_jobs);
+    js->counts.resize(n_jobs);
+    js->each_valid = false;
     for (u32 i = 0; i < n_jobs; ++i) {
+        js->counts[i] = h_jobs[i].count;
         JobRec& r = js->staging[i];
         r.a = h_jobs[i].a;
         r.u0_lo = h_jobs[i].u0.lo;
         r.u0_hi = h_jobs[i].u0.hi;
         r.count = h_jobs[i].count;
         js->prefix[i] = js->staging_prefix[i] = units;
-        units += (h_jobs[i].count + s->tune.unit_j - 1) / s->tune.unit_j;
+        units += (h_jobs[i].count + js->unit_j - 1) / js->unit_j;
     }
     js->prefix[n_jobs] = js->staging_prefix[n_jobs] = units;
-    js->unit_j = s->tune.unit_j;
     js->n_units = units;
     js->candidates = cand;
     js->n_jobs = n_jobs;
@@ -1257,15 +1295,10 @@ extern "C" int ccs_jobset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jo
     return CCS_OK;
 }
 
-static int jobset_check(const ccs_jobset* js) {
-    if (js->unit_j != js->shape->tune.unit_j)
-        return fail(CCS_ERR_INVALID, "the shape's tuning changed since the job set was built: rebuild or update it");
-    return CCS_OK;
-}
-
 static void jobset_args(const ccs_jobset* js, SieveArgs& sa, DeepArgs& da, void* d_out, u32 cap) {
     const ccs_shape* s = js->shape;
     fill_sieve_common(s, js->x, sa);
+    sa.unit_j = js->unit_j;
     sa.mode = 1;
     sa.jobsrc.jobs = js->d_jobs;
     sa.jobsrc.unit_prefix = js->d_prefix;
@@ -1294,7 +1327,6 @@ extern "C" int ccs_jobset_launch(ccs_ctx* c, ccs_jobset* js, ccs_job_survivor* d
     int rc = ctx_check(c);
     if (rc) return rc;
     if (!js || !d_out || !d_count) return fail(CCS_ERR_INVALID, "null argument");
-    if ((rc = jobset_check(js))) return rc;
     SieveArgs sa;
     DeepArgs da;
     jobset_args(js, sa, da, d_out, cap);
@@ -1310,18 +1342,36 @@ extern "C" int ccs_jobset_launch_each(ccs_ctx* c, ccs_jobset* js, ccs_job_surviv
     int rc = ctx_check(c);
     if (rc) return rc;
     if (!js || !d_out || !d_count) return fail(CCS_ERR_INVALID, "null argument");
-    if ((rc = jobset_check(js))) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    // each job launched as a set of its own would be: its own unit size and {0, units} prefix
+    if (!js->each_valid) {
+        if (!js->d_each) HIPCHK(hipMalloc((void**)&js->d_each, (size_t)js->cap_jobs * 16));
+        std::vector<u64> pairs(2 * (size_t)js->n_jobs);
+        js->each_unit.resize(js->n_jobs);
+        for (u32 i = 0; i < js->n_jobs; ++i) {
+            const u64 n = js->counts[i];
+            js->each_unit[i] = set_unit(c, js->shape, 1, [&](u32) { return n; });
+            pairs[2 * i] = 0;
+            pairs[2 * i + 1] = (n + js->each_unit[i] - 1) / js->each_unit[i];
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipMemcpy(js->d_each, pairs.data(), pairs.size() * 8, hipMemcpyHostToDevice));
+        js->each_valid = true;
+    }
     jobset_stats_begin(c, js);
     if (launch_zero(d_count, 1, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
     if (launch_zero((u32*)js->x.stats, 16, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
     for (u32 i = 0; i < js->n_jobs; ++i) {
-        const u64 units = js->prefix[i + 1] - js->prefix[i];
-        if (!units) continue;
+        if (!js->counts[i]) continue;
+        const u64 units = (js->counts[i] + js->each_unit[i] - 1) / js->each_unit[i];
         SieveArgs sa;
         DeepArgs da;
-        jobset_args(js, sa, da, d_out, cap);
-        sa.unit_base = js->prefix[i];
+        jobset_args(js, sa, da, d_out, cap);   // the deep kernel keeps the whole job table
+        sa.unit_j = js->each_unit[i];
+        sa.jobsrc.jobs = js->d_jobs + i;
+        sa.jobsrc.unit_prefix = js->d_each + 2 * (size_t)i;
+        sa.jobsrc.n_jobs = 1;
+        sa.jobsrc.job_base = i;
         rc = run_filter_kernels(c, js->shape, js->x, sa, da, units, d_count, st, false, false);
         if (rc) return rc;
     }
@@ -1361,7 +1411,6 @@ extern "C" int ccs_graph_create(ccs_ctx* c, ccs_jobset* js, ccs_job_survivor* d_
     int rc = ctx_check(c);
     if (rc) return rc;
     if (!js || !d_out || !d_count || !out) return fail(CCS_ERR_INVALID, "null argument");
-    if ((rc = jobset_check(js))) return rc;
     *out = nullptr;
     ccs_graph* g = new ccs_graph();
     g->device = c->device;
@@ -1554,7 +1603,7 @@ static int time_filter(ccs_ctx* c, ccs_shape* s, ccs_u128 ts, uint32_t tile_coun
 }
 
 // time the batched launch over a sample of jobs (a job set is rebuilt for every tuning,
-// since its unit prefix depends on unit_j)
+// since its unit size follows unit_j and the launch shape)
 static int time_jobs(ccs_ctx* c, ccs_shape* s, const ccs_job* jobs, u32 n_jobs, int reps, float* ms_out) {
     int rc = ensure_buf(&c->out, &c->out_bytes, (u64)(1u << 20) * sizeof(ccs_job_survivor) + 256);
     if (rc) return rc;

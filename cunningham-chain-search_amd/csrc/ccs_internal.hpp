@@ -95,7 +95,7 @@ struct JobSrc {
     const JobRec* jobs;
     const u64* unit_prefix;   // [n_jobs + 1] first unit index of each job
     u32 n_jobs;
-    u32 pad;
+    u32 job_base;             // added to the job index recorded as a survivor's owner
 };
 
 struct SieveArgs {
@@ -116,6 +116,7 @@ struct SieveArgs {
     u64 n_units;
     u64 unit_base;            // added to the cursor value: first unit of this launch
     u64* unit_counter;        // work-stealing cursor (zeroed before launch)
+    u64 sched_div;            // a claim takes max(1, remaining units / sched_div), <= 64
     int mode;                 // 0 = plain, 1 = jobs
     PlainSrc plain;
     JobSrc jobsrc;

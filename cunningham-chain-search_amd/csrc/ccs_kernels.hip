@@ -129,7 +129,7 @@ __device__ __forceinline__ bool resolve(const SieveArgs& A, u64 unit, u32 lane, 
     U.us_hi = jr.u0_hi;
     add128(U.us_lo, U.us_hi, beg);
     U.n = (u32)n;
-    U.owner = job;
+    U.owner = job + J.job_base;
     U.j0 = (u32)beg;
     return true;
 }
@@ -228,14 +228,53 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
         valid[i] = ballot(p < A.nb);
     }
 
-    // work stealing; the next unit's index is fetched while the current one is sieved
+    // Stage-B survivors wait in registers (lane k holds the k-th) and go out 64 at a time
+    // with one atomic, so the shared output cursor is not a point of contention when
+    // survivors are dense (low depths).
+    u32 q_n = 0, q_owner = 0, q_j = 0;
+    auto flush = [&]() {
+        if (!q_n) return;
+        u32 base = 0;
+        if (lane == 0) base = atomicAdd(A.mid_count, q_n);
+        base = bcast32(base);
+        if (lane < q_n && base + lane < A.mid_cap) {
+            MidRec rec;
+            rec.owner = q_owner;
+            rec.j = q_j;
+            A.mid[base + lane] = rec;
+        }
+        q_n = 0;
+    };
+
+    // Work stealing by guided self-scheduling: a claim takes max(1, remaining / sched_div)
+    // consecutive units (capped at 64) with one atomic, so the shared cursor sees a few
+    // claims per wave rather than one per unit.  The next claim is in flight while the
+    // current one is sieved.
+    auto claim_size = [&](u64 claimed_to) -> u32 {
+        const u64 rem = claimed_to < A.n_units ? A.n_units - claimed_to : 0;
+        u64 b = rem / A.sched_div;
+        return (u32)(b < 1 ? 1 : (b > 64 ? 64 : b));
+    };
+    // the first unit of wave w is unit w, with no atomic; claims number the units after those
+    const u64 n_waves = (u64)gridDim.x * (blockDim.x >> 6);
+    const u64 wave_id = (u64)blockIdx.x * (blockDim.x >> 6) + (tid >> 6);
+    const u64 n_static = n_waves < A.n_units ? n_waves : A.n_units;
+    u32 next_b = claim_size(n_static);
     u64 next = 0;
-    if (lane == 0) next = atomicAdd(A.unit_counter, 1ull);
+    if (lane == 0) next = n_static + atomicAdd(A.unit_counter, (u64)next_b);
+    u64 cur = wave_id, cur_end = wave_id < n_static ? wave_id + 1 : wave_id;
+    u32 sd = 0;                                             // dense phase (lane < nd)
+    u32 prev_owner = 0xffffffffu, prev_j0 = 0, prev_n = 0;  // the wave's previous unit
     for (;;) {
-        u64 unit = bcast64(next);
-        if (unit >= A.n_units) break;
-        if (lane == 0) next = atomicAdd(A.unit_counter, 1ull);
-        unit += A.unit_base;
+        if (cur == cur_end) {
+            cur = bcast64(next);
+            if (cur >= A.n_units) break;
+            cur_end = cur + next_b < A.n_units ? cur + next_b : A.n_units;
+            next_b = claim_size(cur_end);
+            if (lane == 0) next = n_static + atomicAdd(A.unit_counter, (u64)next_b);
+        }
+        const u64 unit = cur + A.unit_base;
+        ++cur;
         Unit U;
         if (!resolve(A, unit, lane, U)) continue;
         U.a = bcast64(U.a);
@@ -246,14 +285,29 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
         U.j0 = bcast32(U.j0);
 
         // phases: dense prime p held by lane p as an LDS bit address, stage-B prime
-        // (64*i + lane) held in R.s[i]
-        u32 sd = 0;
-        if (lane < A.nd) sd = doff * 32u + phase(A.pmod[lane], U.a, U.us_lo, U.us_hi);
+        // (64*i + lane) held in R.s[i].  A unit that continues the previous one (same
+        // owner, next u, previous unit a whole number of row groups) keeps the dense phases
+        // where the row loop left them and advances the stage-B phases by the previous
+        // unit's length; otherwise the phases are computed from u_start.
+        if (U.owner == prev_owner && U.j0 == prev_j0 + prev_n && prev_n % (u32)(RG * kRowBits) == 0u) {
 #pragma unroll
-        for (int i = 0; i < NBV; ++i) {
-            const u32 p = (u32)i * 64u + lane;
-            R.s[i] = p < A.nb ? phase(A.pmod[A.nd + p], U.a, U.us_lo, U.us_hi) : 0u;
+            for (int i = 0; i < NBV; ++i) {   // x < 4096 + 2^20: the probe's exact quotient
+                const u32 x = R.s[i] + prev_n;
+                const int qt = (int)((float)x * __uint_as_float(R.rcp[i]));
+                R.s[i] = (u32)(__mul24(qt, R.nq[i]) + (int)x);
+            }
+        } else {
+            sd = 0;
+            if (lane < A.nd) sd = doff * 32u + phase(A.pmod[lane], U.a, U.us_lo, U.us_hi);
+#pragma unroll
+            for (int i = 0; i < NBV; ++i) {
+                const u32 p = (u32)i * 64u + lane;
+                R.s[i] = p < A.nb ? phase(A.pmod[A.nd + p], U.a, U.us_lo, U.us_hi) : 0u;
+            }
         }
+        prev_owner = U.owner;
+        prev_j0 = U.j0;
+        prev_n = U.n;
 
         const u32 rows = (U.n + kRowBits - 1) / kRowBits;
         for (u32 row0 = 0; row0 < rows; row0 += RG) {
@@ -314,15 +368,11 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
             // non-empty word (ballot), then that word's bits (readlane); stage B takes them
             // two at a time
             auto emit = [&](u32 j) {
-                if (lane == 0) {
-                    const u32 slot = atomicAdd(A.mid_count, 1u);
-                    if (slot < A.mid_cap) {
-                        MidRec rec;
-                        rec.owner = U.owner;
-                        rec.j = U.j0 + j;
-                        A.mid[slot] = rec;
-                    }
+                if (lane == q_n) {
+                    q_owner = U.owner;
+                    q_j = U.j0 + j;
                 }
+                if (++q_n == 64u) flush();
             };
             u32 pend = 0xffffffffu;
 #pragma unroll
@@ -352,11 +402,20 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
             if (pend != 0xffffffffu && stage_b_pass<NBV>(lds, A.tierb_far, A.nb_lds, R, valid, pend)) emit(pend);
         }
     }
+    flush();
 
     if (A.stats) {   // cnt_dense is uniform (counted from readlane values)
-        if (lane == 0) {
-            atomicAdd(&A.stats[0], (unsigned long long)cnt_dense);
-            atomicAdd(&A.stats[1], (unsigned long long)cnt_dense);
+        // one atomic per block: the waves' counts meet in LDS once every wave is past its
+        // last table read
+        unsigned long long* acc = (unsigned long long*)lds;
+        __syncthreads();
+        if (tid == 0) *acc = 0;
+        __syncthreads();
+        if (lane == 0 && cnt_dense) atomicAdd(acc, (unsigned long long)cnt_dense);
+        __syncthreads();
+        if (tid == 0 && *acc) {
+            atomicAdd(&A.stats[0], *acc);
+            atomicAdd(&A.stats[1], *acc);
         }
     }
 }

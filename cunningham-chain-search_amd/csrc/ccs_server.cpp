@@ -5,7 +5,9 @@
 // chunk of every admitted job, grouped by depth profile, runs one persistent batched
 // filter launch + one prove launch per group, and hands results back per job.  Requests
 // that arrive while slices are on the GPU join the next slice.  Two buffer lanes
-// alternate so the GPU runs slice k+1 while the host distributes slice k.
+// alternate so the GPU runs slice k+1 while the host distributes slice k; within a lane
+// the proves run on a stream of their own, behind an event per group, so they overlap the
+// next groups' filter launches.
 //
 // The reference has no serving mode; its closest pattern is the v16 dispatch ring with
 // an asynchronous prove pool (v16...cu:2938-3279, :5594-5965).
@@ -60,6 +62,7 @@ struct Lane {           // buffers of one in-flight slice for one depth group
     uint32_t* h_count = nullptr;          // pinned
     std::vector<ccs_job> jobs;
     std::vector<Entry> entries;
+    hipEvent_t filtered = nullptr;        // the group's filter launch is done (prove may start)
     bool busy = false;
 };
 
@@ -70,7 +73,8 @@ struct ccs_server {
     ccs_server_params p{};
     std::vector<const ccs_shape*> shapes;   // index = depth group
     std::vector<int32_t> depths;
-    hipStream_t streams[2] = {nullptr, nullptr};
+    hipStream_t streams[2] = {nullptr, nullptr};    // per lane: filter launches
+    hipStream_t pstreams[2] = {nullptr, nullptr};   // per lane: prove launches and copies
     std::vector<Lane> lanes[2];             // [lane][group]
     std::mutex mu;
     std::condition_variable cv;
@@ -107,7 +111,8 @@ static int collect_lane(ccs_server* s, int li) {
     bool any = false;
     for (auto& L : s->lanes[li]) any |= L.busy;
     if (!any) return CCS_OK;
-    if (hipStreamSynchronize(s->streams[li]) != hipSuccess) return CCS_ERR_HIP;
+    if (hipStreamSynchronize(s->streams[li]) != hipSuccess || hipStreamSynchronize(s->pstreams[li]) != hipSuccess)
+        return CCS_ERR_HIP;
     const double t = now_s();
     std::lock_guard<std::mutex> g(s->mu);
     for (auto& L : s->lanes[li]) {
@@ -227,20 +232,25 @@ static void server_loop(ccs_server* s) {
             L.entries = plan[gi];
             int rc = ccs_jobset_update(s->ctx, L.js, L.jobs.data(), (uint32_t)L.jobs.size(), s->streams[li]);
             if (!rc) rc = ccs_jobset_launch(s->ctx, L.js, L.d_surv, s->p.max_slice_survivors, L.d_count, s->streams[li]);
+            // the prove of this group runs on the lane's prove stream, so the next group's
+            // filter launch does not wait behind the prove's latency-bound tail
+            if (!rc && (hipEventRecord(L.filtered, s->streams[li]) != hipSuccess ||
+                        hipStreamWaitEvent(s->pstreams[li], L.filtered, 0) != hipSuccess))
+                rc = CCS_ERR_HIP;
             if (!rc) {
                 ccs_prove_params pp = s->p.prove;
                 rc = ccs_prove_jobset_launch(s->ctx, L.js, L.d_surv, L.d_count, s->p.max_slice_survivors, &pp,
-                                             L.d_res, s->streams[li]);
+                                             L.d_res, s->pstreams[li]);
             }
-            if (!rc && hipMemcpyAsync(L.h_count, L.d_count, 4, hipMemcpyDeviceToHost, s->streams[li]) != hipSuccess)
+            if (!rc && hipMemcpyAsync(L.h_count, L.d_count, 4, hipMemcpyDeviceToHost, s->pstreams[li]) != hipSuccess)
                 rc = CCS_ERR_HIP;
             // survivors are rare: copy a bounded prefix now, the rest (if any) at collection
             const size_t pre = std::min<uint32_t>(kEagerCopy, s->p.max_slice_survivors);
             if (!rc && hipMemcpyAsync(L.h_surv, L.d_surv, pre * sizeof(ccs_job_survivor),
-                                      hipMemcpyDeviceToHost, s->streams[li]) != hipSuccess)
+                                      hipMemcpyDeviceToHost, s->pstreams[li]) != hipSuccess)
                 rc = CCS_ERR_HIP;
             if (!rc && hipMemcpyAsync(L.h_res, L.d_res, pre * sizeof(ccs_chain_result),
-                                      hipMemcpyDeviceToHost, s->streams[li]) != hipSuccess)
+                                      hipMemcpyDeviceToHost, s->pstreams[li]) != hipSuccess)
                 rc = CCS_ERR_HIP;
             if (rc) { server_fail(s, rc, "slice launch"); return; }
             L.busy = true;
@@ -277,7 +287,9 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
     }
     int rc = CCS_OK;
     for (int li = 0; li < 2 && !rc; ++li) {
-        if (hipStreamCreateWithFlags(&s->streams[li], hipStreamNonBlocking) != hipSuccess) rc = CCS_ERR_HIP;
+        if (hipStreamCreateWithFlags(&s->streams[li], hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&s->pstreams[li], hipStreamNonBlocking) != hipSuccess)
+            rc = CCS_ERR_HIP;
         s->lanes[li].resize(n_shapes);
         for (uint32_t gi = 0; gi < n_shapes && !rc; ++gi) {
             Lane& L = s->lanes[li][gi];
@@ -293,6 +305,7 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
             if (!rc && hipHostMalloc((void**)&L.h_surv, (size_t)s->p.max_slice_survivors * sizeof(ccs_job_survivor)) != hipSuccess) rc = CCS_ERR_NOMEM;
             if (!rc && hipHostMalloc((void**)&L.h_res, (size_t)s->p.max_slice_survivors * sizeof(ccs_chain_result)) != hipSuccess) rc = CCS_ERR_NOMEM;
             if (!rc && hipHostMalloc((void**)&L.h_count, 64) != hipSuccess) rc = CCS_ERR_NOMEM;
+            if (!rc && hipEventCreateWithFlags(&L.filtered, hipEventDisableTiming) != hipSuccess) rc = CCS_ERR_HIP;
         }
     }
     if (rc) {
@@ -394,7 +407,9 @@ extern "C" int ccs_server_shutdown(ccs_server* s) {
     if (s->th.joinable()) s->th.join();
     for (int li = 0; li < 2; ++li) {
         if (s->streams[li]) (void)hipStreamSynchronize(s->streams[li]);
+        if (s->pstreams[li]) (void)hipStreamSynchronize(s->pstreams[li]);
         for (auto& L : s->lanes[li]) {
+            if (L.filtered) (void)hipEventDestroy(L.filtered);
             if (L.js) ccs_jobset_free(L.js);
             if (L.d_surv) ccs_dfree(s->ctx, L.d_surv);
             if (L.d_res) ccs_dfree(s->ctx, L.d_res);
@@ -404,6 +419,7 @@ extern "C" int ccs_server_shutdown(ccs_server* s) {
             if (L.h_count) (void)hipHostFree(L.h_count);
         }
         if (s->streams[li]) (void)hipStreamDestroy(s->streams[li]);
+        if (s->pstreams[li]) (void)hipStreamDestroy(s->pstreams[li]);
     }
     const int err = s->error;
     delete s;

@@ -109,13 +109,33 @@ def test_autotune_jobs_keeps_results(ctx, oracle):
     assert (st["candidates"], st["alive_dense"], st["alive_tierb"], st["survivors"]) == tuple(alive)
 
 
-def test_jobset_rejects_stale_tuning(ctx):
-    jobs, W = WL.batched_jobs(n_jobs=4, seed=0xB8, lo_exp=10, hi_exp=12)
-    s = WL.make_shape(ctx, "c4", classes=False)
+def test_jobset_unit_size_and_tuning_change(ctx, oracle):
+    """A set sizes its own work units (smaller for small sets, per job in the per-job loop):
+    results do not depend on the unit size, on a tuning change after the set was built, or
+    on an update to fewer jobs."""
+    jobs, W = WL.batched_jobs(n_jobs=24, seed=0xB8, lo_exp=14, hi_exp=21)
+    s = WL.make_shape(ctx, "c4", classes=False, depth=8)
     js = ccs.JobSet(s, jobs)
+    cap = 1 << 18
+    d_out, d_cnt = ctx.dalloc(cap * 8), ctx.dalloc(64)
+    say("[batch] unit sizes")
+    js.launch(d_out, cap, d_cnt)
+    ctx.synchronize()
+    st0 = js.stats(d_cnt)
+    want, _ = oracle_jobs(oracle, jobs, W, 8, [st0["dense_end"], st0["tierb_end"], len(s.filter_primes)])
+    assert read_out(ctx, d_out, d_cnt) == want and len(want) > 20
     uj = s.tuning["unit_j"]
-    s.set_tuning(unit_j=uj // 2 if uj > 2048 else uj * 2)
-    d_out, d_cnt = ctx.dalloc(1 << 16), ctx.dalloc(64)
-    with pytest.raises(ccs.CcsError):
-        js.launch(d_out, 1 << 12, d_cnt)
+    s.set_tuning(unit_j=uj // 2 if uj > 65536 else uj * 2)
+    js.launch(d_out, cap, d_cnt)
+    ctx.synchronize()
+    assert read_out(ctx, d_out, d_cnt) == want
+    js.launch_each(d_out, cap, d_cnt)
+    ctx.synchronize()
+    assert read_out(ctx, d_out, d_cnt) == want
+    js.update(jobs[:5])
+    js.launch(d_out, cap, d_cnt)
+    ctx.synchronize()
+    assert read_out(ctx, d_out, d_cnt) == [x for x in want if x[0] < 5]
     js.close()
+    ctx.dfree(d_out)
+    ctx.dfree(d_cnt)
