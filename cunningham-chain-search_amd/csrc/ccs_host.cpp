@@ -435,7 +435,8 @@ static u32 tierb_in_lds(const ccs_shape* s, const ccs_tuning* t) {
     u32 words = 0;
     for (auto sl : dense_slots(s, t)) words += dense_ext_words(slot_period(s, sl), t->rows_per_group);
     const u32 bpc = t->blocks_per_cu ? t->blocks_per_cu : 1;
-    const u32 budget = std::max(s->max_lds_words / bpc, std::min(s->max_lds_words, 16384u));
+    const u32 cap = s->max_lds_words - kLdsCtlWords - 1;   // the scheduler words follow the tables
+    const u32 budget = std::min(cap, std::max(s->max_lds_words / bpc, std::min(s->max_lds_words, 16384u)) - kLdsCtlWords - 1);
     u32 n = 0;
     while (n < t->n_tierb) {
         const u32 end = std::min(n + (u32)kTierBGroup, t->n_tierb);
@@ -445,7 +446,7 @@ static u32 tierb_in_lds(const ccs_shape* s, const ccs_tuning* t) {
         words += w;
         n = end;
     }
-    if (words > s->max_lds_words || (t->n_tierb && n == 0)) return ~0u;
+    if (words > cap || (t->n_tierb && n == 0)) return ~0u;
     return n;
 }
 
@@ -825,6 +826,7 @@ static void fill_sieve_common(const ccs_shape* s, const Scratch& x, SieveArgs& a
     a.W = s->W;
     a.lds_image = s->d_lds_image;
     a.lds_words = s->lds_words;
+    a.lds_ctl = (s->lds_words + 1) & ~1u;
     a.nd = s->nd;
     a.nb = s->nb;
     a.nb_lds = s->nb_lds;
@@ -854,7 +856,7 @@ static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, 
     const int nbv = s->nb ? (int)((s->nb + 63) / 64) : 1;
     const u32 wpb = s->tune.waves_per_block;
     const int block = 64 * (int)wpb;
-    const size_t lds = (size_t)s->lds_words * 4;
+    const size_t lds = (size_t)(((s->lds_words + 1) & ~1u) + kLdsCtlWords) * 4;
     u64 want_grid = (u64)c->n_cu * s->tune.blocks_per_cu;
     const u64 blocks_needed = (n_units + wpb - 1) / wpb;
     if (want_grid > blocks_needed) want_grid = blocks_needed ? blocks_needed : 1;

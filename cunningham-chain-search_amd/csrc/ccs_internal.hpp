@@ -20,6 +20,7 @@ constexpr int kMaxDense = 64;       // dense primes held one per lane
 constexpr int kMaxHashSlots = 64;   // forbidden-residue hash per deep prime
 constexpr int kTierBGroup = 64;     // stage-B primes tested per wave step (one per lane)
 constexpr int kMaxTierB = 512;      // stage-B primes: at most 8 groups of 64
+constexpr unsigned kLdsCtlWords = 8;   // sieve scheduler words after the LDS tables
 
 // Per-modulus constants for exact residues of u64/u128 values: y = sum limb_k*2^(32k)
 // folded with c_k = 2^(32k) mod q (< 2^52 for q < 2^18), then reduced with a double
@@ -103,6 +104,7 @@ struct SieveArgs {
     // tables
     const u32* lds_image;     // dense patterns + tier-B bitsets, copied to LDS at start
     u32 lds_words;
+    u32 lds_ctl;              // word offset of the scheduler words (kLdsCtlWords) in LDS
     u32 nd;                   // dense slots (one or two primes each)
     u32 nb;                   // tier-B primes
     u32 nb_lds;               // tier-B primes with bitsets in LDS (whole groups of 64)

@@ -200,6 +200,8 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
     extern __shared__ u32 lds[];
     const u32 tid = threadIdx.x;
     for (u32 i = tid; i < A.lds_words; i += blockDim.x) lds[i] = A.lds_image[i];
+    u32* const ctl = lds + A.lds_ctl;   // scheduler words past the tables (8-byte aligned)
+    if (tid == 0) ctl[0] = 0;
     __syncthreads();
 
     const u32 lane = lane_id();
@@ -246,32 +248,62 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
         q_n = 0;
     };
 
-    // Work stealing by guided self-scheduling: a claim takes max(1, remaining / sched_div)
-    // consecutive units (capped at 64) with one atomic, so the shared cursor sees a few
-    // claims per wave rather than one per unit.  The next claim is in flight while the
-    // current one is sieved.
-    auto claim_size = [&](u64 claimed_to) -> u32 {
-        const u64 rem = claimed_to < A.n_units ? A.n_units - claimed_to : 0;
-        u64 b = rem / A.sched_div;
-        return (u32)(b < 1 ? 1 : (b > 64 ? 64 : b));
-    };
-    // the first unit of wave w is unit w, with no atomic; claims number the units after those
-    const u64 n_waves = (u64)gridDim.x * (blockDim.x >> 6);
-    const u64 wave_id = (u64)blockIdx.x * (blockDim.x >> 6) + (tid >> 6);
-    const u64 n_static = n_waves < A.n_units ? n_waves : A.n_units;
-    u32 next_b = claim_size(n_static);
-    u64 next = 0;
-    if (lane == 0) next = n_static + atomicAdd(A.unit_counter, (u64)next_b);
-    u64 cur = wave_id, cur_end = wave_id < n_static ? wave_id + 1 : wave_id;
+    // Work distribution with few global atomics, in two phases.
+    //  A: the first n_a units are split into equal contiguous ranges, one per block; the
+    //     block's waves take runs of consecutive units from it through an LDS cursor
+    //     (a run is 1/(8 * waves per block) of the range's rest, 1..64 units).
+    //  B: the last n_pool units (a sixteenth, at least one per wave of the grid) form a
+    //     pool shared by all blocks, drawn in block-wide rounds: thread 0 claims one unit
+    //     per wave for the whole block with one global atomic (the next round's claim is in
+    //     flight while this round is sieved).  Every wave passes the two barriers of every
+    //     round and all leave in the same round (the one whose claim starts past the
+    //     pool), so the barriers always match.
+    const u32 wpb = blockDim.x >> 6, wib = tid >> 6;
+    const u64 n_blocks = gridDim.x;
+    u64 n_pool = A.n_units / 16;
+    if (n_pool < n_blocks * wpb) n_pool = n_blocks * wpb;
+    if (n_pool > A.n_units) n_pool = A.n_units;
+    const u64 n_a = A.n_units - n_pool;
+    const u64 blk_lo = n_a * blockIdx.x / n_blocks, blk_hi = n_a * (blockIdx.x + 1) / n_blocks;
+    const u32 blk_n = (u32)(blk_hi - blk_lo);
+    u32 seen = 0;                       // phase-A cursor value this wave last drew
+    bool in_pool = false;
+    u64 next_g = 0;   // thread 0: the claim for the next round (one unit per wave)
+    if (tid == 0) next_g = atomicAdd(A.unit_counter, (u64)wpb);
+    u64 cur = 0, cur_end = 0;
     u32 sd = 0;                                             // dense phase (lane < nd)
     u32 prev_owner = 0xffffffffu, prev_j0 = 0, prev_n = 0;  // the wave's previous unit
     for (;;) {
-        if (cur == cur_end) {
-            cur = bcast64(next);
-            if (cur >= A.n_units) break;
-            cur_end = cur + next_b < A.n_units ? cur + next_b : A.n_units;
-            next_b = claim_size(cur_end);
-            if (lane == 0) next = n_static + atomicAdd(A.unit_counter, (u64)next_b);
+        if (cur == cur_end && !in_pool) {
+            const u32 rem = blk_n > seen ? blk_n - seen : 0;
+            u32 c = rem / (8 * wpb);
+            c = c < 1 ? 1 : (c > 64 ? 64 : c);
+            u32 got = 0;
+            if (lane == 0) got = atomicAdd(&ctl[0], c);
+            got = bcast32(got);
+            if (got < blk_n) {
+                cur = blk_lo + got;
+                cur_end = got + c < blk_n ? blk_lo + got + c : blk_hi;
+                seen = got + c;
+            } else {
+                in_pool = true;
+            }
+        }
+        if (cur == cur_end) {   // in the pool: next round(s) until this wave has a share
+            bool done = false;
+            while (cur == cur_end) {
+                __syncthreads();                       // the previous round is finished
+                if (tid == 0) *(u64*)(ctl + 2) = next_g;
+                __syncthreads();                       // the round's claim is visible
+                const u64 round_g = *(const u64*)(ctl + 2);
+                if (round_g >= n_pool) { done = true; break; }
+                if (tid == 0) next_g = atomicAdd(A.unit_counter, (u64)wpb);
+                if (round_g + wib < n_pool) {
+                    cur = n_a + round_g + wib;
+                    cur_end = cur + 1;
+                }
+            }
+            if (done) break;
         }
         const u64 unit = cur + A.unit_base;
         ++cur;
