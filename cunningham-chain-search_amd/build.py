@@ -8,7 +8,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libccs.so")
 CLI = os.path.join(HERE, "ccs_search")
-SOURCES = ["ccs_kernels.hip", "ccs_host.cpp", "ccs_server.cpp", "ccs_confirm.cpp", "ccs_campaign.cpp"]
+SOURCES = ["ccs_kernels.hip", "ccs_host.cpp", "ccs_server.cpp", "ccs_confirm.cpp", "ccs_campaign.cpp",
+           "ccs_family.cpp"]
 HEADERS = ["ccs_internal.hpp", "ccs_arith.hpp", os.path.join("..", "..", "include", "ccs.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -289,6 +289,78 @@ class Shape:
             int(self.bases()[int(s["base_idx"])])
 
 
+class FamilyShape(Shape):
+    """Fingerprint family of the v16 Q-iteration front end (ccs_family_shape_build):
+    p_0 = Q*D*2^E - 1, sieved at positions [0, depth) by the filter primes that do not
+    divide D.  Jobs are Q windows (0, Q0, count) for JobSet / batch / prove_jobs."""
+
+    def __init__(self, ctx, D, exp_start, target, depth, filter_primes, tuning=None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        fp_a, fp_p = _u32arr(filter_primes)
+        self._keep = [fp_a]
+        self.crt, self.wheel = [], []
+        self.filter_primes = [int(x) for x in fp_a if D % int(x)]
+        self.target, self.depth, self.pos0_max_q, self.stage_end = target, depth, None, []
+        self.D, self.exp_start = D, exp_start
+        p = L.FamilyParams(L.U128.of(D), exp_start, target, depth, len(fp_a), fp_p)
+        self.h = C.c_void_p()
+        tp = None
+        if tuning is not None:
+            tp = C.byref(tuning if isinstance(tuning, L.Tuning) else L.Tuning(**tuning))
+        L.check(self.lib.ccs_family_shape_build(ctx.h, C.byref(p), tp, C.byref(self.h)), "ccs_family_shape_build")
+        self._refresh_info()
+        self.W = D << exp_start
+
+    def window(self, bits_lo, bits_hi):
+        return family_window(self.D, self.exp_start, bits_lo, bits_hi)
+
+    def n_of_q(self, q):
+        return q * self.W - 1
+
+
+def family_window(D, exp_start, bits_lo, bits_hi):
+    """(q_min, q_max) of candidates with bits_lo..bits_hi bits, or None when empty."""
+    lo, hi = L.U128(), L.U128()
+    rc = L.check(L.load().ccs_family_window(L.U128.of(D), exp_start, bits_lo, bits_hi, C.byref(lo), C.byref(hi)),
+                 "ccs_family_window")
+    if rc == L.CCS_DONE:
+        return None
+    return lo.lo | (lo.hi << 64), hi.lo | (hi.hi << 64)
+
+
+def _pool(fn, arg, keep, cap):
+    k = np.ascontiguousarray(np.asarray(list(keep), np.uint32))
+    while True:
+        out = (L.PoolEntry * max(cap, 1))()
+        n = C.c_uint32(0)
+        rc = L.check(fn(arg, k.ctypes.data if len(k) else None, len(k), out, cap, C.byref(n)), "pool")
+        if rc == L.CCS_OVERFLOW:
+            cap = n.value
+            continue
+        return [out[i].as_dict() for i in range(n.value)]
+
+
+def pool_parse(text, keep=(), cap=1024):
+    """Pool list "Z#/X,P*Q^k,auto:Z#:K" -> [{name, D, primes}] (ccs_pool_parse)."""
+    return _pool(L.load().ccs_pool_parse, text.encode(), keep, cap)
+
+
+def pool_load(path, keep=(), cap=4096):
+    """Pool file (one list per line, '#' comments) -> [{name, D, primes}] (ccs_pool_load)."""
+    return _pool(L.load().ccs_pool_load, str(path).encode(), keep, cap)
+
+
+def dedup(values):
+    """First-occurrence flags of integer values < 2^128 (ccs_dedup_u128)."""
+    arr = np.zeros((len(values), 2), np.uint64)
+    for i, v in enumerate(values):
+        arr[i] = (v & ((1 << 64) - 1), v >> 64)
+    first = np.zeros(len(values), np.uint8)
+    L.check(L.load().ccs_dedup_u128(arr.ctypes.data, len(values), first.ctypes.data), "ccs_dedup_u128")
+    return first.astype(bool)
+
+
 class JobSet:
     """Class-form jobs uploaded once (ccs_jobset): batched launches, the per-job loop, graphs."""
 
@@ -342,6 +414,11 @@ class JobSet:
 
     def graph_launch(self, g, stream=None):
         L.check(self.lib.ccs_graph_launch(self.ctx.h, g, stream), "graph_launch")
+
+    def top_gate(self, d_in, d_in_count, in_cap, top_pos, reps, d_out, out_cap, d_out_count, stream=None):
+        """K1.5 top-position MR gate of a family set (ccs_family_top_gate_launch)."""
+        L.check(self.lib.ccs_family_top_gate_launch(self.ctx.h, self.h, d_in, d_in_count, in_cap, top_pos, reps,
+                                                    d_out, out_cap, d_out_count, stream), "top_gate")
 
     def prove_launch(self, d_surv, d_count, cap, d_out, bits=0, test="bpsw", bases=(2,), stream=None):
         pp = _prove_params(bits, test, bases)

@@ -36,7 +36,8 @@ EXPORTS = [
     "ccs_confirm_values", "ccs_confirm_host", "ccs_confirm_stats_add",
     "ccs_event_synchronize", "ccs_parse_prefix", "ccs_shape_decompose", "ccs_campaign_create", "ccs_campaign_create_range",
     "ccs_campaign_free", "ccs_campaign_next", "ccs_campaign_record", "ccs_campaign_get_state",
-    "ccs_campaign_save", "ccs_campaign_resume",
+    "ccs_campaign_save", "ccs_campaign_resume", "ccs_family_shape_build", "ccs_family_window",
+    "ccs_pool_parse", "ccs_pool_load", "ccs_dedup_u128", "ccs_family_top_gate_launch",
 ]
 
 
@@ -65,7 +66,22 @@ class ShapeInfo(C.Structure):
         ("n_bases", C.c_uint32), ("n_filter", C.c_uint32), ("n_classes", C.c_uint64),
         ("n_dense", C.c_uint32), ("n_tierb", C.c_uint32), ("n_deep", C.c_uint32),
         ("rows_per_group", C.c_uint32), ("unit_j", C.c_uint32),
+        ("W_hi", C.c_uint64), ("family", C.c_int32), ("exp_start", C.c_int32), ("D", U128),
     ]
+
+
+class FamilyParams(C.Structure):
+    _fields_ = [("D", U128), ("exp_start", C.c_int32), ("target", C.c_int32), ("depth", C.c_int32),
+                ("n_filter", C.c_uint32), ("filter_primes", C.POINTER(C.c_uint32))]
+
+
+class PoolEntry(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("D", U128), ("primes", C.c_uint32 * 64), ("n_primes", C.c_uint32),
+                ("_pad", C.c_uint32)]
+
+    def as_dict(self):
+        return {"name": self.name.decode(), "D": self.D.lo | (self.D.hi << 64),
+                "primes": [int(self.primes[i]) for i in range(self.n_primes)]}
 
 
 class Tuning(C.Structure):
@@ -284,6 +300,13 @@ def load():
     L.ccs_server_drain.argtypes = [vp]
     L.ccs_server_stats.argtypes = [vp, C.POINTER(ServerCounters)]
     L.ccs_server_shutdown.argtypes = [vp]
+    L.ccs_family_shape_build.argtypes = [vp, C.POINTER(FamilyParams), C.POINTER(Tuning), C.POINTER(vp)]
+    L.ccs_family_window.argtypes = [U128, C.c_int32, C.c_int32, C.c_int32, C.POINTER(U128), C.POINTER(U128)]
+    L.ccs_pool_parse.argtypes = [C.c_char_p, vp, C.c_uint32, vp, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.ccs_pool_load.argtypes = [C.c_char_p, vp, C.c_uint32, vp, C.c_uint32, C.POINTER(C.c_uint32)]
+    L.ccs_dedup_u128.argtypes = [vp, C.c_uint32, vp]
+    L.ccs_family_top_gate_launch.argtypes = [vp, vp, vp, vp, C.c_uint32, C.c_int32, C.c_int32, vp,
+                                             C.c_uint32, vp, vp]
     L.ccs_profile_start.argtypes = [vp]
     L.ccs_profile_stop.argtypes = [vp, C.POINTER(KernelTimes)]
     _lib = L

@@ -143,10 +143,10 @@ static u64 mod_inv(u64 a, u64 m) {
     return (u64)(t < 0 ? t + (int64_t)m : t);
 }
 
-static PrimeMod make_primemod(u32 q, u64 W) {
+static PrimeMod make_primemod(u32 q, u128 W) {
     PrimeMod pm;
     pm.q = q;
-    pm.winv = W ? (u32)mod_inv(W % q, q) : 0;
+    pm.winv = W ? (u32)mod_inv((u64)(W % q), q) : 0;
     pm.c32 = (u32)(((u128)1 << 32) % q);
     pm.c64 = (u32)(((u128)1 << 64) % q);
     pm.c96 = (u32)(((u128)1 << 96) % q);
@@ -344,6 +344,10 @@ struct ccs_shape {
     u32 pos0_max_q = 97;
     u32 flags = 0;
     u64 M = 1, WP = 1, W = 1;
+    u64 W_hi = 0;                            // the modulus is u128 for family shapes
+    bool family = false;                     // n = Q*D*2^E - 1 (ccs_family_shape_build)
+    u128 D = 0;
+    int exp_start = 0;
     std::vector<u64> bases;
     std::vector<ClassRec> classes;
     u64 per_rho[6] = {0, 0, 0, 0, 0, 0};
@@ -372,6 +376,8 @@ static bool is_prime_u32(u32 q) {
 }
 
 // F_q: residues (2^-i - 1) mod q for i in [lo, min(hi, q-1)) (v13.cu:771-787)
+static u128 wfull(const ccs_shape* s) { return ((u128)s->W_hi << 64) | s->W; }
+
 static std::vector<u32> forbidden(u32 q, int lo, int hi) {
     std::vector<u32> out;
     if (hi > (int)q - 1) hi = (int)q - 1;
@@ -515,7 +521,7 @@ static int upload_tables(ccs_shape* s) {
     // u-space kill sets: u + (a * W^-1) = f * W^-1 (mod q)  for f in F_q
     auto uset = [&](u32 i) {
         const u32 q = s->fp[i];
-        const u64 winv = mod_pow(s->W % q, q - 2, q);
+        const u64 winv = mod_pow((u64)(wfull(s) % q), q - 2, q);
         std::vector<u8> kill(q, 0);
         for (u32 f : s->forb[i]) kill[(u64)f * winv % q] = 1;
         return kill;
@@ -533,7 +539,7 @@ static int upload_tables(ccs_shape* s) {
         d.step = (u32)(((u64)kRowBits * rg) % Q);
         d.pad = 0;
         dense[k] = d;
-        pmod[k] = make_primemod(Q, s->W);
+        pmod[k] = make_primemod(Q, wfull(s));
         for (u32 w = 0; w < words; ++w) {
             u32 v = 0;
             for (u32 b = 0; b < 32; ++b) {
@@ -564,7 +570,7 @@ static int upload_tables(ccs_shape* s) {
             }
             dst.push_back(v);
         }
-        pmod[slots.size() + i] = make_primemod(q, s->W);
+        pmod[slots.size() + i] = make_primemod(q, wfull(s));
     }
     if (far_bits.empty()) far_bits.push_back(0);
     std::vector<DeepPrime> deep(nc ? nc : 1);
@@ -572,7 +578,7 @@ static int upload_tables(ccs_shape* s) {
     for (u32 i = 0; i < nc; ++i) {
         const u32 idx = nd + nb + i;
         DeepPrime d;
-        d.pm = make_primemod(s->fp[idx], s->W);
+        d.pm = make_primemod(s->fp[idx], wfull(s));
         d.hash_off = i * kMaxHashSlots;
         d.pad[0] = d.pad[1] = d.pad[2] = 0;
         deep[i] = d;
@@ -721,6 +727,58 @@ extern "C" int ccs_shape_build(ccs_ctx* c, const ccs_shape_params* p, const ccs_
     return CCS_OK;
 }
 
+// Family shape: candidates n = Q*D*2^E - 1 as the class W - 1 mod W = D*2^E.  Forbidden
+// sets at every position [0, depth) of every kept prime, as v16 builds its per-V masks
+// (v16...cu:5104-5227: Q = inv(D*2^(E+j)) mod q, which is n = 2^-j - 1 mod q in n-space).
+extern "C" int ccs_family_shape_build(ccs_ctx* c, const ccs_family_params* p, const ccs_tuning* tuning,
+                                      ccs_shape** out) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!p || !out || (p->n_filter && !p->filter_primes)) return fail(CCS_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (p->depth < 1 || p->depth > 32) return fail(CCS_ERR_INVALID, "depth must be in 1..32");
+    if (p->target < 1 || p->target > 64) return fail(CCS_ERR_INVALID, "target must be in 1..64");
+    if (p->exp_start < 0 || p->exp_start > 126) return fail(CCS_ERR_INVALID, "exp_start must be in 0..126");
+    const u128 D = ((u128)p->D.hi << 64) | p->D.lo;
+    if (D == 0) return fail(CCS_ERR_INVALID, "D must be >= 1");
+    if (D >> (127 - p->exp_start)) return fail(CCS_ERR_INVALID, "D*2^E must be below 2^127");
+    const u128 W = D << p->exp_start;
+    ccs_shape* s = new ccs_shape();
+    s->device = c->device;
+    s->max_lds_words = c->max_lds_words;
+    s->target = p->target;
+    s->depth = p->depth;
+    s->pos0_max_q = 0xffffffffu;
+    s->flags = CCS_SHAPE_NO_CLASSES;
+    s->family = true;
+    s->D = D;
+    s->exp_start = p->exp_start;
+    s->W = (u64)W;
+    s->W_hi = (u64)(W >> 64);
+    u32 last = 2;
+    for (u32 i = 0; i < p->n_filter; ++i) {
+        const u32 q = p->filter_primes[i];
+        if (!is_prime_u32(q) || q <= last || q >= (1u << 18)) {
+            delete s;
+            return fail(CCS_ERR_INVALID, "filter primes must be ascending odd primes < 2^18");
+        }
+        last = q;
+        if (D % q == 0) continue;   // q | D: p_j = -1 (mod q) for every j, nothing to sieve
+        s->fp.push_back(q);
+        s->forb.push_back(forbidden(q, 0, p->depth));
+    }
+    ccs_tuning t;
+    if (tuning) t = *tuning;
+    else default_tuning(s, &t);
+    rc = validate_tuning(s, &t);
+    if (rc) { delete s; return rc; }
+    s->tune = t;
+    rc = upload_tables(s);
+    if (rc) { free_device_tables(s); delete s; return rc; }
+    *out = s;
+    return CCS_OK;
+}
+
 extern "C" void ccs_shape_free(ccs_shape* s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
@@ -748,6 +806,11 @@ extern "C" int ccs_shape_query(const ccs_shape* s, ccs_shape_info* o) {
     o->M = s->M;
     o->WP = s->WP;
     o->W = s->W;
+    o->W_hi = s->W_hi;
+    o->family = s->family ? 1 : 0;
+    o->exp_start = s->exp_start;
+    o->D.lo = (u64)s->D;
+    o->D.hi = (u64)(s->D >> 64);
     o->n_bases = (u32)s->bases.size();
     o->n_filter = (u32)s->fp.size();
     o->n_classes = s->classes.size();
@@ -824,6 +887,7 @@ static Scratch ctx_scratch(ccs_ctx* c) { return Scratch{c->mid, c->mid_cap, c->c
 static void fill_sieve_common(const ccs_shape* s, const Scratch& x, SieveArgs& a) {
     memset(&a, 0, sizeof(a));
     a.W = s->W;
+    a.W_hi = s->W_hi;
     a.lds_image = s->d_lds_image;
     a.lds_words = s->lds_words;
     a.lds_ctl = (s->lds_words + 1) & ~1u;
@@ -908,6 +972,7 @@ static int filter_launch_impl(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint3
     DeepArgs da;
     memset(&da, 0, sizeof(da));
     da.W = s->W;
+    da.W_hi = s->W_hi;
     da.deep = s->d_deep;
     da.hash = s->d_hash;
     da.nc = s->nc;
@@ -1006,6 +1071,7 @@ static int fill_prove(const ccs_shape* s, const ccs_prove_params* p, ProveArgs& 
     a.WP = s->WP;
     a.M = s->M;
     a.W = s->W;
+    a.W_hi = s->W_hi;
     a.bases = s->d_bases;
     a.bits = p ? p->bits : 0;
     a.test_kind = p ? p->test_kind : CCS_TEST_BPSW;
@@ -1027,6 +1093,7 @@ extern "C" int ccs_prove_launch(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, con
     int rc = ctx_check(c);
     if (rc) return rc;
     if (!s || !d_surv || !d_count || !d_out) return fail(CCS_ERR_INVALID, "null argument");
+    if (s->bases.empty()) return fail(CCS_ERR_INVALID, "tile survivors need a CRT shape (this one has no bases)");
     ProveArgs a;
     rc = fill_prove(s, p, a);
     if (rc) return rc;
@@ -1151,6 +1218,7 @@ struct ccs_jobset {
     u64* staging_prefix = nullptr;
     hipEvent_t staged = nullptr;    // the last staging upload has been consumed
     std::vector<u64> counts;        // candidates of each job (host copy)
+    std::vector<JobRec> hjobs;      // the device job records (host copy)
     u64* d_each = nullptr;          // per-job {0, units} pairs of the one-job-per-launch loop
     std::vector<u32> each_unit;     // unit size of each job launched on its own
     bool each_valid = false;
@@ -1173,6 +1241,36 @@ extern "C" void ccs_jobset_free(ccs_jobset* js) {
     if (js->staging_prefix) (void)hipHostFree(js->staging_prefix);
     if (js->staged) (void)hipEventDestroy(js->staged);
     delete js;
+}
+
+// The device record of a job.  CRT shapes: n = a + W*u with the job's class a < W.
+// Family shapes: every job is the class W - 1, n = W*Q - 1 = (W - 1) + W*(Q - 1), so `a`
+// must be 0, the device u0 is Q0 - 1 (survivor offsets stay Q - Q0), Q starts at 1 or
+// above and the last candidate stays below 2^127.
+static int job_record(const ccs_shape* s, const ccs_job& j, JobRec& r) {
+    if (j.count > (1ull << 32)) return fail(CCS_ERR_INVALID, "job count must be <= 2^32");
+    const u128 u0 = ((u128)j.u0.hi << 64) | j.u0.lo;
+    if (s->family) {
+        if (j.a != 0) return fail(CCS_ERR_INVALID, "family jobs have no residue: a must be 0 (the class is W - 1)");
+        const u128 W = wfull(s), a = W - 1;
+        const u128 lim = ((u128)1 << 127) / W;   // W*Q - 1 < 2^127 for Q <= lim
+        if (j.count && (u0 == 0 || u0 > lim || (u128)(j.count - 1) > lim - u0))
+            return fail(CCS_ERR_INVALID, "family job: Q must be >= 1 and W*Q - 1 < 2^127");
+        // n = (W - 1) + W*u = W*(u + 1) - 1: the device u runs from Q0 - 1
+        const u128 d0 = u0 ? u0 - 1 : 0;
+        r.a = (u64)a;
+        r.a_hi = (u64)(a >> 64);
+        r.u0_lo = (u64)d0;
+        r.u0_hi = (u64)(d0 >> 64);
+    } else {
+        if (j.a >= s->W) return fail(CCS_ERR_INVALID, "job residue a must be < W");
+        r.a = j.a;
+        r.a_hi = 0;
+        r.u0_lo = j.u0.lo;
+        r.u0_hi = j.u0.hi;
+    }
+    r.count = j.count;
+    return CCS_OK;
 }
 
 // Work-unit size of a job set: the shape's unit_j, halved (down to kMinSetUnit) while the
@@ -1214,15 +1312,11 @@ extern "C" int ccs_jobset_create(ccs_ctx* c, const ccs_shape* s, const ccs_job* 
     js->n_jobs = n_jobs;
     js->prefix.resize((size_t)n_jobs + 1);
     for (u32 i = 0; i < n_jobs; ++i) {
-        if (jobs[i].a >= s->W) { delete js; return fail(CCS_ERR_INVALID, "job residue a must be < W"); }
-        if (jobs[i].count > (1ull << 32)) { delete js; return fail(CCS_ERR_INVALID, "job count must be <= 2^32"); }
+        if ((rc = job_record(s, jobs[i], jr[i]))) { delete js; return rc; }
     }
+    js->hjobs = jr;
     js->unit_j = set_unit(c, s, jobs, n_jobs);
     for (u32 i = 0; i < n_jobs; ++i) {
-        jr[i].a = jobs[i].a;
-        jr[i].u0_lo = jobs[i].u0.lo;
-        jr[i].u0_hi = jobs[i].u0.hi;
-        jr[i].count = jobs[i].count;
         js->prefix[i] = js->n_units;
         js->n_units += (jobs[i].count + js->unit_j - 1) / js->unit_j;
         js->candidates += jobs[i].count;
@@ -1266,8 +1360,8 @@ extern "C" int ccs_jobset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jo
     }
     u64 units = 0, cand = 0;
     for (u32 i = 0; i < n_jobs; ++i) {
-        if (h_jobs[i].a >= s->W) return fail(CCS_ERR_INVALID, "job residue a must be < W");
-        if (h_jobs[i].count > (1ull << 32)) return fail(CCS_ERR_INVALID, "job count must be <= 2^32");
+        JobRec tmp;
+        if ((rc = job_record(s, h_jobs[i], tmp))) return rc;
         cand += h_jobs[i].count;
     }
     if ((u64)(cand * s->est_survival_b * 2.0) + (1u << 16) > js->x.mid_cap)
@@ -1275,14 +1369,12 @@ extern "C" int ccs_jobset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jo
     js->prefix.resize((size_t)n_jobs + 1);
     js->unit_j = set_unit(c, s, h_jobs, n_jobs);
     js->counts.resize(n_jobs);
+    js->hjobs.resize(n_jobs);
     js->each_valid = false;
     for (u32 i = 0; i < n_jobs; ++i) {
         js->counts[i] = h_jobs[i].count;
-        JobRec& r = js->staging[i];
-        r.a = h_jobs[i].a;
-        r.u0_lo = h_jobs[i].u0.lo;
-        r.u0_hi = h_jobs[i].u0.hi;
-        r.count = h_jobs[i].count;
+        (void)job_record(s, h_jobs[i], js->staging[i]);
+        js->hjobs[i] = js->staging[i];
         js->prefix[i] = js->staging_prefix[i] = units;
         units += (h_jobs[i].count + js->unit_j - 1) / js->unit_j;
     }
@@ -1307,6 +1399,7 @@ static void jobset_args(const ccs_jobset* js, SieveArgs& sa, DeepArgs& da, void*
     sa.jobsrc.n_jobs = js->n_jobs;
     memset(&da, 0, sizeof(da));
     da.W = s->W;
+    da.W_hi = s->W_hi;
     da.deep = s->d_deep;
     da.hash = s->d_hash;
     da.nc = s->nc;
@@ -1543,6 +1636,43 @@ extern "C" int ccs_prove_jobset_launch(ccs_ctx* c, ccs_jobset* js, const ccs_job
     hipError_t e = launch_prove(a, cap, st);
     if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("prove launch: ") + hipGetErrorString(e));
     prof_end(c, st);
+    return CCS_OK;
+}
+
+extern "C" int ccs_family_top_gate_launch(ccs_ctx* c, ccs_jobset* js, const ccs_job_survivor* d_in,
+                                          const uint32_t* d_in_count, uint32_t in_cap, int32_t top_pos,
+                                          int32_t reps, ccs_job_survivor* d_out, uint32_t out_cap,
+                                          uint32_t* d_out_count, void* stream) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!js || !d_in || !d_in_count || !d_out || !d_out_count) return fail(CCS_ERR_INVALID, "null argument");
+    const ccs_shape* s = js->shape;
+    if (!s->family) return fail(CCS_ERR_INVALID, "the top gate needs a family shape");
+    if (top_pos < 0 || top_pos > 63) return fail(CCS_ERR_INVALID, "top_pos must be in 0..63");
+    if (reps != 2 && reps != 3) return fail(CCS_ERR_INVALID, "reps must be 2 ({2,3}) or 3 ({2,3,5})");
+    // every job's p_top = (W*Q) << top_pos - 1 must be below 2^127 (Montgomery range)
+    const u128 W = wfull(s);
+    const u128 lim = (((u128)1 << 127) >> top_pos) / W;
+    for (u32 i = 0; i < js->n_jobs; ++i) {
+        const JobRec& r = js->hjobs[i];
+        if (!r.count) continue;
+        const u128 qlast = (((u128)r.u0_hi << 64) | r.u0_lo) + r.count;   // device u0 = Q0 - 1
+        if (qlast > lim) return fail(CCS_ERR_INVALID, "p_top of a job reaches 2^127");
+    }
+    TopGateArgs a;
+    a.jobs = js->d_jobs;
+    a.W = s->W;
+    a.W_hi = s->W_hi;
+    a.in = d_in;
+    a.in_count = d_in_count;
+    a.in_cap = in_cap;
+    a.top_pos = top_pos;
+    a.reps = reps;
+    a.out = d_out;
+    a.out_cap = out_cap;
+    a.out_count = d_out_count;
+    hipError_t e = launch_top_gate(a, stream ? (hipStream_t)stream : c->stream);
+    if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("top gate launch: ") + hipGetErrorString(e));
     return CCS_OK;
 }
 

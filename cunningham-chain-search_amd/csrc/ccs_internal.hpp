@@ -71,8 +71,8 @@ struct ClassRec {   // plain lattice class (base_idx, k, rho) and its residue a 
     u32 pad;
 };
 
-struct JobRec {     // class-form job: n = a + W*(u0 + j), j in [0, count)
-    u64 a;
+struct JobRec {     // class-form job: n = a + W*(u0 + j), j in [0, count); a and W are u128
+    u64 a, a_hi;
     u64 u0_lo, u0_hi;
     u64 count;
 };
@@ -100,7 +100,7 @@ struct JobSrc {
 };
 
 struct SieveArgs {
-    u64 W;
+    u64 W, W_hi;              // the shape modulus (u128)
     // tables
     const u32* lds_image;     // dense patterns + tier-B bitsets, copied to LDS at start
     u32 lds_words;
@@ -131,7 +131,7 @@ struct SieveArgs {
 };
 
 struct DeepArgs {
-    u64 W;
+    u64 W, W_hi;              // the shape modulus (u128)
     const DeepPrime* deep;
     const u32* hash;
     u32 nc;
@@ -154,8 +154,8 @@ struct ProveArgs {
     u64 T0_lo, T0_hi;
     u64 WP, M;
     const u64* bases;
-    // jobs reconstruction n = a + W*(u0 + u_off)
-    u64 W;
+    // jobs reconstruction n = a + W*(u0 + u_off), W u128
+    u64 W, W_hi;
     const JobRec* jobs;
     const void* surv;
     const u32* surv_count;
@@ -169,6 +169,20 @@ struct ProveArgs {
     u32* chain_count;          // queue length (scratch, cleared by the launcher)
 };
 
+// K1.5 family top-position gate
+struct TopGateArgs {
+    const JobRec* jobs;
+    u64 W, W_hi;
+    const void* in;            // ccs_job_survivor[in_cap]
+    const u32* in_count;
+    u32 in_cap;
+    int top_pos;               // 0..63
+    int reps;                  // bases {2, 3, 5}[0..reps)
+    void* out;                 // ccs_job_survivor[out_cap]
+    u32 out_cap;
+    u32* out_count;
+};
+
 // sets the text returned by ccs_last_error() and returns code (ccs_host.cpp)
 int set_error(int code, const char* msg);
 
@@ -178,5 +192,6 @@ hipError_t launch_sieve(const SieveArgs& a, int rg, int nbv, int grid, int block
 hipError_t launch_deep(const DeepArgs& a, u32 max_entries, hipStream_t s);
 hipError_t launch_prove(const ProveArgs& a, u32 max_entries, hipStream_t s);
 hipError_t launch_zero(u32* p, size_t n_words, hipStream_t s);
+hipError_t launch_top_gate(const TopGateArgs& a, hipStream_t s);
 
 }  // namespace ccs

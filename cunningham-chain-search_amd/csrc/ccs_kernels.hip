@@ -51,8 +51,8 @@ __device__ __forceinline__ void wave_sync() {
 
 // phase of u_start for a prime: s = (u_start + a * W^-1) mod q, so that a candidate
 // j units after u_start is killed iff (s + j) mod q lies in the prime's pattern.
-__device__ __forceinline__ u32 phase(const PrimeMod& pm, u64 a, u64 us_lo, u64 us_hi) {
-    u32 am = mod128(a, 0, pm);
+__device__ __forceinline__ u32 phase(const PrimeMod& pm, u64 a, u64 a_hi, u64 us_lo, u64 us_hi) {
+    u32 am = mod128(a, a_hi, pm);
     u32 um = mod128(us_lo, us_hi, pm);
     u32 t = mod52((u64)am * pm.winv, pm.q, pm.rcp);
     u32 s = um + t;
@@ -65,7 +65,7 @@ __device__ __forceinline__ u32 phase(const PrimeMod& pm, u64 a, u64 us_lo, u64 u
 // ---------------------------------------------------------------------------
 
 struct Unit {
-    u64 a;
+    u64 a, a_hi;
     u64 us_lo, us_hi;
     u32 n;       // candidates in the unit
     u32 owner;   // class or job index
@@ -109,6 +109,7 @@ __device__ __forceinline__ bool resolve(const SieveArgs& A, u64 unit, u32 lane, 
         if (end > jhi) end = jhi;
         if (end <= beg) return false;
         U.a = cr.a;
+        U.a_hi = 0;
         U.us_lo = P.U0_lo;
         U.us_hi = P.U0_hi;
         add128(U.us_lo, U.us_hi, beg);
@@ -125,6 +126,7 @@ __device__ __forceinline__ bool resolve(const SieveArgs& A, u64 unit, u32 lane, 
     u64 n = jr.count - beg;
     if (n > A.unit_j) n = A.unit_j;
     U.a = jr.a;
+    U.a_hi = jr.a_hi;
     U.us_lo = jr.u0_lo;
     U.us_hi = jr.u0_hi;
     add128(U.us_lo, U.us_hi, beg);
@@ -310,6 +312,7 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
         Unit U;
         if (!resolve(A, unit, lane, U)) continue;
         U.a = bcast64(U.a);
+        U.a_hi = bcast64(U.a_hi);
         U.us_lo = bcast64(U.us_lo);
         U.us_hi = bcast64(U.us_hi);
         U.n = bcast32(U.n);
@@ -330,11 +333,11 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
             }
         } else {
             sd = 0;
-            if (lane < A.nd) sd = doff * 32u + phase(A.pmod[lane], U.a, U.us_lo, U.us_hi);
+            if (lane < A.nd) sd = doff * 32u + phase(A.pmod[lane], U.a, U.a_hi, U.us_lo, U.us_hi);
 #pragma unroll
             for (int i = 0; i < NBV; ++i) {
                 const u32 p = (u32)i * 64u + lane;
-                R.s[i] = p < A.nb ? phase(A.pmod[A.nd + p], U.a, U.us_lo, U.us_hi) : 0u;
+                R.s[i] = p < A.nb ? phase(A.pmod[A.nd + p], U.a, U.a_hi, U.us_lo, U.us_hi) : 0u;
             }
         }
         prev_owner = U.owner;
@@ -463,9 +466,11 @@ __device__ __forceinline__ bool test_prime(U2 n, const ProveArgs& A, bool& ok) {
 // K1c: deep stage
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ U2 cand_n(u64 W, u64 a, u64 u_lo, u64 u_hi) {
+// n = a + W*u (mod 2^128) for u128 a, W, u
+__device__ __forceinline__ U2 cand_n(u64 W, u64 W_hi, u64 a, u64 a_hi, u64 u_lo, u64 u_hi) {
     U2 n = mul_u64(mk(u_lo, u_hi), W);
-    return add(n, mk(a, 0));
+    n.hi += u_lo * W_hi;
+    return add(n, mk(a, a_hi));
 }
 
 // one wave per stage-B survivor; the 64 lanes test 64 deep primes at a time, so the rare
@@ -481,7 +486,7 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
     for (u32 e = wid; e < total; e += waves) {
         const MidRec rec = A.mid[e];
         const u32 owner = rec.owner, j = rec.j;
-        u64 ulo, uhi, a;
+        u64 ulo, uhi, a, a_hi = 0;
         if (A.mode == 0) {
             const ClassRec cr = A.plain.cls[owner];
             a = cr.a;
@@ -490,11 +495,12 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
         } else {
             const JobRec jr = A.jobsrc.jobs[owner];
             a = jr.a;
+            a_hi = jr.a_hi;
             ulo = jr.u0_lo;
             uhi = jr.u0_hi;
         }
         add128(ulo, uhi, j);
-        const U2 n = cand_n(A.W, a, ulo, uhi);
+        const U2 n = cand_n(A.W, A.W_hi, a, a_hi, ulo, uhi);
         bool dead = false;
         for (u32 p0 = 0; p0 < A.nc; p0 += 64) {
             const u32 p = p0 + lane;
@@ -549,7 +555,7 @@ __device__ __forceinline__ U2 survivor_n(const ProveArgs& A, u32 idx) {
     const JobRec jr = A.jobs[s.job_idx];
     u64 ulo = jr.u0_lo, uhi = jr.u0_hi;
     add128(ulo, uhi, s.u_off);
-    return cand_n(A.W, jr.a, ulo, uhi);
+    return cand_n(A.W, A.W_hi, jr.a, jr.a_hi, ulo, uhi);
 }
 
 // K2a: per survivor the gates of prover_worker (v13.cu:1828-1842): bit size, trial division
@@ -677,6 +683,32 @@ __global__ __launch_bounds__(256) void chain_kernel(ProveArgs A) {
     }
 }
 
+// K1.5: top-position gate of family survivors (v16_k15_topmr_kernel, v16...cu:2185-2296):
+// p_top = (W*Q) << top_pos - 1 with W = D*2^E must pass strong MR to the first `reps` bases
+// of {2, 3, 5}; the passing survivors are compacted into the output.
+__global__ __launch_bounds__(256) void top_gate_kernel(TopGateArgs A) {
+    u32 n = *A.in_count;
+    if (n > A.in_cap) n = A.in_cap;
+    const u32 bases[3] = {2u, 3u, 5u};
+    const u32 stride = gridDim.x * blockDim.x;
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const ccs_job_survivor sv = ((const ccs_job_survivor*)A.in)[i];
+        const JobRec jr = A.jobs[sv.job_idx];
+        u64 qlo = jr.u0_lo, qhi = jr.u0_hi;
+        add128(qlo, qhi, sv.u_off);
+        U2 t = add(cand_n(A.W, A.W_hi, jr.a, jr.a_hi, qlo, qhi), mk(1, 0));   // p_0 + 1 = W*Q
+        if (A.top_pos > 0) {
+            t.hi = (t.hi << A.top_pos) | (t.lo >> (64 - A.top_pos));
+            t.lo <<= A.top_pos;
+        }
+        const U2 p_top = sub(t, mk(1, 0));
+        if (is_prime_mr(p_top, bases, A.reps)) {
+            const u32 slot = atomicAdd(A.out_count, 1u);
+            if (slot < A.out_cap) ((ccs_job_survivor*)A.out)[slot] = sv;
+        }
+    }
+}
+
 __global__ void zero_kernel(u32* p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0;
@@ -726,6 +758,16 @@ hipError_t launch_prove(const ProveArgs& a, u32 max_entries, hipStream_t s) {
     if (cblocks < 1) cblocks = 1;
     if (cblocks > 2048) cblocks = 2048;
     hipLaunchKernelGGL(chain_kernel, dim3(cblocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_top_gate(const TopGateArgs& a, hipStream_t s) {
+    (void)hipGetLastError();
+    u32 blocks = (a.in_cap + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, a.out_count, (size_t)1);
+    hipLaunchKernelGGL(top_gate_kernel, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

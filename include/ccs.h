@@ -62,13 +62,31 @@ typedef struct {
 #define CCS_SHAPE_NO_CLASSES 1u     /* build filter tables only (class-form jobs)      */
 
 typedef struct {
-    uint64_t M, WP, W;              /* W = 6*WP*M                                       */
+    uint64_t M, WP, W;              /* W = 6*WP*M (family shapes: low 64 bits of D*2^E)  */
     uint32_t n_bases;
     uint32_t n_filter;
     uint64_t n_classes;             /* admissible (base, k) pairs = classes mod W       */
     uint32_t n_dense, n_tierb, n_deep;   /* how the filter primes are split on the GPU  */
     uint32_t rows_per_group, unit_j;
+    uint64_t W_hi;                  /* high 64 bits of W (family shapes)                */
+    int32_t family, exp_start;      /* family shape: n = Q*D*2^exp_start - 1            */
+    ccs_u128 D;                     /* family shape: immune-factor product D            */
 } ccs_shape_info;
+
+/* ---- fingerprint families (the v16 Q-iteration front end, v16...cu:1-30, :556-924,
+ *      :4405-4468, :5104-5227): p_0 = Q*D*2^E - 1, p_j = Q*D*2^(E+j) - 1.  Every prime
+ *      dividing D divides p_j + 1 for all j (the family is immune there); the search
+ *      variable is Q.  A family shape is the class W - 1 mod W = D*2^E (u128): its jobs
+ *      are Q windows [u0, u0 + count) with a = 0, and survivors are (job, Q - u0). ---- */
+typedef struct {
+    ccs_u128 D;                     /* immune-factor product D_V (a pool entry)          */
+    int32_t exp_start;              /* E (v16 --exp-start)                               */
+    int32_t target;                 /* chain length of interest (top position = target-1) */
+    int32_t depth;                  /* positions [0, depth) are sieved for every prime   */
+    uint32_t n_filter;
+    const uint32_t* filter_primes;  /* ascending odd primes < 2^18; those dividing D are  */
+                                    /*   immune and dropped (v16 per-V active primes)    */
+} ccs_family_params;
 
 /* Survivor record — identical to v13 `Survivor` (v13.cu:207-212): tile_idx is relative
  * to the launch's tile_start. */
@@ -179,6 +197,41 @@ int ccs_event_synchronize(ccs_ctx* ctx, void* event);   /* host waits for the ev
  *      gpu_upload_tables :1065-1222) ---- */
 int ccs_shape_build(ccs_ctx* ctx, const ccs_shape_params* params, const ccs_tuning* tuning,
                     ccs_shape** out);
+/* Family shape (the v16 Q-iteration sieve: per-V forbid masks v16...cu:5104-5227 with the
+ * active-prime lists :5229-5300): W = D*2^E, the filter tables of the primes that do not
+ * divide D, positions [0, depth) for every prime (v16_forbidden_residues :1244-1262).
+ * Job sets, batched launches, graphs and proves work on it as on any class-form shape;
+ * plain tile launches, tile ranges and lattice decomposition do not. */
+int ccs_family_shape_build(ccs_ctx* ctx, const ccs_family_params* params, const ccs_tuning* tuning,
+                           ccs_shape** out);
+/* Q window of a family for candidates of bits_lo..bits_hi bits (compute_q_windows_subband,
+ * v16...cu:4405-4468): q_min = ceil(2^(bits_lo-1) / (D*2^E)),
+ * q_max = floor((2^bits_hi - 1) / (D*2^E)); CCS_DONE when q_min > q_max. */
+int ccs_family_window(ccs_u128 D, int32_t exp_start, int32_t bits_lo, int32_t bits_hi,
+                      ccs_u128* q_min, ccs_u128* q_max);
+
+/* A pool entry (v16 v16_pool_seed, v16...cu:137-144): a named D and its prime factors
+ * (a prime power p^k appears k times). */
+typedef struct {
+    char name[64];
+    ccs_u128 D;
+    uint32_t primes[64];
+    uint32_t n_primes, _pad;
+} ccs_pool_entry;
+/* Comma-separated pool list (v16 --seed-pool: parse_seed_pool :650-690, parse_pool_spec
+ * :556-648, expand_auto_pool_one :828-880): "Z#" (primes <= Z), "Z#/X/Y" (with X, Y
+ * omitted), "P*Q^k*R" (explicit, k <= 16), "auto:Z#:K" or "auto:Z#:K1-K2" (every way of
+ * dropping K primes <= Z that are not in keep[]).  Appends to out[]; *n_out = entries
+ * produced (CCS_OVERFLOW when more than cap). */
+int ccs_pool_parse(const char* list, const uint32_t* keep, uint32_t n_keep, ccs_pool_entry* out,
+                   uint32_t cap, uint32_t* n_out);
+/* Pool file: one list per line; everything from the first '#' on a line is a comment
+ * (parse_seed_pool_file, v16...cu:697-738). */
+int ccs_pool_load(const char* path, const uint32_t* keep, uint32_t n_keep, ccs_pool_entry* out,
+                  uint32_t cap, uint32_t* n_out);
+/* first[i] = 1 iff values[i] does not occur at a lower index (the v16 dedup of p_0 over
+ * (Q, V) framings of one chain, v16...cu:2478-2541). */
+int ccs_dedup_u128(const ccs_u128* values, uint32_t count, uint8_t* first);
 void ccs_shape_free(ccs_shape* shape);
 int ccs_shape_query(const ccs_shape* shape, ccs_shape_info* out);
 int ccs_shape_bases(const ccs_shape* shape, uint64_t* h_out, uint32_t cap);
@@ -343,6 +396,14 @@ int ccs_graph_create(ccs_ctx* ctx, ccs_jobset* js, ccs_job_survivor* d_out, uint
                      uint32_t* d_count, ccs_graph** out);
 int ccs_graph_launch(ccs_ctx* ctx, ccs_graph* graph, void* stream);
 void ccs_graph_free(ccs_graph* graph);
+/* K1.5 top-position gate of a family set (v16_k15_topmr_kernel, v16...cu:2185-2296): keeps
+ * the survivors whose p_top = Q*D*2^(E+top_pos) - 1 passes strong Miller-Rabin to the first
+ * `reps` (2 or 3) bases of {2, 3, 5}; compacted into d_out in unspecified order, *d_out_count
+ * = their true count.  Every job's p_top must stay below 2^127. */
+int ccs_family_top_gate_launch(ccs_ctx* ctx, ccs_jobset* js, const ccs_job_survivor* d_in,
+                               const uint32_t* d_in_count, uint32_t in_cap, int32_t top_pos,
+                               int32_t reps, ccs_job_survivor* d_out, uint32_t out_cap,
+                               uint32_t* d_out_count, void* stream);
 int ccs_prove_jobset_launch(ccs_ctx* ctx, ccs_jobset* js, const ccs_job_survivor* d_surv,
                             const uint32_t* d_count, uint32_t cap, const ccs_prove_params* params,
                             ccs_chain_result* d_out, void* stream);

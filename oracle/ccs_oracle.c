@@ -804,6 +804,43 @@ ORC_EXPORT int64_t orc_filter_class(void *hp, u64 a, u64 W, u64 u0_lo, u64 u0_hi
 }
 
 /* ------------------------------------------------------------------------- */
+/* fingerprint family: p_j = Q*D*2^(E+j) - 1 (the v16 Q-iteration sieve)       */
+/* ------------------------------------------------------------------------- */
+
+/* Survivors Q in [q0, q0 + count) of the family D, E: Q is killed when a prime q of the
+ * list with q not dividing D divides some p_j, j in [0, depth), i.e. when
+ * (Q*D*2^E mod q) * 2^j = 1 (mod q) (v16_forbidden_residues v16...cu:1244-1262 and the
+ * per-Q mask probe of v16_q_iter_kernel :2038-2164).  out receives Q - q0. */
+ORC_EXPORT int64_t orc_filter_family(u64 d_lo, u64 d_hi, int E, int depth, const u32 *primes, int np,
+                                     u64 q0_lo, u64 q0_hi, u64 count, u32 *out, int64_t cap) {
+    u128 D = ((u128)d_hi << 64) | d_lo, q0 = ((u128)q0_hi << 64) | q0_lo;
+    u32 *wq = (u32 *)malloc((np ? np : 1) * 4), *qm = (u32 *)malloc((np ? np : 1) * 4);
+    for (int i = 0; i < np; i++) {
+        u32 q = primes[i];
+        u64 w = (u64)(D % q);
+        for (int e = 0; e < E; e++) w = (w * 2) % q;
+        wq[i] = (u32)w;                       /* D*2^E mod q (0: immune) */
+        qm[i] = umod128(q0, q);
+    }
+    int64_t ns = 0;
+    for (u64 k = 0; k < count; k++) {
+        int dead = 0;
+        for (int i = 0; i < np && !dead; i++) {
+            u32 q = primes[i];
+            if (!wq[i]) continue;
+            u64 t = (u64)wq[i] * ((qm[i] + k % q) % q) % q;   /* p_0 + 1 mod q */
+            for (int j = 0; j < depth && !dead; j++) {
+                if (t == 1) dead = 1;             /* q | 2^j (p_0 + 1) - 1 */
+                t = t * 2 % q;
+            }
+        }
+        if (!dead) { if (ns < cap) out[ns] = (u32)k; ns++; }
+    }
+    free(wq); free(qm);
+    return ns;
+}
+
+/* ------------------------------------------------------------------------- */
 /* 64-candidate bit-vector filter (the v34 CPU baseline)                      */
 /* ------------------------------------------------------------------------- */
 

@@ -89,6 +89,9 @@ def lib():
     L.orc_filter_class.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                    C.c_uint64, u32p, C.c_int64, C.c_void_p, C.c_int, C.c_void_p]
     L.orc_filter_class.restype = C.c_int64
+    L.orc_filter_family.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, u32p, C.c_int, C.c_uint64,
+                                    C.c_uint64, C.c_uint64, u32p, C.c_int64]
+    L.orc_filter_family.restype = C.c_int64
     L.orc_filter_bv.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, u32p, C.c_int64,
                                 C.c_int, C.POINTER(C.c_uint64), C.c_int]
     L.orc_filter_bv.restype = C.c_int64
@@ -298,3 +301,13 @@ def tile_range(M, WP, min_base, max_base, bits, prefix=None, lanes=0, lane_id=0)
         start = pmin + lane_id * size + min(lane_id, rem)
         return start, start + size + (1 if lane_id < rem else 0)
     return pmin, pmax
+
+
+def filter_family(D, E, depth, primes, q0, count, cap=1 << 22):
+    """Survivor offsets Q - q0 of the family p_j = Q*D*2^(E+j) - 1 (orc_filter_family)."""
+    pr = np.ascontiguousarray(np.asarray(list(primes), np.uint32))
+    out = np.zeros(cap, np.uint32)
+    n = lib().orc_filter_family(*split(D), E, depth, pr, len(pr), *split(q0), count, out, cap)
+    if n > cap:
+        raise RuntimeError("oracle survivor overflow")
+    return out[:n].copy()
