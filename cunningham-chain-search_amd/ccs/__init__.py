@@ -426,6 +426,33 @@ class JobSet:
                                                  d_out, stream), "prove_jobset_launch")
 
 
+class GroupSet(JobSet):
+    """Jobs of several shapes in one persistent launch (ccs_groupset_create): jobs is a list
+    of (shape_index, (a, u0, count)) grouped by shape index."""
+
+    def __init__(self, shapes, jobs):
+        self.shapes = list(shapes)
+        self.shape, self.ctx, self.lib = self.shapes[0], self.shapes[0].ctx, self.shapes[0].lib
+        self.job_shape = np.ascontiguousarray(np.asarray([k for k, _ in jobs], np.uint32))
+        self.jobs = jobs_array([j for _, j in jobs])
+        arr = (C.c_void_p * len(self.shapes))(*[sh.h.value for sh in self.shapes])
+        self._arr = arr
+        self.h = C.c_void_p()
+        L.check(self.lib.ccs_groupset_create(self.ctx.h, arr, len(self.shapes), self.jobs.ctypes.data,
+                                             self.job_shape.ctypes.data, len(self.jobs), C.byref(self.h)),
+                "ccs_groupset_create")
+        self.candidates = int(self.jobs["count"].sum())
+        self._graphs = []
+
+    def update(self, jobs, stream=None):
+        job_shape = np.ascontiguousarray(np.asarray([k for k, _ in jobs], np.uint32))
+        arr = jobs_array([j for _, j in jobs])
+        L.check(self.lib.ccs_groupset_update(self.ctx.h, self.h, arr.ctypes.data, job_shape.ctypes.data, len(arr),
+                                             stream), "ccs_groupset_update")
+        self.jobs, self.job_shape = arr, job_shape
+        self.candidates = int(arr["count"].sum())
+
+
 class Server:
     """Continuous-batching request loop (ccs_server): submit jobs of any depth / width /
     length, poll completions with their timestamps, fetch per-job results."""

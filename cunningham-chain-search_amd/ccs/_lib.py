@@ -38,6 +38,7 @@ EXPORTS = [
     "ccs_campaign_free", "ccs_campaign_next", "ccs_campaign_record", "ccs_campaign_get_state",
     "ccs_campaign_save", "ccs_campaign_resume", "ccs_family_shape_build", "ccs_family_window",
     "ccs_pool_parse", "ccs_pool_load", "ccs_dedup_u128", "ccs_family_top_gate_launch",
+    "ccs_groupset_create", "ccs_groupset_update",
 ]
 
 
@@ -307,6 +308,8 @@ def load():
     L.ccs_dedup_u128.argtypes = [vp, C.c_uint32, vp]
     L.ccs_family_top_gate_launch.argtypes = [vp, vp, vp, vp, C.c_uint32, C.c_int32, C.c_int32, vp,
                                              C.c_uint32, vp, vp]
+    L.ccs_groupset_create.argtypes = [vp, vp, C.c_uint32, vp, vp, C.c_uint32, C.POINTER(vp)]
+    L.ccs_groupset_update.argtypes = [vp, vp, vp, vp, C.c_uint32, vp]
     L.ccs_profile_start.argtypes = [vp]
     L.ccs_profile_stop.argtypes = [vp, C.POINTER(KernelTimes)]
     _lib = L

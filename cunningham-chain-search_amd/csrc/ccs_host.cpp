@@ -366,6 +366,7 @@ struct ccs_shape {
     u32 lds_words = 0, nd = 0, nd_primes = 0, nb = 0, nc = 0;   // nd: dense slots
     u32 nb_lds = 0;                          // stage-B primes with their bitsets in LDS
     double est_survival_b = 1.0;             // expected fraction alive after stage B
+    double est_survival_d = 1.0;             // expected fraction alive after the dense stage
 };
 
 static bool is_prime_u32(u32 q) {
@@ -591,7 +592,11 @@ static int upload_tables(ccs_shape* s) {
     if (image.empty()) image.push_back(0);
     // expected survival after the GPU's stage B (sizes the stage-B survivor buffer)
     double surv = 1.0;
-    for (u32 i = 0; i < nd + nb; ++i) surv *= 1.0 - (double)s->forb[i].size() / s->fp[i];
+    for (u32 i = 0; i < nd + nb; ++i) {
+        surv *= 1.0 - (double)s->forb[i].size() / s->fp[i];
+        if (i + 1 == nd) s->est_survival_d = surv;
+    }
+    if (nd == 0) s->est_survival_d = 1.0;
     s->est_survival_b = surv;
 
     free_device_tables(s);
@@ -884,22 +889,39 @@ struct Scratch {
 
 static Scratch ctx_scratch(ccs_ctx* c) { return Scratch{c->mid, c->mid_cap, c->counters, c->stats}; }
 
+static ShapeTab shape_tab(const ccs_shape* s) {
+    ShapeTab t;
+    t.lds_image = s->d_lds_image;
+    t.lds_words = s->lds_words;
+    t.nd = s->nd;
+    t.nb = s->nb;
+    t.nb_lds = s->nb_lds;
+    t.dense = s->d_dense;
+    t.tierb = s->d_tierb;
+    t.tierb_far = s->d_tierb_far;
+    t.pmod = s->d_pmod;
+    return t;
+}
+
+static DeepTab deep_tab(const ccs_shape* s) {
+    DeepTab t;
+    t.deep = s->d_deep;
+    t.hash = s->d_hash;
+    t.nc = s->nc;
+    t.pad = 0;
+    return t;
+}
+
 static void fill_sieve_common(const ccs_shape* s, const Scratch& x, SieveArgs& a) {
     memset(&a, 0, sizeof(a));
     a.W = s->W;
     a.W_hi = s->W_hi;
-    a.lds_image = s->d_lds_image;
-    a.lds_words = s->lds_words;
+    a.tab = shape_tab(s);
+    a.group = nullptr;
+    a.n_group = 1;
     a.lds_ctl = (s->lds_words + 1) & ~1u;
-    a.nd = s->nd;
-    a.nb = s->nb;
-    a.nb_lds = s->nb_lds;
     a.one = 1;
     a.unit_j = s->tune.unit_j;
-    a.dense = s->d_dense;
-    a.tierb = s->d_tierb;
-    a.tierb_far = s->d_tierb_far;
-    a.pmod = s->d_pmod;
     a.unit_counter = (u64*)(x.counters + 2);
     a.mid = x.mid;
     a.mid_cap = x.mid_cap;
@@ -911,20 +933,21 @@ static void fill_sieve_common(const ccs_shape* s, const Scratch& x, SieveArgs& a
 // (the per-job loop appends several launches into one output).
 static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, SieveArgs& sa, DeepArgs& da,
                               u64 n_units, u32* d_count, hipStream_t st, bool zero_out = true,
-                              bool zero_stats = true) {
+                              bool zero_stats = true, u32 nb_max = 0) {
     // fill kernels rather than memset nodes, so the sequence can be graph-captured
     if (launch_zero(x.counters, 16, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
     if (zero_stats && launch_zero((u32*)x.stats, 16, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
     if (zero_out && launch_zero(d_count, 1, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
     sa.n_units = n_units;
-    const int nbv = s->nb ? (int)((s->nb + 63) / 64) : 1;
+    // a group launch takes its stage-B register width from its widest shape (nb_max)
+    const u32 nbm = nb_max > s->nb ? nb_max : s->nb;
+    const int nbv = nbm ? (int)((nbm + 63) / 64) : 1;
     const u32 wpb = s->tune.waves_per_block;
     const int block = 64 * (int)wpb;
-    const size_t lds = (size_t)(((s->lds_words + 1) & ~1u) + kLdsCtlWords) * 4;
+    const size_t lds = (size_t)(sa.lds_ctl + kLdsCtlWords) * 4;
     u64 want_grid = (u64)c->n_cu * s->tune.blocks_per_cu;
     const u64 blocks_needed = (n_units + wpb - 1) / wpb;
     if (want_grid > blocks_needed) want_grid = blocks_needed ? blocks_needed : 1;
-    sa.sched_div = 4 * want_grid * wpb;   // claims of a quarter of each wave's fair share
     c->last_timed = c->timed;
     if (c->timed) HIPCHK(hipEventRecord(c->ev[0], st));
     prof_begin(c, 0, st);
@@ -1070,8 +1093,6 @@ static int fill_prove(const ccs_shape* s, const ccs_prove_params* p, ProveArgs& 
     memset(&a, 0, sizeof(a));
     a.WP = s->WP;
     a.M = s->M;
-    a.W = s->W;
-    a.W_hi = s->W_hi;
     a.bases = s->d_bases;
     a.bits = p ? p->bits : 0;
     a.test_kind = p ? p->test_kind : CCS_TEST_BPSW;
@@ -1225,6 +1246,14 @@ struct ccs_jobset {
     void* chain = nullptr;          // prove chain queue of launches on this set
     u64 chain_bytes = 0;
     Scratch x = {nullptr, 0, nullptr, nullptr};
+    // group sets: jobs of several shapes, grouped by shape
+    std::vector<const ccs_shape*> group;
+    ShapeTab* d_group = nullptr;
+    DeepTab* d_deep_group = nullptr;
+    u64* d_shape_units = nullptr;   // [n_group + 1] first unit of each shape's jobs
+    u64* staging_units = nullptr;
+    std::vector<u32> job_shape;
+    u32 lds_ctl = 0, nb_max = 0;
 };
 
 extern "C" void ccs_jobset_free(ccs_jobset* js) {
@@ -1237,6 +1266,10 @@ extern "C" void ccs_jobset_free(ccs_jobset* js) {
     (void)hipFree(js->x.stats);
     (void)hipFree(js->chain);
     (void)hipFree(js->d_each);
+    (void)hipFree(js->d_group);
+    (void)hipFree(js->d_deep_group);
+    (void)hipFree(js->d_shape_units);
+    if (js->staging_units) (void)hipHostFree(js->staging_units);
     if (js->staging) (void)hipHostFree(js->staging);
     if (js->staging_prefix) (void)hipHostFree(js->staging_prefix);
     if (js->staged) (void)hipEventDestroy(js->staged);
@@ -1270,6 +1303,10 @@ static int job_record(const ccs_shape* s, const ccs_job& j, JobRec& r) {
         r.u0_hi = j.u0.hi;
     }
     r.count = j.count;
+    r.W = s->W;
+    r.W_hi = s->W_hi;
+    r.shape = 0;
+    r.unit_j = 0;
     return CCS_OK;
 }
 
@@ -1342,6 +1379,190 @@ extern "C" int ccs_jobset_create(ccs_ctx* c, const ccs_shape* s, const ccs_job* 
     return CCS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// group sets: jobs of several shapes in one launch (the sieve kernel loads each shape's
+// tables as its work reaches that shape's jobs)
+// ---------------------------------------------------------------------------
+
+// Work-unit sizes of a group set.  The shapes of a group can differ several-fold in sieve
+// time per candidate (the dense survivors go one by one through stage B; depth 12 leaves
+// ~30x more of them than depth 20), so each shape gets units of about equal cost:
+// cost per candidate ~ 1 + 900 * (dense-stage survival), the ratio of the per-survivor to
+// the per-candidate time measured on the 31# depth profiles.  The cheapest shape takes the
+// size set_unit picks for the whole set's weighted work, the others proportionally
+// smaller units (powers of two from 2^14).
+static double sieve_weight(const ccs_shape* s) { return 1.0 + 900.0 * s->est_survival_d; }
+
+static void group_units_per_job(const ccs_ctx* c, ccs_jobset* js, const ccs_job* jobs, u32 n_jobs, JobRec* rec) {
+    double wmin = 1e300;
+    for (const ccs_shape* g : js->group) wmin = std::min(wmin, sieve_weight(g));
+    const u32 base = set_unit(c, js->shape, n_jobs, [&](u32 i) {
+        return (u64)((double)jobs[i].count * sieve_weight(js->group[js->job_shape[i]]) / wmin);
+    });
+    js->unit_j = base;
+    std::vector<u32> per(js->group.size());
+    for (size_t k = 0; k < per.size(); ++k) {
+        const double want = (double)base * wmin / sieve_weight(js->group[k]);
+        u32 u = 1u << 14;
+        while ((double)u * 2 <= want && u < base && u < (1u << 20)) u *= 2;
+        per[k] = u;
+    }
+    for (u32 i = 0; i < n_jobs; ++i) rec[i].unit_j = per[js->job_shape[i]];
+}
+
+// the first unit of each shape's jobs (jobs are grouped by shape)
+static void group_units(const ccs_jobset* js, std::vector<u64>& units) {
+    const u32 ng = (u32)js->group.size();
+    units.assign((size_t)ng + 1, js->n_units);
+    u32 k = 0;
+    for (u32 i = 0; i < js->n_jobs; ++i)
+        while (k <= js->job_shape[i]) units[k++] = js->prefix[i];
+    while (k <= ng) units[k++] = js->n_units;
+}
+
+static int group_check(const ccs_shape* const* shapes, uint32_t n_shapes, const uint32_t* job_shape,
+                       uint32_t n_jobs) {
+    if (!shapes || !n_shapes || !job_shape) return fail(CCS_ERR_INVALID, "null argument");
+    for (u32 k = 0; k < n_shapes; ++k) {
+        if (!shapes[k]) return fail(CCS_ERR_INVALID, "null shape");
+        if (shapes[k]->device != shapes[0]->device) return fail(CCS_ERR_INVALID, "shapes on different devices");
+        if (shapes[k]->tune.rows_per_group != shapes[0]->tune.rows_per_group)
+            return fail(CCS_ERR_INVALID, "the shapes of a group need the same rows_per_group");
+    }
+    for (u32 i = 0; i < n_jobs; ++i) {
+        if (job_shape[i] >= n_shapes) return fail(CCS_ERR_INVALID, "job_shape out of range");
+        if (i && job_shape[i] < job_shape[i - 1]) return fail(CCS_ERR_INVALID, "jobs must be grouped by shape (job_shape non-decreasing)");
+    }
+    return CCS_OK;
+}
+
+extern "C" int ccs_groupset_create(ccs_ctx* c, const ccs_shape* const* shapes, uint32_t n_shapes,
+                                   const ccs_job* jobs, const uint32_t* job_shape, uint32_t n_jobs,
+                                   ccs_jobset** out) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!jobs || !out || !n_jobs) return fail(CCS_ERR_INVALID, "null argument or no jobs");
+    *out = nullptr;
+    if ((rc = group_check(shapes, n_shapes, job_shape, n_jobs))) return rc;
+    std::vector<JobRec> jr(n_jobs);
+    for (u32 i = 0; i < n_jobs; ++i) {
+        if ((rc = job_record(shapes[job_shape[i]], jobs[i], jr[i]))) return rc;
+        jr[i].shape = job_shape[i];
+    }
+    ccs_jobset* js = new ccs_jobset();
+    js->device = c->device;
+    js->shape = shapes[0];
+    js->n_jobs = n_jobs;
+    js->group.assign(shapes, shapes + n_shapes);
+    js->job_shape.assign(job_shape, job_shape + n_jobs);
+    double surv = 0;
+    for (u32 k = 0; k < n_shapes; ++k) {
+        js->lds_ctl = std::max(js->lds_ctl, (shapes[k]->lds_words + 1) & ~1u);
+        js->nb_max = std::max(js->nb_max, shapes[k]->nb);
+    }
+    group_units_per_job(c, js, jobs, n_jobs, jr.data());
+    js->hjobs = jr;
+    js->prefix.resize((size_t)n_jobs + 1);
+    for (u32 i = 0; i < n_jobs; ++i) {
+        js->prefix[i] = js->n_units;
+        js->n_units += (jobs[i].count + jr[i].unit_j - 1) / jr[i].unit_j;
+        js->candidates += jobs[i].count;
+        js->counts.push_back(jobs[i].count);
+        surv += (double)jobs[i].count * shapes[job_shape[i]]->est_survival_b;
+    }
+    js->prefix[n_jobs] = js->n_units;
+    js->cap_jobs = n_jobs;
+    std::vector<ShapeTab> tabs(n_shapes);
+    std::vector<DeepTab> dtabs(n_shapes);
+    for (u32 k = 0; k < n_shapes; ++k) {
+        tabs[k] = shape_tab(shapes[k]);
+        dtabs[k] = deep_tab(shapes[k]);
+    }
+    std::vector<u64> units;
+    group_units(js, units);
+    u64 want = (u64)(surv * 2.0) + (1u << 20);
+    if (want > (1u << 28)) want = 1u << 28;
+    hipError_t e = hipMalloc((void**)&js->d_jobs, (size_t)n_jobs * sizeof(JobRec));
+    if (e == hipSuccess) e = hipMalloc((void**)&js->d_prefix, ((size_t)n_jobs + 1) * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&js->d_group, n_shapes * sizeof(ShapeTab));
+    if (e == hipSuccess) e = hipMalloc((void**)&js->d_deep_group, n_shapes * sizeof(DeepTab));
+    if (e == hipSuccess) e = hipMalloc((void**)&js->d_shape_units, ((size_t)n_shapes + 1) * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&js->x.mid, want * sizeof(MidRec));
+    if (e == hipSuccess) e = hipMalloc((void**)&js->x.counters, 64);
+    if (e == hipSuccess) e = hipMalloc((void**)&js->x.stats, 64);
+    if (e == hipSuccess) e = hipMemcpy(js->d_jobs, jr.data(), jr.size() * sizeof(JobRec), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(js->d_prefix, js->prefix.data(), js->prefix.size() * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(js->d_group, tabs.data(), tabs.size() * sizeof(ShapeTab), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(js->d_deep_group, dtabs.data(), dtabs.size() * sizeof(DeepTab), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(js->d_shape_units, units.data(), units.size() * 8, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        ccs_jobset_free(js);
+        return fail(CCS_ERR_HIP, std::string("group set upload: ") + hipGetErrorString(e));
+    }
+    js->x.mid_cap = (u32)want;
+    *out = js;
+    return CCS_OK;
+}
+
+// new jobs (at most as many as the set was created with) for the same shapes; asynchronous
+// on `stream` like ccs_jobset_update
+extern "C" int ccs_groupset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jobs, const uint32_t* job_shape,
+                                   uint32_t n_jobs, void* stream) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!js || !h_jobs || !n_jobs) return fail(CCS_ERR_INVALID, "null argument or no jobs");
+    if (js->group.empty()) return fail(CCS_ERR_INVALID, "not a group set (use ccs_jobset_update)");
+    if (n_jobs > js->cap_jobs) return fail(CCS_ERR_INVALID, "more jobs than the set was created with");
+    if ((rc = group_check(js->group.data(), (u32)js->group.size(), job_shape, n_jobs))) return rc;
+    if (!js->staging) {
+        HIPCHK(hipHostMalloc((void**)&js->staging, (size_t)js->cap_jobs * sizeof(JobRec)));
+        HIPCHK(hipHostMalloc((void**)&js->staging_prefix, ((size_t)js->cap_jobs + 1) * 8));
+        HIPCHK(hipHostMalloc((void**)&js->staging_units, (js->group.size() + 1) * 8));
+        HIPCHK(hipEventCreateWithFlags(&js->staged, hipEventDisableTiming));
+    } else {
+        HIPCHK(hipEventSynchronize(js->staged));
+    }
+    double surv = 0;
+    for (u32 i = 0; i < n_jobs; ++i) {
+        JobRec tmp;
+        if ((rc = job_record(js->group[job_shape[i]], h_jobs[i], tmp))) return rc;
+        surv += (double)h_jobs[i].count * js->group[job_shape[i]]->est_survival_b;
+    }
+    if ((u64)(surv * 2.0) + (1u << 16) > js->x.mid_cap)
+        return fail(CCS_ERR_INVALID, "jobs exceed the stage-B buffer of the set");
+    js->prefix.resize((size_t)n_jobs + 1);
+    js->counts.resize(n_jobs);
+    js->hjobs.resize(n_jobs);
+    js->job_shape.assign(job_shape, job_shape + n_jobs);
+    js->each_valid = false;
+    for (u32 i = 0; i < n_jobs; ++i) {
+        (void)job_record(js->group[job_shape[i]], h_jobs[i], js->staging[i]);
+        js->staging[i].shape = job_shape[i];
+    }
+    group_units_per_job(c, js, h_jobs, n_jobs, js->staging);
+    u64 units = 0, cand = 0;
+    for (u32 i = 0; i < n_jobs; ++i) {
+        js->counts[i] = h_jobs[i].count;
+        js->hjobs[i] = js->staging[i];
+        js->prefix[i] = js->staging_prefix[i] = units;
+        units += (h_jobs[i].count + js->staging[i].unit_j - 1) / js->staging[i].unit_j;
+        cand += h_jobs[i].count;
+    }
+    js->prefix[n_jobs] = js->staging_prefix[n_jobs] = units;
+    js->n_units = units;
+    js->candidates = cand;
+    js->n_jobs = n_jobs;
+    std::vector<u64> su;
+    group_units(js, su);
+    std::memcpy(js->staging_units, su.data(), su.size() * 8);
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    HIPCHK(hipMemcpyAsync(js->d_jobs, js->staging, (size_t)n_jobs * sizeof(JobRec), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(js->d_prefix, js->staging_prefix, ((size_t)n_jobs + 1) * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(js->d_shape_units, js->staging_units, su.size() * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipEventRecord(js->staged, st));
+    return CCS_OK;
+}
+
 // replace the jobs of a set (n_jobs <= the count it was created with; same shape) with an
 // asynchronous upload on `stream`; h_jobs must stay valid until the stream reaches it
 extern "C" int ccs_jobset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jobs, uint32_t n_jobs,
@@ -1350,6 +1571,7 @@ extern "C" int ccs_jobset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jo
     if (rc) return rc;
     if (!js || !h_jobs || n_jobs == 0) return fail(CCS_ERR_INVALID, "null argument or no jobs");
     if (n_jobs > js->cap_jobs) return fail(CCS_ERR_INVALID, "more jobs than the set was created with");
+    if (!js->group.empty()) return fail(CCS_ERR_INVALID, "a group set takes ccs_groupset_update");
     const ccs_shape* s = js->shape;
     if (!js->staging) {
         HIPCHK(hipHostMalloc((void**)&js->staging, (size_t)js->cap_jobs * sizeof(JobRec)));
@@ -1404,6 +1626,14 @@ static void jobset_args(const ccs_jobset* js, SieveArgs& sa, DeepArgs& da, void*
     da.hash = s->d_hash;
     da.nc = s->nc;
     da.mode = 1;
+    if (!js->group.empty()) {
+        sa.unit_j = 0;                 // each job's own unit size
+        sa.group = js->d_group;
+        sa.n_group = (u32)js->group.size();
+        sa.shape_units = js->d_shape_units;
+        sa.lds_ctl = js->lds_ctl;
+        da.group = js->d_deep_group;
+    }
     da.jobsrc = sa.jobsrc;
     da.out = d_out;
     da.out_cap = cap;
@@ -1427,7 +1657,7 @@ extern "C" int ccs_jobset_launch(ccs_ctx* c, ccs_jobset* js, ccs_job_survivor* d
     jobset_args(js, sa, da, d_out, cap);
     jobset_stats_begin(c, js);
     return run_filter_kernels(c, js->shape, js->x, sa, da, js->n_units, d_count,
-                              stream ? (hipStream_t)stream : c->stream);
+                              stream ? (hipStream_t)stream : c->stream, true, true, js->nb_max);
 }
 
 // the plain per-shape loop: one launch per job (the v16 dispatch pattern, v16...cu:5611-5965),
@@ -1467,7 +1697,13 @@ extern "C" int ccs_jobset_launch_each(ccs_ctx* c, ccs_jobset* js, ccs_job_surviv
         sa.jobsrc.unit_prefix = js->d_each + 2 * (size_t)i;
         sa.jobsrc.n_jobs = 1;
         sa.jobsrc.job_base = i;
-        rc = run_filter_kernels(c, js->shape, js->x, sa, da, units, d_count, st, false, false);
+        if (!js->group.empty()) {   // one shape per launch: the job's own
+            const ccs_shape* sh = js->group[js->job_shape[i]];
+            sa.group = nullptr;
+            sa.n_group = 1;
+            sa.tab = shape_tab(sh);
+        }
+        rc = run_filter_kernels(c, js->shape, js->x, sa, da, units, d_count, st, false, false, js->nb_max);
         if (rc) return rc;
     }
     return CCS_OK;
@@ -1522,7 +1758,7 @@ extern "C" int ccs_graph_create(ccs_ctx* c, ccs_jobset* js, ccs_job_survivor* d_
     SieveArgs sa;
     DeepArgs da;
     jobset_args(js, sa, da, d_out, cap);
-    rc = run_filter_kernels(c, js->shape, js->x, sa, da, js->n_units, d_count, c->stream);
+    rc = run_filter_kernels(c, js->shape, js->x, sa, da, js->n_units, d_count, c->stream, true, true, js->nb_max);
     e = hipStreamEndCapture(c->stream, &g->graph);
     c->profiling = prof;
     c->timed = timed;
@@ -1648,21 +1884,20 @@ extern "C" int ccs_family_top_gate_launch(ccs_ctx* c, ccs_jobset* js, const ccs_
     if (!js || !d_in || !d_in_count || !d_out || !d_out_count) return fail(CCS_ERR_INVALID, "null argument");
     const ccs_shape* s = js->shape;
     if (!s->family) return fail(CCS_ERR_INVALID, "the top gate needs a family shape");
+    for (const ccs_shape* g : js->group)
+        if (!g->family) return fail(CCS_ERR_INVALID, "the top gate needs family shapes");
     if (top_pos < 0 || top_pos > 63) return fail(CCS_ERR_INVALID, "top_pos must be in 0..63");
     if (reps != 2 && reps != 3) return fail(CCS_ERR_INVALID, "reps must be 2 ({2,3}) or 3 ({2,3,5})");
     // every job's p_top = (W*Q) << top_pos - 1 must be below 2^127 (Montgomery range)
-    const u128 W = wfull(s);
-    const u128 lim = (((u128)1 << 127) >> top_pos) / W;
     for (u32 i = 0; i < js->n_jobs; ++i) {
         const JobRec& r = js->hjobs[i];
         if (!r.count) continue;
+        const u128 lim = (((u128)1 << 127) >> top_pos) / (((u128)r.W_hi << 64) | r.W);
         const u128 qlast = (((u128)r.u0_hi << 64) | r.u0_lo) + r.count;   // device u0 = Q0 - 1
         if (qlast > lim) return fail(CCS_ERR_INVALID, "p_top of a job reaches 2^127");
     }
     TopGateArgs a;
     a.jobs = js->d_jobs;
-    a.W = s->W;
-    a.W_hi = s->W_hi;
     a.in = d_in;
     a.in_count = d_in_count;
     a.in_cap = in_cap;

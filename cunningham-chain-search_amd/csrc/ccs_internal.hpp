@@ -75,6 +75,9 @@ struct JobRec {     // class-form job: n = a + W*(u0 + j), j in [0, count); a an
     u64 a, a_hi;
     u64 u0_lo, u0_hi;
     u64 count;
+    u64 W, W_hi;      // the modulus of the job's shape
+    u32 shape;        // the job's shape within a group launch (0 otherwise)
+    u32 unit_j;       // candidates per work unit of this job (a launch's unit_j overrides)
 };
 
 struct MidRec {     // stage-B survivor: owner (class or job index) and j
@@ -99,26 +102,34 @@ struct JobSrc {
     u32 job_base;             // added to the job index recorded as a survivor's owner
 };
 
-struct SieveArgs {
-    u64 W, W_hi;              // the shape modulus (u128)
-    // tables
-    const u32* lds_image;     // dense patterns + tier-B bitsets, copied to LDS at start
+// One shape's sieve tables (device pointers).  A launch sieves one shape, or several
+// (a group: a pool of families, the depth profiles of a server) whose jobs are grouped by
+// shape; a block then loads each shape's tables into LDS as its work reaches that shape.
+struct ShapeTab {
+    const u32* lds_image;     // dense patterns + tier-B bitsets, copied to LDS
     u32 lds_words;
-    u32 lds_ctl;              // word offset of the scheduler words (kLdsCtlWords) in LDS
     u32 nd;                   // dense slots (one or two primes each)
     u32 nb;                   // tier-B primes
     u32 nb_lds;               // tier-B primes with bitsets in LDS (whole groups of 64)
-    u32 unit_j;               // candidates (u values) per work unit
-    u32 one;                  // 1 (see the dense loop)
     const DensePrime* dense;
     const TierBPrime* tierb;
     const u32* tierb_far;     // bitsets of stage-B primes >= nb_lds (lds_off indexes here)
     const PrimeMod* pmod;     // [nd + nb] residue constants: dense slot moduli, then tier-B primes
+};
+
+struct SieveArgs {
+    u64 W, W_hi;              // the shape modulus (u128; one-shape launches)
+    ShapeTab tab;             // the tables of a one-shape launch
+    const ShapeTab* group;    // [n_group] tables of a group launch (null: one shape)
+    u32 n_group;
+    u32 lds_ctl;              // word offset of the scheduler words (kLdsCtlWords) in LDS
+    const u64* shape_units;   // [n_group + 1] first unit of each shape's jobs (group launch)
+    u32 unit_j;               // candidates per work unit (job launches: 0 = each job's own)
+    u32 one;                  // 1 (see the dense loop)
     // work
     u64 n_units;
     u64 unit_base;            // added to the cursor value: first unit of this launch
     u64* unit_counter;        // work-stealing cursor (zeroed before launch)
-    u64 sched_div;            // a claim takes max(1, remaining units / sched_div), <= 64
     int mode;                 // 0 = plain, 1 = jobs
     PlainSrc plain;
     JobSrc jobsrc;
@@ -130,11 +141,20 @@ struct SieveArgs {
                                // survivors accumulated by the deep kernel (may be null)
 };
 
-struct DeepArgs {
-    u64 W, W_hi;              // the shape modulus (u128)
+// One shape's deep-stage tables (group launches index them by the job's shape).
+struct DeepTab {
     const DeepPrime* deep;
     const u32* hash;
     u32 nc;
+    u32 pad;
+};
+
+struct DeepArgs {
+    u64 W, W_hi;              // the shape modulus (u128; plain launches)
+    const DeepPrime* deep;
+    const u32* hash;
+    u32 nc;
+    const DeepTab* group;     // per-shape deep tables of a group launch (null: one shape)
     int mode;
     PlainSrc plain;
     JobSrc jobsrc;
@@ -154,8 +174,7 @@ struct ProveArgs {
     u64 T0_lo, T0_hi;
     u64 WP, M;
     const u64* bases;
-    // jobs reconstruction n = a + W*(u0 + u_off), W u128
-    u64 W, W_hi;
+    // jobs reconstruction n = a + W*(u0 + u_off) from the job records
     const JobRec* jobs;
     const void* surv;
     const u32* surv_count;
@@ -171,8 +190,7 @@ struct ProveArgs {
 
 // K1.5 family top-position gate
 struct TopGateArgs {
-    const JobRec* jobs;
-    u64 W, W_hi;
+    const JobRec* jobs;        // p_0 = a + W*u of each job (family shapes: W - 1 + W*u)
     const void* in;            // ccs_job_survivor[in_cap]
     const u32* in_count;
     u32 in_cap;

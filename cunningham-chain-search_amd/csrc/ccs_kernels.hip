@@ -121,10 +121,11 @@ __device__ __forceinline__ bool resolve(const SieveArgs& A, u64 unit, u32 lane, 
     const JobSrc& J = A.jobsrc;
     u32 job = find_job(J, unit, lane);
     JobRec jr = J.jobs[job];
-    u64 beg = (unit - J.unit_prefix[job]) * A.unit_j;
+    const u32 uj = A.unit_j ? A.unit_j : jr.unit_j;
+    u64 beg = (unit - J.unit_prefix[job]) * uj;
     if (beg >= jr.count) return false;
     u64 n = jr.count - beg;
-    if (n > A.unit_j) n = A.unit_j;
+    if (n > uj) n = uj;
     U.a = jr.a;
     U.a_hi = jr.a_hi;
     U.us_lo = jr.u0_lo;
@@ -197,40 +198,56 @@ __device__ __forceinline__ bool stage_b_pass(const u32* __restrict__ lds, const 
            stage_b_rest<NBV>(lds, far, nb_lds, R, valid, j);
 }
 
-template <int RG, int NBV>
+template <int RG, int NBV, bool GROUP>
 __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
     extern __shared__ u32 lds[];
     const u32 tid = threadIdx.x;
-    for (u32 i = tid; i < A.lds_words; i += blockDim.x) lds[i] = A.lds_image[i];
     u32* const ctl = lds + A.lds_ctl;   // scheduler words past the tables (8-byte aligned)
-    if (tid == 0) ctl[0] = 0;
-    __syncthreads();
-
     const u32 lane = lane_id();
 
-    // dense prime `lane` (lane < nd): modulus, pattern offset, per-row-group advance
+    // the tables of the shape whose patterns are in LDS (a one-shape launch reads them from
+    // its arguments); per lane: dense slot `lane` (modulus, pattern offset, per-row-group
+    // advance) and stage-B prime 64*i + lane
+    u32 g_nd = 0, g_nb = 0, g_nb_lds = 0;
+    const u32* g_far = nullptr;
+    const PrimeMod* g_pmod = nullptr;
+    auto t_nd = [&]() -> u32 { if constexpr (GROUP) return g_nd; else return A.tab.nd; };
+    auto t_nb = [&]() -> u32 { if constexpr (GROUP) return g_nb; else return A.tab.nb; };
+    auto t_nb_lds = [&]() -> u32 { if constexpr (GROUP) return g_nb_lds; else return A.tab.nb_lds; };
+    auto t_far = [&]() -> const u32* { if constexpr (GROUP) return g_far; else return A.tab.tierb_far; };
+    auto t_pmod = [&]() -> const PrimeMod* { if constexpr (GROUP) return g_pmod; else return A.tab.pmod; };
     u32 dq = 1, doff = 0, dstep = 0;
-    if (lane < A.nd) {
-        const DensePrime d = A.dense[lane];
-        dq = d.q;
-        doff = d.lds_off;
-        dstep = d.step;
-    }
-    u64 cnt_dense = 0;   // candidates alive after the dense stage (= tested in stage B)
     TierBRegs<NBV> R;
     u64 valid[NBV];   // lanes holding a stage-B prime, per slot
+    auto set_regs = [&](const ShapeTab T) {
+        g_nd = T.nd;
+        g_nb = T.nb;
+        g_nb_lds = T.nb_lds;
+        g_far = T.tierb_far;
+        g_pmod = T.pmod;
+        dq = 1;
+        doff = 0;
+        dstep = 0;
+        if (lane < T.nd) {
+            const DensePrime d = T.dense[lane];
+            dq = d.q;
+            doff = d.lds_off;
+            dstep = d.step;
+        }
 #pragma unroll
-    for (int i = 0; i < NBV; ++i) {
-        const u32 p = (u32)i * 64u + lane;
-        TierBPrime tp;
-        tp.q = 1; tp.lds_off = 0; tp.rcp = 1.0f; tp.pad = 0;
-        if (p < A.nb) tp = A.tierb[p];
-        R.s[i] = 0;
-        R.nq[i] = -(int)tp.q;
-        R.off[i] = tp.lds_off;
-        R.rcp[i] = __float_as_uint(tp.rcp);
-        valid[i] = ballot(p < A.nb);
-    }
+        for (int i = 0; i < NBV; ++i) {
+            const u32 p = (u32)i * 64u + lane;
+            TierBPrime tp;
+            tp.q = 1; tp.lds_off = 0; tp.rcp = 1.0f; tp.pad = 0;
+            if (p < T.nb) tp = T.tierb[p];
+            R.s[i] = 0;
+            R.nq[i] = -(int)tp.q;
+            R.off[i] = tp.lds_off;
+            R.rcp[i] = __float_as_uint(tp.rcp);
+            valid[i] = ballot(p < T.nb);
+        }
+    };
+    u64 cnt_dense = 0;   // candidates alive after the dense stage (= tested in stage B)
 
     // Stage-B survivors wait in registers (lane k holds the k-th) and go out 64 at a time
     // with one atomic, so the shared output cursor is not a point of contention when
@@ -257,184 +274,239 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
     //  B: the last n_pool units (a sixteenth, at least one per wave of the grid) form a
     //     pool shared by all blocks, drawn in block-wide rounds: thread 0 claims one unit
     //     per wave for the whole block with one global atomic (the next round's claim is in
-    //     flight while this round is sieved).  Every wave passes the two barriers of every
-    //     round and all leave in the same round (the one whose claim starts past the
-    //     pool), so the barriers always match.
+    //     flight while this round is sieved).
+    // The work comes in steps: a phase-A range, or a round, cut where the units change
+    // shape.  Every step starts with two barriers (and a table load between them when the
+    // shape changes), every round with two more; the step sequence depends only on
+    // block-uniform values (the ranges, the claims read back from LDS, the shape bounds),
+    // so every wave passes the same barriers, and all leave in the same round.
     const u32 wpb = blockDim.x >> 6, wib = tid >> 6;
-    const u64 n_blocks = gridDim.x;
-    u64 n_pool = A.n_units / 16;
+    const u32 n_blocks = gridDim.x;
+    const u32 n_units = (u32)A.n_units;   // < 2^32 (checked at launch)
+    u32 n_pool = n_units / 16;
     if (n_pool < n_blocks * wpb) n_pool = n_blocks * wpb;
-    if (n_pool > A.n_units) n_pool = A.n_units;
-    const u64 n_a = A.n_units - n_pool;
-    const u64 blk_lo = n_a * blockIdx.x / n_blocks, blk_hi = n_a * (blockIdx.x + 1) / n_blocks;
-    const u32 blk_n = (u32)(blk_hi - blk_lo);
-    u32 seen = 0;                       // phase-A cursor value this wave last drew
+    if (n_pool > n_units) n_pool = n_units;
+    const u32 n_a = n_units - n_pool;
+    const u32 blk_lo = (u32)((u64)n_a * blockIdx.x / n_blocks), blk_hi = (u32)((u64)n_a * (blockIdx.x + 1) / n_blocks);
+    // shape of a unit, and the end of that shape's units (one shape outside group launches)
+    auto shape_of = [&](u32 u) -> u32 {
+        if constexpr (!GROUP) return 0;
+        u32 lo = 0, hi = A.n_group;   // shape_units[lo] <= u < shape_units[hi]
+        while (hi - lo > 1) {
+            const u32 mid = (lo + hi) >> 1;
+            if ((u32)A.shape_units[mid] <= u) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    };
+    auto shape_end = [&](u32 sh) -> u32 {
+        if constexpr (GROUP) return (u32)A.shape_units[sh + 1];
+        return n_units;
+    };
+
+    if constexpr (!GROUP) set_regs(A.tab);   // one shape: its lane constants are loop-invariant
+    u32 next_g = 0;   // thread 0: the claim for the next round (one unit per wave)
+    if (tid == 0) next_g = (u32)atomicAdd(A.unit_counter, (u64)wpb);
+    u32 cur_shape = 0xffffffffu;
+    u32 a_next = blk_lo;                       // phase A: start of the next step
     bool in_pool = false;
-    u64 next_g = 0;   // thread 0: the claim for the next round (one unit per wave)
-    if (tid == 0) next_g = atomicAdd(A.unit_counter, (u64)wpb);
-    u64 cur = 0, cur_end = 0;
+    u32 r_base = 0, r_next = 0, r_end = 0;     // phase B: the round, the next step's start
     u32 sd = 0;                                             // dense phase (lane < nd)
     u32 prev_owner = 0xffffffffu, prev_j0 = 0, prev_n = 0;  // the wave's previous unit
     for (;;) {
-        if (cur == cur_end && !in_pool) {
-            const u32 rem = blk_n > seen ? blk_n - seen : 0;
-            u32 c = rem / (8 * wpb);
-            c = c < 1 ? 1 : (c > 64 ? 64 : c);
-            u32 got = 0;
-            if (lane == 0) got = atomicAdd(&ctl[0], c);
-            got = bcast32(got);
-            if (got < blk_n) {
-                cur = blk_lo + got;
-                cur_end = got + c < blk_n ? blk_lo + got + c : blk_hi;
-                seen = got + c;
-            } else {
-                in_pool = true;
-            }
-        }
-        if (cur == cur_end) {   // in the pool: next round(s) until this wave has a share
-            bool done = false;
-            while (cur == cur_end) {
-                __syncthreads();                       // the previous round is finished
-                if (tid == 0) *(u64*)(ctl + 2) = next_g;
-                __syncthreads();                       // the round's claim is visible
-                const u64 round_g = *(const u64*)(ctl + 2);
-                if (round_g >= n_pool) { done = true; break; }
-                if (tid == 0) next_g = atomicAdd(A.unit_counter, (u64)wpb);
-                if (round_g + wib < n_pool) {
-                    cur = n_a + round_g + wib;
-                    cur_end = cur + 1;
-                }
-            }
-            if (done) break;
-        }
-        const u64 unit = cur + A.unit_base;
-        ++cur;
-        Unit U;
-        if (!resolve(A, unit, lane, U)) continue;
-        U.a = bcast64(U.a);
-        U.a_hi = bcast64(U.a_hi);
-        U.us_lo = bcast64(U.us_lo);
-        U.us_hi = bcast64(U.us_hi);
-        U.n = bcast32(U.n);
-        U.owner = bcast32(U.owner);
-        U.j0 = bcast32(U.j0);
-
-        // phases: dense prime p held by lane p as an LDS bit address, stage-B prime
-        // (64*i + lane) held in R.s[i].  A unit that continues the previous one (same
-        // owner, next u, previous unit a whole number of row groups) keeps the dense phases
-        // where the row loop left them and advances the stage-B phases by the previous
-        // unit's length; otherwise the phases are computed from u_start.
-        if (U.owner == prev_owner && U.j0 == prev_j0 + prev_n && prev_n % (u32)(RG * kRowBits) == 0u) {
-#pragma unroll
-            for (int i = 0; i < NBV; ++i) {   // x < 4096 + 2^20: the probe's exact quotient
-                const u32 x = R.s[i] + prev_n;
-                const int qt = (int)((float)x * __uint_as_float(R.rcp[i]));
-                R.s[i] = (u32)(__mul24(qt, R.nq[i]) + (int)x);
-            }
+        u32 st_lo, st_hi, sh;
+        if (!in_pool && a_next < blk_hi) {
+            st_lo = a_next;
+            sh = shape_of(st_lo);
+            st_hi = shape_end(sh) < blk_hi ? shape_end(sh) : blk_hi;
+            a_next = st_hi;
         } else {
-            sd = 0;
-            if (lane < A.nd) sd = doff * 32u + phase(A.pmod[lane], U.a, U.a_hi, U.us_lo, U.us_hi);
-#pragma unroll
-            for (int i = 0; i < NBV; ++i) {
-                const u32 p = (u32)i * 64u + lane;
-                R.s[i] = p < A.nb ? phase(A.pmod[A.nd + p], U.a, U.a_hi, U.us_lo, U.us_hi) : 0u;
+            in_pool = true;
+            if (r_next >= r_end) {
+                __syncthreads();                       // the previous round is finished
+                if (tid == 0) ctl[2] = next_g;
+                __syncthreads();                       // the round's claim is visible
+                const u32 g = bcast32(ctl[2]);            // uniform: keep the schedule in SGPRs
+                if (g >= n_pool) break;
+                if (tid == 0) next_g = (u32)atomicAdd(A.unit_counter, (u64)wpb);
+                r_base = n_a + g;
+                r_next = r_base;
+                r_end = n_units - r_base > wpb ? r_base + wpb : n_units;
             }
+            st_lo = r_next;
+            sh = shape_of(st_lo);
+            st_hi = shape_end(sh) < r_end ? shape_end(sh) : r_end;
+            r_next = st_hi;
         }
-        prev_owner = U.owner;
-        prev_j0 = U.j0;
-        prev_n = U.n;
+        const bool reload = sh != cur_shape;
+        __syncthreads();                               // every wave is past the last step
+        if (reload) {
+            ShapeTab T;
+            if constexpr (GROUP) T = A.group[sh];
+            else T = A.tab;
+            for (u32 i = tid; i < T.lds_words; i += blockDim.x) lds[i] = T.lds_image[i];
+        }
+        if (tid == 0) ctl[0] = 0;
+        __syncthreads();                               // the tables and the cursor are ready
+        if (reload) {
+            if constexpr (GROUP) set_regs(A.group[sh]);
+            cur_shape = sh;
+            prev_owner = 0xffffffffu;
+        }
+        const u32 st_n = st_hi - st_lo;
+        u32 seen = 0;                      // phase-A cursor value this wave last drew
+        bool took = false;                 // phase B: this wave's unit of the round taken
+        u32 cur = 0, cur_end = 0;
+        for (;;) {
+            if (cur == cur_end) {
+                if (!in_pool) {
+                    const u32 rem = st_n > seen ? st_n - seen : 0;
+                    u32 c = rem / (8 * wpb);
+                    c = c < 1 ? 1 : (c > 64 ? 64 : c);
+                    u32 got = 0;
+                    if (lane == 0) got = atomicAdd(&ctl[0], c);
+                    got = bcast32(got);
+                    if (got >= st_n) break;
+                    cur = st_lo + got;
+                    cur_end = st_n - got > c ? st_lo + got + c : st_hi;
+                    seen = got + c;
+                } else {
+                    const u32 mine = r_base + wib;
+                    if (took || mine < st_lo || mine >= st_hi) break;
+                    took = true;
+                    cur = mine;
+                    cur_end = mine + 1;
+                }
+            }
+            const u64 unit = (u64)cur + A.unit_base;
+            ++cur;
+            Unit U;
+            if (!resolve(A, unit, lane, U)) continue;
+            U.a = bcast64(U.a);
+            U.a_hi = bcast64(U.a_hi);
+            U.us_lo = bcast64(U.us_lo);
+            U.us_hi = bcast64(U.us_hi);
+            U.n = bcast32(U.n);
+            U.owner = bcast32(U.owner);
+            U.j0 = bcast32(U.j0);
 
-        const u32 rows = (U.n + kRowBits - 1) / kRowBits;
-        for (u32 row0 = 0; row0 < rows; row0 += RG) {
-            u32 acc[RG];
-#pragma unroll
-            for (int r = 0; r < RG; ++r) acc[r] = 0;
-            // SPS patterns per step so their LDS reads (2*RG per pattern) are in flight together
-            constexpr int SPS = RG >= 8 ? 2 : 4;
-            u32 p = 0;
-            for (; p + SPS <= A.nd; p += SPS) {
-                u32 ba[SPS], lo[SPS][RG], hi[SPS][RG];
-#pragma unroll
-                for (int u = 0; u < SPS; ++u) {
-                    ba[u] = __builtin_amdgcn_readlane(sd, p + (u32)u);
-                    const u32* src = lds + (ba[u] >> 5) + lane;
-                    // A.one == 1: an offset the compiler cannot fold, so the lo words of
-                    // row pairs (and the hi words) load with one stride-64 instruction each
-                    const u32* srch = src + A.one;
-#pragma unroll
-                    for (int r = 0; r < RG; ++r) {
-                        lo[u][r] = src[64 * r];
-                        hi[u][r] = srch[64 * r];
-                    }
+            // phases: dense prime p held by lane p as an LDS bit address, stage-B prime
+            // (64*i + lane) held in R.s[i].  A unit that continues the previous one (same
+            // owner, next u, previous unit a whole number of row groups) keeps the dense phases
+            // where the row loop left them and advances the stage-B phases by the previous
+            // unit's length; otherwise the phases are computed from u_start.
+            if (U.owner == prev_owner && U.j0 == prev_j0 + prev_n && prev_n % (u32)(RG * kRowBits) == 0u) {
+    #pragma unroll
+                for (int i = 0; i < NBV; ++i) {   // x < 4096 + 2^20: the probe's exact quotient
+                    const u32 x = R.s[i] + prev_n;
+                    const int qt = (int)((float)x * __uint_as_float(R.rcp[i]));
+                    R.s[i] = (u32)(__mul24(qt, R.nq[i]) + (int)x);
                 }
-#pragma unroll
-                for (int u = 0; u < SPS; ++u)
-#pragma unroll
-                    for (int r = 0; r < RG; ++r) acc[r] |= __builtin_amdgcn_alignbit(hi[u][r], lo[u][r], ba[u] & 31u);
-            }
-            for (; p < A.nd; ++p) {
-                const u32 ba = __builtin_amdgcn_readlane(sd, p);
-                const u32* src = lds + (ba >> 5) + lane;
-#pragma unroll
-                for (int r = 0; r < RG; ++r) {
-                    const u32 lo = src[64 * r];
-                    const u32 hi = src[64 * r + 1];
-                    acc[r] |= __builtin_amdgcn_alignbit(hi, lo, ba & 31u);
+            } else {
+                sd = 0;
+                if (lane < t_nd()) sd = doff * 32u + phase(t_pmod()[lane], U.a, U.a_hi, U.us_lo, U.us_hi);
+    #pragma unroll
+                for (int i = 0; i < NBV; ++i) {
+                    const u32 p = (u32)i * 64u + lane;
+                    R.s[i] = p < t_nb() ? phase(t_pmod()[t_nd() + p], U.a, U.a_hi, U.us_lo, U.us_hi) : 0u;
                 }
             }
-            if (lane < A.nd) {
-                u32 s = sd - doff * 32u + dstep;
-                if (s >= dq) s -= dq;
-                sd = doff * 32u + s;
-            }
-            u32 alive[RG];
-#pragma unroll
-            for (int r = 0; r < RG; ++r) {
-                const u32 rowbase = (row0 + (u32)r) * (u32)kRowBits;
-                u32 v = ~acc[r];
-                if (rowbase + (u32)kRowBits > U.n) {   // the unit's last, partial row (uniform)
-                    const u32 jb = rowbase + lane * 32u;
-                    if (jb >= U.n) v = 0;
-                    else if (U.n - jb < 32u) v &= (1u << (U.n - jb)) - 1u;
-                }
-                alive[r] = v;
-            }
-            // survivors of the dense stage in uniform control flow: the lanes with a
-            // non-empty word (ballot), then that word's bits (readlane); stage B takes them
-            // two at a time
-            auto emit = [&](u32 j) {
-                if (lane == q_n) {
-                    q_owner = U.owner;
-                    q_j = U.j0 + j;
-                }
-                if (++q_n == 64u) flush();
-            };
-            u32 pend = 0xffffffffu;
-#pragma unroll
-            for (int r = 0; r < RG; ++r) {
-                u64 m = ballot(alive[r] != 0u);
-                while (m) {
-                    const u32 l = (u32)__builtin_ctzll(m);
-                    m &= m - 1u;
-                    u32 w = __builtin_amdgcn_readlane(alive[r], l);
-                    cnt_dense += (u32)__builtin_popcount(w);
-                    const u32 jb = (row0 + (u32)r) * (u32)kRowBits + l * 32u;
-                    while (w) {
-                        const u32 j = jb + (u32)__builtin_ctz(w);
-                        w &= w - 1u;
-                        if (pend == 0xffffffffu) {
-                            pend = j;
-                            continue;
+            prev_owner = U.owner;
+            prev_j0 = U.j0;
+            prev_n = U.n;
+
+            const u32 rows = (U.n + kRowBits - 1) / kRowBits;
+            for (u32 row0 = 0; row0 < rows; row0 += RG) {
+                u32 acc[RG];
+    #pragma unroll
+                for (int r = 0; r < RG; ++r) acc[r] = 0;
+                // SPS patterns per step so their LDS reads (2*RG per pattern) are in flight together
+                constexpr int SPS = RG >= 8 ? 2 : 4;
+                u32 p = 0;
+                for (; p + SPS <= t_nd(); p += SPS) {
+                    u32 ba[SPS], lo[SPS][RG], hi[SPS][RG];
+    #pragma unroll
+                    for (int u = 0; u < SPS; ++u) {
+                        ba[u] = __builtin_amdgcn_readlane(sd, p + (u32)u);
+                        const u32* src = lds + (ba[u] >> 5) + lane;
+                        // A.one == 1: an offset the compiler cannot fold, so the lo words of
+                        // row pairs (and the hi words) load with one stride-64 instruction each
+                        const u32* srch = src + A.one;
+    #pragma unroll
+                        for (int r = 0; r < RG; ++r) {
+                            lo[u][r] = src[64 * r];
+                            hi[u][r] = srch[64 * r];
                         }
-                        bool pa, pb;
-                        stage_b_pair<NBV>(lds, A.tierb_far, A.nb_lds, R, valid, pend, j, pa, pb);
-                        if (pa) emit(pend);
-                        if (pb) emit(j);
-                        pend = 0xffffffffu;
+                    }
+    #pragma unroll
+                    for (int u = 0; u < SPS; ++u)
+    #pragma unroll
+                        for (int r = 0; r < RG; ++r) acc[r] |= __builtin_amdgcn_alignbit(hi[u][r], lo[u][r], ba[u] & 31u);
+                }
+                for (; p < t_nd(); ++p) {
+                    const u32 ba = __builtin_amdgcn_readlane(sd, p);
+                    const u32* src = lds + (ba >> 5) + lane;
+    #pragma unroll
+                    for (int r = 0; r < RG; ++r) {
+                        const u32 lo = src[64 * r];
+                        const u32 hi = src[64 * r + 1];
+                        acc[r] |= __builtin_amdgcn_alignbit(hi, lo, ba & 31u);
                     }
                 }
+                if (lane < t_nd()) {
+                    u32 s = sd - doff * 32u + dstep;
+                    if (s >= dq) s -= dq;
+                    sd = doff * 32u + s;
+                }
+                u32 alive[RG];
+    #pragma unroll
+                for (int r = 0; r < RG; ++r) {
+                    const u32 rowbase = (row0 + (u32)r) * (u32)kRowBits;
+                    u32 v = ~acc[r];
+                    if (rowbase + (u32)kRowBits > U.n) {   // the unit's last, partial row (uniform)
+                        const u32 jb = rowbase + lane * 32u;
+                        if (jb >= U.n) v = 0;
+                        else if (U.n - jb < 32u) v &= (1u << (U.n - jb)) - 1u;
+                    }
+                    alive[r] = v;
+                }
+                // survivors of the dense stage in uniform control flow: the lanes with a
+                // non-empty word (ballot), then that word's bits (readlane); stage B takes them
+                // two at a time
+                auto emit = [&](u32 j) {
+                    if (lane == q_n) {
+                        q_owner = U.owner;
+                        q_j = U.j0 + j;
+                    }
+                    if (++q_n == 64u) flush();
+                };
+                u32 pend = 0xffffffffu;
+    #pragma unroll
+                for (int r = 0; r < RG; ++r) {
+                    u64 m = ballot(alive[r] != 0u);
+                    while (m) {
+                        const u32 l = (u32)__builtin_ctzll(m);
+                        m &= m - 1u;
+                        u32 w = __builtin_amdgcn_readlane(alive[r], l);
+                        cnt_dense += (u32)__builtin_popcount(w);
+                        const u32 jb = (row0 + (u32)r) * (u32)kRowBits + l * 32u;
+                        while (w) {
+                            const u32 j = jb + (u32)__builtin_ctz(w);
+                            w &= w - 1u;
+                            if (pend == 0xffffffffu) {
+                                pend = j;
+                                continue;
+                            }
+                            bool pa, pb;
+                            stage_b_pair<NBV>(lds, t_far(), t_nb_lds(), R, valid, pend, j, pa, pb);
+                            if (pa) emit(pend);
+                            if (pb) emit(j);
+                            pend = 0xffffffffu;
+                        }
+                    }
+                }
+                if (pend != 0xffffffffu && stage_b_pass<NBV>(lds, t_far(), t_nb_lds(), R, valid, pend)) emit(pend);
             }
-            if (pend != 0xffffffffu && stage_b_pass<NBV>(lds, A.tierb_far, A.nb_lds, R, valid, pend)) emit(pend);
         }
     }
     flush();
@@ -486,7 +558,10 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
     for (u32 e = wid; e < total; e += waves) {
         const MidRec rec = A.mid[e];
         const u32 owner = rec.owner, j = rec.j;
-        u64 ulo, uhi, a, a_hi = 0;
+        u64 ulo, uhi, a, a_hi = 0, W = A.W, W_hi = A.W_hi;
+        const DeepPrime* deep = A.deep;
+        const u32* hash = A.hash;
+        u32 nc = A.nc;
         if (A.mode == 0) {
             const ClassRec cr = A.plain.cls[owner];
             a = cr.a;
@@ -498,17 +573,25 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
             a_hi = jr.a_hi;
             ulo = jr.u0_lo;
             uhi = jr.u0_hi;
+            W = jr.W;
+            W_hi = jr.W_hi;
+            if (A.group) {
+                const DeepTab t = A.group[jr.shape];
+                deep = t.deep;
+                hash = t.hash;
+                nc = t.nc;
+            }
         }
         add128(ulo, uhi, j);
-        const U2 n = cand_n(A.W, A.W_hi, a, a_hi, ulo, uhi);
+        const U2 n = cand_n(W, W_hi, a, a_hi, ulo, uhi);
         bool dead = false;
-        for (u32 p0 = 0; p0 < A.nc; p0 += 64) {
+        for (u32 p0 = 0; p0 < nc; p0 += 64) {
             const u32 p = p0 + lane;
             bool kill = false;
-            if (p < A.nc) {
-                const DeepPrime dp = A.deep[p];
+            if (p < nc) {
+                const DeepPrime dp = deep[p];
                 const u32 r = mod128(n.lo, n.hi, dp.pm);
-                const u32* h = A.hash + dp.hash_off;
+                const u32* h = hash + dp.hash_off;
                 u32 slot = r & (kMaxHashSlots - 1);
                 for (int probe = 0; probe < kMaxHashSlots; ++probe) {
                     const u32 v = h[slot];
@@ -555,7 +638,7 @@ __device__ __forceinline__ U2 survivor_n(const ProveArgs& A, u32 idx) {
     const JobRec jr = A.jobs[s.job_idx];
     u64 ulo = jr.u0_lo, uhi = jr.u0_hi;
     add128(ulo, uhi, s.u_off);
-    return cand_n(A.W, A.W_hi, jr.a, jr.a_hi, ulo, uhi);
+    return cand_n(jr.W, jr.W_hi, jr.a, jr.a_hi, ulo, uhi);
 }
 
 // K2a: per survivor the gates of prover_worker (v13.cu:1828-1842): bit size, trial division
@@ -696,7 +779,7 @@ __global__ __launch_bounds__(256) void top_gate_kernel(TopGateArgs A) {
         const JobRec jr = A.jobs[sv.job_idx];
         u64 qlo = jr.u0_lo, qhi = jr.u0_hi;
         add128(qlo, qhi, sv.u_off);
-        U2 t = add(cand_n(A.W, A.W_hi, jr.a, jr.a_hi, qlo, qhi), mk(1, 0));   // p_0 + 1 = W*Q
+        U2 t = add(cand_n(jr.W, jr.W_hi, jr.a, jr.a_hi, qlo, qhi), mk(1, 0));   // p_0 + 1 = W*Q
         if (A.top_pos > 0) {
             t.hi = (t.hi << A.top_pos) | (t.lo >> (64 - A.top_pos));
             t.lo <<= A.top_pos;
@@ -721,7 +804,10 @@ __global__ void zero_kernel(u32* p, size_t n) {
 template <int RG, int NBV>
 static hipError_t launch_sieve_t(const SieveArgs& a, int grid, int block, size_t lds, hipStream_t s) {
     (void)hipGetLastError();   // launch errors only: drop any earlier non-launch error
-    hipLaunchKernelGGL((sieve_kernel<RG, NBV>), dim3(grid), dim3(block), lds, s, a);
+    if (a.group)
+        hipLaunchKernelGGL((sieve_kernel<RG, NBV, true>), dim3(grid), dim3(block), lds, s, a);
+    else
+        hipLaunchKernelGGL((sieve_kernel<RG, NBV, false>), dim3(grid), dim3(block), lds, s, a);
     return hipGetLastError();
 }
 

@@ -61,6 +61,7 @@ struct Lane {           // buffers of one in-flight slice for one depth group
     ccs_chain_result* h_res = nullptr;    // pinned
     uint32_t* h_count = nullptr;          // pinned
     std::vector<ccs_job> jobs;
+    std::vector<uint32_t> job_shape;      // group lanes: the depth profile of each job
     std::vector<Entry> entries;
     hipEvent_t filtered = nullptr;        // the group's filter launch is done (prove may start)
     bool busy = false;
@@ -75,7 +76,8 @@ struct ccs_server {
     std::vector<int32_t> depths;
     hipStream_t streams[2] = {nullptr, nullptr};    // per lane: filter launches
     hipStream_t pstreams[2] = {nullptr, nullptr};   // per lane: prove launches and copies
-    std::vector<Lane> lanes[2];             // [lane][group]
+    std::vector<Lane> lanes[2];             // [lane][group]; one group set per lane when grouped
+    bool grouped = false;                   // several depth profiles: one launch per slice
     std::mutex mu;
     std::condition_variable cv;
     std::deque<uint64_t> pending;
@@ -204,10 +206,11 @@ static void server_loop(ccs_server* s) {
             uint64_t quota = n_work ? s->p.slice_candidates / n_work : 0;
             quota = std::max<uint64_t>(quota, s->p.min_chunk);
             quota = std::min<uint64_t>(quota, s->p.max_chunk);
+            size_t planned = 0;
             for (uint64_t id : s->active) {
                 Job& j = s->jobs[id];
                 if (j.cursor >= j.req.count) continue;
-                if (plan[j.group].size() >= s->p.max_active) continue;
+                if (planned >= s->p.max_active) continue;   // (active jobs never exceed it)
                 const uint64_t n = std::min<uint64_t>(quota, j.req.count - j.cursor);
                 uint64_t u0lo = j.req.u0.lo + j.cursor;
                 uint64_t u0hi = j.req.u0.hi + (u0lo < j.req.u0.lo);
@@ -218,19 +221,35 @@ static void server_loop(ccs_server* s) {
                 r.count = n;
                 recs[j.group].push_back(r);
                 plan[j.group].push_back(Entry{id, j.cursor});
+                ++planned;
                 j.cursor += n;
                 j.inflight++;
                 s->stat_candidates += n;
             }
         }
-        // launch the slice on lane li
+        // launch the slice on lane li: one group launch over every depth profile, or one
+        // launch per profile
         bool launched = false;
-        for (size_t gi = 0; gi < s->shapes.size(); ++gi) {
+        if (s->grouped) {   // flatten the plan in profile order (group sets want jobs grouped)
+            for (size_t gi = 1; gi < s->shapes.size(); ++gi) {
+                plan[0].insert(plan[0].end(), plan[gi].begin(), plan[gi].end());
+                recs[0].insert(recs[0].end(), recs[gi].begin(), recs[gi].end());
+            }
+        }
+        for (size_t gi = 0; gi < s->lanes[li].size(); ++gi) {
             if (plan[gi].empty()) continue;
             Lane& L = s->lanes[li][gi];
             L.jobs = recs[gi];
             L.entries = plan[gi];
-            int rc = ccs_jobset_update(s->ctx, L.js, L.jobs.data(), (uint32_t)L.jobs.size(), s->streams[li]);
+            int rc;
+            if (s->grouped) {
+                L.job_shape.clear();
+                for (const Entry& e : L.entries) L.job_shape.push_back((uint32_t)s->jobs[e.job].group);
+                rc = ccs_groupset_update(s->ctx, L.js, L.jobs.data(), L.job_shape.data(), (uint32_t)L.jobs.size(),
+                                         s->streams[li]);
+            } else {
+                rc = ccs_jobset_update(s->ctx, L.js, L.jobs.data(), (uint32_t)L.jobs.size(), s->streams[li]);
+            }
             if (!rc) rc = ccs_jobset_launch(s->ctx, L.js, L.d_surv, s->p.max_slice_survivors, L.d_count, s->streams[li]);
             // the prove of this group runs on the lane's prove stream, so the next group's
             // filter launch does not wait behind the prove's latency-bound tail
@@ -286,19 +305,56 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
         s->depths.push_back(depths[i]);
     }
     int rc = CCS_OK;
+    // several depth profiles with one row layout: one group launch per slice
+    s->grouped = n_shapes > 1;
+    {
+        ccs_tuning t0, t;
+        rc = ccs_shape_get_tuning(shapes[0], &t0);
+        for (uint32_t i = 1; i < n_shapes && !rc; ++i) {
+            rc = ccs_shape_get_tuning(shapes[i], &t);
+            if (!rc && t.rows_per_group != t0.rows_per_group) s->grouped = false;
+        }
+    }
+    const uint64_t slice = std::max<uint64_t>(s->p.slice_candidates, s->p.min_chunk);
+    const uint64_t per = std::min<uint64_t>(std::max<uint64_t>(s->p.slice_candidates / s->p.max_active, s->p.min_chunk),
+                                            s->p.max_chunk);
+    // job table sized for max_active jobs, stage-B buffer for a whole slice of any profile
+    auto dummy_job = [&](uint64_t count) {
+        ccs_job d;
+        d.a = 0;
+        d.u0.lo = 1;
+        d.u0.hi = 0;
+        d.count = count;
+        return d;
+    };
     for (int li = 0; li < 2 && !rc; ++li) {
         if (hipStreamCreateWithFlags(&s->streams[li], hipStreamNonBlocking) != hipSuccess ||
             hipStreamCreateWithFlags(&s->pstreams[li], hipStreamNonBlocking) != hipSuccess)
             rc = CCS_ERR_HIP;
-        s->lanes[li].resize(n_shapes);
-        for (uint32_t gi = 0; gi < n_shapes && !rc; ++gi) {
+        s->lanes[li].resize(s->grouped ? 1 : n_shapes);
+        for (uint32_t gi = 0; gi < s->lanes[li].size() && !rc; ++gi) {
             Lane& L = s->lanes[li][gi];
-            // size the set for max_active jobs of the largest slice share
-            std::vector<ccs_job> dummy(s->p.max_active);
-            const uint64_t per = std::max<uint64_t>(s->p.slice_candidates / s->p.max_active, 1);
-            for (auto& d : dummy) { d.a = 5; d.u0.lo = 0; d.u0.hi = 0; d.count = std::min<uint64_t>(std::max<uint64_t>(per, s->p.min_chunk), s->p.max_chunk); }
-            dummy[0].count = std::min<uint64_t>(std::max<uint64_t>(s->p.slice_candidates, s->p.min_chunk), s->p.max_chunk);
-            rc = ccs_jobset_create(ctx, shapes[gi], dummy.data(), (uint32_t)dummy.size(), &L.js);
+            if (s->grouped) {
+                std::vector<ccs_job> dummy;
+                std::vector<uint32_t> dshape;
+                for (uint32_t k = 0; k < n_shapes; ++k)
+                    for (uint64_t left = slice; left > 0;) {
+                        const uint64_t c = std::min<uint64_t>(left, s->p.max_chunk);
+                        dummy.push_back(dummy_job(c));
+                        dshape.push_back(k);
+                        left -= c;
+                    }
+                while (dummy.size() < s->p.max_active) {
+                    dummy.push_back(dummy_job(per));
+                    dshape.push_back(n_shapes - 1);
+                }
+                rc = ccs_groupset_create(ctx, shapes, n_shapes, dummy.data(), dshape.data(), (uint32_t)dummy.size(),
+                                         &L.js);
+            } else {
+                std::vector<ccs_job> dummy(s->p.max_active, dummy_job(per));
+                dummy[0].count = std::min<uint64_t>(slice, s->p.max_chunk);
+                rc = ccs_jobset_create(ctx, shapes[gi], dummy.data(), (uint32_t)dummy.size(), &L.js);
+            }
             if (!rc) rc = ccs_dmalloc(ctx, (void**)&L.d_surv, (uint64_t)s->p.max_slice_survivors * sizeof(ccs_job_survivor));
             if (!rc) rc = ccs_dmalloc(ctx, (void**)&L.d_res, (uint64_t)s->p.max_slice_survivors * sizeof(ccs_chain_result));
             if (!rc) rc = ccs_dmalloc(ctx, (void**)&L.d_count, 64);

@@ -384,6 +384,16 @@ void ccs_jobset_free(ccs_jobset* js);
 /* replace the jobs of a set (at most as many as it was created with, within its stage-B
  * buffer): the serving loop's per-slice job table */
 int ccs_jobset_update(ccs_ctx* ctx, ccs_jobset* js, const ccs_job* h_jobs, uint32_t n_jobs, void* stream);
+/* A job set over several shapes (no reference counterpart: it replaces v16's one launch per
+ * variant, v16...cu:5611-5965, and the served form's one launch per depth profile): jobs[i]
+ * belongs to shapes[job_shape[i]], jobs grouped by shape (job_shape non-decreasing); the
+ * shapes share rows_per_group.  Launch, per-job loop, graph, prove and top gate work on it
+ * like on a one-shape set; survivors name the job, whose shape gives W. */
+int ccs_groupset_create(ccs_ctx* ctx, const ccs_shape* const* shapes, uint32_t n_shapes,
+                        const ccs_job* h_jobs, const uint32_t* job_shape, uint32_t n_jobs,
+                        ccs_jobset** out);
+int ccs_groupset_update(ccs_ctx* ctx, ccs_jobset* js, const ccs_job* h_jobs, const uint32_t* job_shape,
+                        uint32_t n_jobs, void* stream);
 /* one persistent work-stealing launch over all jobs (kernels only: graph-capturable) */
 int ccs_jobset_launch(ccs_ctx* ctx, ccs_jobset* js, ccs_job_survivor* d_out, uint32_t cap,
                       uint32_t* d_count, void* stream);

@@ -139,3 +139,56 @@ def test_jobset_unit_size_and_tuning_change(ctx, oracle):
     js.close()
     ctx.dfree(d_out)
     ctx.dfree(d_cnt)
+
+
+def test_mixed_depth_group_equals_separate_sets(ctx, oracle):
+    """Jobs of three depth profiles of one CRT shape family (the served form's groups) in
+    one group launch give each job exactly the survivors of its own one-shape set; prove
+    results equal the one-shape sets' too."""
+    shapes, jobs = [], []
+    rng = np.random.default_rng(0xD3)
+    for k, depth in enumerate((6, 9, 12)):
+        s = WL.make_shape(ctx, "c4", classes=False, depth=depth)
+        shapes.append(s)
+        jl, _ = WL.batched_jobs(n_jobs=5 + k, seed=0xE0 + k, lo_exp=12, hi_exp=19)
+        jobs += [(k, j) for j in jl]
+    say(f"[batch] group of {len(shapes)} depths, {len(jobs)} jobs")
+    gs = ccs.GroupSet(shapes, jobs)
+    cap = 1 << 18
+    d_out, d_cnt = ctx.dalloc(cap * 8), ctx.dalloc(64)
+    gs.launch(d_out, cap, d_cnt)
+    ctx.synchronize()
+    got = read_out(ctx, d_out, d_cnt)
+    want = []
+    base = 0
+    for k, s in enumerate(shapes):
+        mine = [j for kk, j in jobs if kk == k]
+        js = ccs.JobSet(s, mine)
+        js.launch(d_out, cap, d_cnt)
+        ctx.synchronize()
+        want += [(base + i, u) for i, u in read_out(ctx, d_out, d_cnt)]
+        js.close()
+        base += len(mine)
+    assert got == sorted(want) and len(want) > 50
+    # prove through the group set equals the host engine on the same n
+    gs.launch(d_out, cap, d_cnt)
+    ctx.synchronize()
+    n = np.zeros(1, np.uint32)
+    ctx.d2h(n, d_cnt)
+    d_res = ctx.dalloc(cap * 32)
+    gs.prove_launch(d_out, d_cnt, cap, d_res)
+    ctx.synchronize()
+    surv = np.zeros(int(n[0]), ccs.JOB_SURVIVOR_DTYPE)
+    ctx.d2h(surv, d_out)
+    res = np.zeros(int(n[0]), ccs.CHAIN_RESULT_DTYPE)
+    ctx.d2h(res, d_res)
+    values = []
+    for sv in surv:
+        k, (a, u0, _) = jobs[int(sv["job_idx"])]
+        values.append(a + shapes[k].W * (u0 + int(sv["u_off"])))
+    host, _ = ccs.confirm_values(values)
+    for f in ("flags", "chain_len", "back_steps", "root_lo", "root_hi"):
+        assert (res[f] == host[f]).all(), f
+    gs.close()
+    for p in (d_out, d_cnt, d_res):
+        ctx.dfree(p)

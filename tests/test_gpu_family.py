@@ -178,3 +178,62 @@ def test_family_validation(ctx):
         kw.update(bad)
         with pytest.raises(ccs.CcsError):
             ccs.FamilyShape(ctx, **kw)
+
+
+def test_pool_group_launch_matches_oracle(ctx, oracle):
+    """All ten families of a pool in one persistent launch (a group set), jobs of uneven
+    length, some families with several windows and one with an empty job: the same
+    survivors as the oracle per family, also through the per-job loop and a HIP graph."""
+    pool = ccs.pool_load(os.path.join(GOLDEN, "pools", "known_fingerprints_top10.txt"))
+    rng = random.Random(0x6A)
+    shapes = [ccs.FamilyShape(ctx, e["D"], 1, 18, 8, PRIMES) for e in pool]
+    jobs = []
+    for k, sh in enumerate(shapes):
+        qmin, qmax = sh.window(89, 92)
+        for _ in range(1 + k % 3):
+            count = 0 if (k, _) == (4, 1) else rng.randrange(1 << 12, 1 << 17)
+            jobs.append((k, (0, rng.randrange(qmin, qmax - (1 << 17)), count)))
+    want = []
+    for i, (k, (_, q0, count)) in enumerate(jobs):
+        want += [(i, int(x)) for x in oracle.filter_family(pool[k]["D"], 1, 8, PRIMES, q0, count)]
+    want.sort()
+    say(f"[family] group set: {len(shapes)} shapes, {len(jobs)} jobs, {len(want)} survivors")
+    gs = ccs.GroupSet(shapes, jobs)
+    cap = 1 << 20
+    d_out, d_cnt = ctx.dalloc(cap * 8), ctx.dalloc(64)
+
+    def read():
+        n = np.zeros(1, np.uint32)
+        ctx.d2h(n, d_cnt)
+        out = np.zeros(int(n[0]), ccs.JOB_SURVIVOR_DTYPE)
+        if len(out):
+            ctx.d2h(out, d_out)
+        return sorted(zip(out["job_idx"].tolist(), out["u_off"].tolist()))
+
+    gs.launch(d_out, cap, d_cnt)
+    ctx.synchronize()
+    assert read() == want and len(want) > 100
+    gs.launch_each(d_out, cap, d_cnt)
+    ctx.synchronize()
+    assert read() == want
+    g = gs.graph(d_out, cap, d_cnt)
+    for _ in range(3):
+        gs.graph_launch(g)
+        ctx.synchronize()
+        assert read() == want
+    # an update to the jobs of three families
+    sub = [j for j in jobs if j[0] in (2, 5, 9)]
+    gs.update(sub)
+    gs.launch(d_out, cap, d_cnt)
+    ctx.synchronize()
+    want2 = []
+    for i, (k, (_, q0, count)) in enumerate(sub):
+        want2 += [(i, int(x)) for x in oracle.filter_family(pool[k]["D"], 1, 8, PRIMES, q0, count)]
+    assert read() == sorted(want2)
+    with pytest.raises(ccs.CcsError):
+        gs.update([(5, (0, 10 ** 20, 10)), (2, (0, 10 ** 20, 10))])   # not grouped by shape
+    gs.close()
+    ctx.dfree(d_out)
+    ctx.dfree(d_cnt)
+    for sh in shapes:
+        sh.close()
