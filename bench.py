@@ -184,6 +184,63 @@ def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_c
             "slice_candidates": slice_candidates, "tuning": {str(d): t for d, t in tuned.items()}}
 
 
+def measure_families(ccs, ctx, depth=20, q_per_family=1 << 28, exp_start=1, bits=(90, 127)):
+    """The v16 Q-iteration workload: the reference's pool of the 100 most frequent immune
+    fingerprints (known_fingerprints.txt), p_0 = Q*D*2^E - 1, each family's Q window for
+    90..127-bit p_0, sieved at depth 20 by the primes 3..1009 (v16's GPU primes to 503 and
+    its CPU ext sieve 509..1009).  One group launch over all families against one launch
+    per family (v16's per-variant dispatch); the unit is (Q, V) pairs per second."""
+    import random
+    pool = ccs.pool_load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "pools",
+                                      "known_fingerprints.txt"))
+    primes = [q for q in range(3, 1010) if all(q % d for d in range(2, int(q ** 0.5) + 1))]
+    rng = random.Random(0x16)
+    shapes, jobs = [], []
+    for k, e in enumerate(pool):
+        sh = ccs.FamilyShape(ctx, e["D"], exp_start, depth, depth, primes)
+        qmin, qmax = sh.window(*bits)
+        shapes.append(sh)
+        left, q0 = q_per_family, rng.randrange(qmin, qmax - q_per_family)
+        while left:
+            c = min(left, 1 << 32)
+            jobs.append((k, (0, q0, c)))
+            q0 += c
+            left -= c
+    total = q_per_family * len(shapes)
+    cap = 1 << 22
+    d_out, d_cnt = ctx.dalloc(cap * 8), ctx.dalloc(64)
+
+    def best_of(fn, reps=3):
+        fn()
+        ctx.synchronize()
+        best = 1e30
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ctx.synchronize()
+            best = min(best, time.perf_counter() - t)
+        return best
+
+    gs = ccs.GroupSet(shapes, jobs)
+    tg = best_of(lambda: gs.launch(d_out, cap, d_cnt))
+    survivors = gs.stats(d_cnt)["survivors"]
+    sets = [ccs.JobSet(shapes[k], [j for kk, j in jobs if kk == k]) for k in range(len(shapes))]
+    tl = best_of(lambda: [js.launch(d_out, cap, d_cnt) for js in sets])
+    for js in sets:
+        js.close()
+    gs.close()
+    ctx.dfree(d_out)
+    ctx.dfree(d_cnt)
+    for sh in shapes:
+        sh.close()
+    return {"workload": f"v16 Q-iteration pool: {len(pool)} fingerprint families (known_fingerprints.txt), "
+                        f"p_0 = Q*D*2^{exp_start} - 1 in {bits[0]}..{bits[1]} bits, depth {depth}, primes 3..1009, "
+                        f"2^{q_per_family.bit_length() - 1} Q per family",
+            "families": len(pool), "pairs": total, "group_s": tg, "per_family_loop_s": tl,
+            "group_pairs_per_s": total / tg, "per_family_loop_pairs_per_s": total / tl,
+            "survivors": survivors}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -333,6 +390,8 @@ def main():
         out["batched"] = measure_batched(ccs, WL, ctx)
         log("[bench] served ...")
         out["served"] = measure_served(ccs, WL, ctx, n_jobs=args.served_jobs)
+        log("[bench] fingerprint families ...")
+        out["families"] = measure_families(ccs, ctx)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(name, t0, args.cpu_seconds)
