@@ -578,10 +578,12 @@ static int upload_tables(ccs_shape* s) {
     std::vector<u32> hash((size_t)(nc ? nc : 1) * kMaxHashSlots, 0xffffffffu);
     for (u32 i = 0; i < nc; ++i) {
         const u32 idx = nd + nb + i;
+        const PrimeMod pm = make_primemod(s->fp[idx], wfull(s));
         DeepPrime d;
-        d.pm = make_primemod(s->fp[idx], wfull(s));
-        d.hash_off = i * kMaxHashSlots;
-        d.pad[0] = d.pad[1] = d.pad[2] = 0;
+        d.q = pm.q;
+        d.c32 = pm.c32;
+        d.c64 = pm.c64;
+        d.c96 = pm.c96;
         deep[i] = d;
         for (u32 f : s->forb[idx]) {
             u32 slot = f & (kMaxHashSlots - 1);

@@ -57,10 +57,11 @@ struct TierBPrime {
 
 // Deep stage (K1c): forbidden residues of n mod q in a 64-slot open-addressing hash,
 // key r, slot (r & 63), linear probing, empty slot = 0xFFFFFFFF.
+// Deep-stage prime: 16 bytes, one dwordx4 load per lane and step (the deep kernel is bound
+// by these gathers); 1/q is rebuilt in the kernel and prime i's hash is at i * kMaxHashSlots.
 struct DeepPrime {
-    PrimeMod pm;
-    u32 hash_off;   // u32 offset into the hash table array
-    u32 pad[3];
+    u32 q;
+    u32 c32, c64, c96;   // 2^(32k) mod q
 };
 
 struct ClassRec {   // plain lattice class (base_idx, k, rho) and its residue a mod W

@@ -545,6 +545,19 @@ __device__ __forceinline__ U2 cand_n(u64 W, u64 W_hi, u64 a, u64 a_hi, u64 u_lo,
     return add(n, mk(a, a_hi));
 }
 
+// n mod q for a deep prime: limb folding with 2^(32k) mod q, then the double-reciprocal
+// reduction of mod52 with 1/q rebuilt from the f32 estimate by two Newton steps (relative
+// error below 2^-52, so the truncated quotient is within the reduction's +-2 correction)
+__device__ __forceinline__ u32 deep_mod(U2 n, const DeepPrime& d) {
+    const u64 y = (n.lo & 0xffffffffull) + (n.lo >> 32) * (u64)d.c32 + (n.hi & 0xffffffffull) * (u64)d.c64 +
+                  (n.hi >> 32) * (u64)d.c96;
+    const double q = (double)d.q;
+    double r = (double)__builtin_amdgcn_rcpf((float)d.q);
+    r = __fma_rn(r, __fma_rn(-q, r, 1.0), r);
+    r = __fma_rn(r, __fma_rn(-q, r, 1.0), r);
+    return mod52(y, d.q, r);
+}
+
 // one wave per stage-B survivor; the 64 lanes test 64 deep primes at a time, so the rare
 // survivors are not serialised over thousands of dependent per-prime steps
 __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
@@ -590,8 +603,8 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
             bool kill = false;
             if (p < nc) {
                 const DeepPrime dp = deep[p];
-                const u32 r = mod128(n.lo, n.hi, dp.pm);
-                const u32* h = hash + dp.hash_off;
+                const u32 r = deep_mod(n, dp);
+                const u32* h = hash + (size_t)p * kMaxHashSlots;
                 u32 slot = r & (kMaxHashSlots - 1);
                 for (int probe = 0; probe < kMaxHashSlots; ++probe) {
                     const u32 v = h[slot];
