@@ -364,6 +364,7 @@ struct ccs_shape {
     DeepPrime* d_deep = nullptr;
     u32* d_hash = nullptr;
     u32 lds_words = 0, nd = 0, nd_primes = 0, nb = 0, nc = 0;   // nd: dense slots
+    u32 zero_lds = 0, zero_far = 0;          // zero words of the LDS image / device bitsets
     u32 nb_lds = 0;                          // stage-B primes with their bitsets in LDS
     double est_survival_b = 1.0;             // expected fraction alive after stage B
     double est_survival_d = 1.0;             // expected fraction alive after the dense stage
@@ -442,8 +443,8 @@ static u32 tierb_in_lds(const ccs_shape* s, const ccs_tuning* t) {
     u32 words = 0;
     for (auto sl : dense_slots(s, t)) words += dense_ext_words(slot_period(s, sl), t->rows_per_group);
     const u32 bpc = t->blocks_per_cu ? t->blocks_per_cu : 1;
-    const u32 cap = s->max_lds_words - kLdsCtlWords - 1;   // the scheduler words follow the tables
-    const u32 budget = std::min(cap, std::max(s->max_lds_words / bpc, std::min(s->max_lds_words, 16384u)) - kLdsCtlWords - 1);
+    const u32 cap = s->max_lds_words - kLdsCtlWords - 2;   // the zero word and scheduler words follow
+    const u32 budget = std::min(cap, std::max(s->max_lds_words / bpc, std::min(s->max_lds_words, 16384u)) - kLdsCtlWords - 2);
     u32 n = 0;
     while (n < t->n_tierb) {
         const u32 end = std::min(n + (u32)kTierBGroup, t->n_tierb);
@@ -573,7 +574,11 @@ static int upload_tables(ccs_shape* s) {
         }
         pmod[slots.size() + i] = make_primemod(q, wfull(s));
     }
-    if (far_bits.empty()) far_bits.push_back(0);
+    // a zero word in each memory: the stage-B lanes past the last prime probe it
+    s->zero_lds = (u32)image.size();
+    image.push_back(0);
+    s->zero_far = (u32)far_bits.size();
+    far_bits.push_back(0);
     std::vector<DeepPrime> deep(nc ? nc : 1);
     std::vector<u32> hash((size_t)(nc ? nc : 1) * kMaxHashSlots, 0xffffffffu);
     for (u32 i = 0; i < nc; ++i) {
@@ -591,7 +596,7 @@ static int upload_tables(ccs_shape* s) {
             hash[(size_t)i * kMaxHashSlots + slot] = f;
         }
     }
-    if (image.empty()) image.push_back(0);
+
     // expected survival after the GPU's stage B (sizes the stage-B survivor buffer)
     double surv = 1.0;
     for (u32 i = 0; i < nd + nb; ++i) {
@@ -902,6 +907,8 @@ static ShapeTab shape_tab(const ccs_shape* s) {
     t.tierb = s->d_tierb;
     t.tierb_far = s->d_tierb_far;
     t.pmod = s->d_pmod;
+    t.zero_lds = s->zero_lds;
+    t.zero_far = s->zero_far;
     return t;
 }
 

@@ -116,6 +116,7 @@ struct ShapeTab {
     const TierBPrime* tierb;
     const u32* tierb_far;     // bitsets of stage-B primes >= nb_lds (lds_off indexes here)
     const PrimeMod* pmod;     // [nd + nb] residue constants: dense slot moduli, then tier-B primes
+    u32 zero_lds, zero_far;   // a zero word in the LDS image / the device bitsets
 };
 
 struct SieveArgs {

@@ -142,12 +142,17 @@ __device__ __forceinline__ bool resolve(const SieveArgs& A, u64 unit, u32 lane, 
 // ---------------------------------------------------------------------------
 
 // per-lane copies of the stage-B prime constants: lane l holds prime 64*i + l in slot i
+// (group 0, probed for every dense survivor, keeps q and the offset apart; the rarely
+// probed later groups pack them into one register: offset | q << 20, q < 4096)
 template <int NBV>
 struct TierBRegs {
     u32 s[NBV];      // phase of the current unit
-    int nq[NBV];     // -q
-    u32 off[NBV];    // LDS dword offset of the bitset
+    int nq0;         // group 0: -q
+    u32 off0;        // group 0: dword offset of the bitset (LDS)
+    u32 qo[NBV];     // groups >= 1: offset | q << 20 (LDS or device memory); [0] unused
     u32 rcp[NBV];    // f32 bits of the rounded-up 1/q
+    __device__ __forceinline__ int nq(int i) const { return i == 0 ? nq0 : -(int)(qo[i] >> 20); }
+    __device__ __forceinline__ u32 off(int i) const { return i == 0 ? off0 : (qo[i] & 0xfffffu); }
 };
 
 // Stage B for one candidate j (uniform across the wave): the stage-B primes are held one
@@ -162,19 +167,18 @@ template <int NBV>
 __device__ __forceinline__ bool stage_b_bit(const u32* __restrict__ mem, const TierBRegs<NBV>& R, int i, u32 j) {
     const u32 x = j + R.s[i];
     const int qt = (int)((float)x * __uint_as_float(R.rcp[i]));
-    const u32 r = (u32)(__mul24(qt, R.nq[i]) + (int)x);                       // x mod q
-    const u32 w = mem[R.off[i] + __builtin_amdgcn_ubfe(r, 5u, 11u)];
+    const u32 r = (u32)(__mul24(qt, R.nq(i)) + (int)x);                       // x mod q
+    const u32 w = mem[R.off(i) + __builtin_amdgcn_ubfe(r, 5u, 11u)];
     return __builtin_amdgcn_ubfe(w, r, 1u) != 0u;
 }
 
 // groups from..NBV-1 for one candidate (groups past the first live in device memory)
 template <int NBV>
 __device__ __forceinline__ bool stage_b_rest(const u32* __restrict__ lds, const u32* __restrict__ far,
-                                             u32 nb_lds, const TierBRegs<NBV>& R, const u64 (&valid)[NBV],
-                                             u32 j) {
+                                             u32 nb_lds, const TierBRegs<NBV>& R, u32 j) {
 #pragma unroll
     for (int i = 1; i < NBV; ++i)
-        if (ballot(stage_b_bit<NBV>((u32)i * 64u < nb_lds ? lds : far, R, i, j)) & valid[i]) return false;
+        if (ballot(stage_b_bit<NBV>((u32)i * 64u < nb_lds ? lds : far, R, i, j))) return false;
     return true;
 }
 
@@ -182,20 +186,18 @@ __device__ __forceinline__ bool stage_b_rest(const u32* __restrict__ lds, const 
 // in flight together; the rare candidates that pass the first group continue alone.
 template <int NBV>
 __device__ __forceinline__ void stage_b_pair(const u32* __restrict__ lds, const u32* __restrict__ far,
-                                             u32 nb_lds, const TierBRegs<NBV>& R, const u64 (&valid)[NBV],
-                                             u32 ja, u32 jb, bool& pa, bool& pb) {
+                                             u32 nb_lds, const TierBRegs<NBV>& R, u32 ja, u32 jb, bool& pa,
+                                             bool& pb) {
     const bool ba = stage_b_bit<NBV>(lds, R, 0, ja);
     const bool bb = stage_b_bit<NBV>(lds, R, 0, jb);
-    pa = !(ballot(ba) & valid[0]) && stage_b_rest<NBV>(lds, far, nb_lds, R, valid, ja);
-    pb = !(ballot(bb) & valid[0]) && stage_b_rest<NBV>(lds, far, nb_lds, R, valid, jb);
+    pa = !ballot(ba) && stage_b_rest<NBV>(lds, far, nb_lds, R, ja);
+    pb = !ballot(bb) && stage_b_rest<NBV>(lds, far, nb_lds, R, jb);
 }
 
 template <int NBV>
 __device__ __forceinline__ bool stage_b_pass(const u32* __restrict__ lds, const u32* __restrict__ far,
-                                             u32 nb_lds, const TierBRegs<NBV>& R, const u64 (&valid)[NBV],
-                                             u32 j) {
-    return !(ballot(stage_b_bit<NBV>(lds, R, 0, j)) & valid[0]) &&
-           stage_b_rest<NBV>(lds, far, nb_lds, R, valid, j);
+                                             u32 nb_lds, const TierBRegs<NBV>& R, u32 j) {
+    return !ballot(stage_b_bit<NBV>(lds, R, 0, j)) && stage_b_rest<NBV>(lds, far, nb_lds, R, j);
 }
 
 template <int RG, int NBV, bool GROUP>
@@ -217,8 +219,9 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
     auto t_far = [&]() -> const u32* { if constexpr (GROUP) return g_far; else return A.tab.tierb_far; };
     auto t_pmod = [&]() -> const PrimeMod* { if constexpr (GROUP) return g_pmod; else return A.tab.pmod; };
     u32 dq = 1, doff = 0, dstep = 0;
+    // lanes past the last stage-B prime hold q = 1 and point at a zero word of the group's
+    // memory, so their probe never kills and no lane mask is needed
     TierBRegs<NBV> R;
-    u64 valid[NBV];   // lanes holding a stage-B prime, per slot
     auto set_regs = [&](const ShapeTab T) {
         g_nd = T.nd;
         g_nb = T.nb;
@@ -238,13 +241,18 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
         for (int i = 0; i < NBV; ++i) {
             const u32 p = (u32)i * 64u + lane;
             TierBPrime tp;
-            tp.q = 1; tp.lds_off = 0; tp.rcp = 1.0f; tp.pad = 0;
+            tp.q = 1;
+            tp.lds_off = (i == 0 || (u32)i * 64u < T.nb_lds) ? T.zero_lds : T.zero_far;   // group 0 is in LDS
+            tp.rcp = 1.0f;
+            tp.pad = 0;
             if (p < T.nb) tp = T.tierb[p];
             R.s[i] = 0;
-            R.nq[i] = -(int)tp.q;
-            R.off[i] = tp.lds_off;
+            if (i == 0) {
+                R.nq0 = -(int)tp.q;
+                R.off0 = tp.lds_off;
+            }
+            R.qo[i] = tp.lds_off | (tp.q << 20);
             R.rcp[i] = __float_as_uint(tp.rcp);
-            valid[i] = ballot(p < T.nb);
         }
     };
     u64 cnt_dense = 0;   // candidates alive after the dense stage (= tested in stage B)
@@ -400,7 +408,7 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                 for (int i = 0; i < NBV; ++i) {   // x < 4096 + 2^20: the probe's exact quotient
                     const u32 x = R.s[i] + prev_n;
                     const int qt = (int)((float)x * __uint_as_float(R.rcp[i]));
-                    R.s[i] = (u32)(__mul24(qt, R.nq[i]) + (int)x);
+                    R.s[i] = (u32)(__mul24(qt, R.nq(i)) + (int)x);
                 }
             } else {
                 sd = 0;
@@ -498,14 +506,14 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                                 continue;
                             }
                             bool pa, pb;
-                            stage_b_pair<NBV>(lds, t_far(), t_nb_lds(), R, valid, pend, j, pa, pb);
+                            stage_b_pair<NBV>(lds, t_far(), t_nb_lds(), R, pend, j, pa, pb);
                             if (pa) emit(pend);
                             if (pb) emit(j);
                             pend = 0xffffffffu;
                         }
                     }
                 }
-                if (pend != 0xffffffffu && stage_b_pass<NBV>(lds, t_far(), t_nb_lds(), R, valid, pend)) emit(pend);
+                if (pend != 0xffffffffu && stage_b_pass<NBV>(lds, t_far(), t_nb_lds(), R, pend)) emit(pend);
             }
         }
     }
