@@ -295,24 +295,26 @@ def main():
     bits = n_start.bit_length()
     cand = shape.count_candidates(t0, tc)
     cap, pcap = 1 << 20, 1 << 16
-    # two survivor buffers: the prove of step k (stream B) overlaps the filter of step k+1
-    # (stream A), as the reference's prover threads consume one batch while the GPU
-    # filters the next
-    d_surv = [ctx.dalloc(cap * 16) for _ in range(2)]
-    d_count = [ctx.dalloc(64) for _ in range(2)]
-    d_res = [ctx.dalloc(pcap * 32) for _ in range(2)]
+    # The prove of step k (stream B) overlaps the filters of the next steps (stream A), as
+    # the reference's prover threads consume one batch while the GPU filters the next.
+    # Three survivor buffers: filter k reuses the buffer of step k-3 once that step's
+    # prove has read it, so a prove whose tail outlasts the next filter stalls nothing.
+    nbuf = 3
+    d_surv = [ctx.dalloc(cap * 16) for _ in range(nbuf)]
+    d_count = [ctx.dalloc(64) for _ in range(nbuf)]
+    d_res = [ctx.dalloc(pcap * 32) for _ in range(nbuf)]
     s_filter, s_prove = ctx.stream(), ctx.stream()
-    ev_filtered = [ctx.event() for _ in range(2)]
-    ev_proved = [ctx.event() for _ in range(2)]
+    ev_filtered = [ctx.event() for _ in range(nbuf)]
+    ev_proved = [ctx.event() for _ in range(nbuf)]
     n_step = [0]
 
     def step():
         k = n_step[0]
-        b = k % 2
-        if k >= 2:
-            ctx.wait(s_filter, ev_proved[b])      # prove k-2 has read buffer b
-        shape.filter_launch(t0, tc, d_surv[b], cap, d_count[b], stream=s_filter)
-        ctx.record(ev_filtered[b], s_filter)
+        b, sf = k % nbuf, s_filter
+        if k >= nbuf:
+            ctx.wait(sf, ev_proved[b])            # prove k-3 has read buffer b
+        shape.filter_launch(t0, tc, d_surv[b], cap, d_count[b], stream=sf)
+        ctx.record(ev_filtered[b], sf)
         ctx.wait(s_prove, ev_filtered[b])
         shape.prove_launch(t0, d_surv[b], d_count[b], pcap, d_res[b], bits=bits, stream=s_prove)
         ctx.record(ev_proved[b], s_prove)
@@ -338,7 +340,7 @@ def main():
     ctx.synchronize()
     dt = time.perf_counter() - t
     kt = ctx.profile_stop()
-    last = (n_step[0] - 1) % 2
+    last = (n_step[0] - 1) % nbuf
     n_surv = np.zeros(1, np.uint32)
     ctx.d2h(n_surv, d_count[last])
     res = np.zeros(min(int(n_surv[0]), pcap), ccs.CHAIN_RESULT_DTYPE)

@@ -50,23 +50,31 @@ extern "C" const char* ccs_last_error(void) { return g_err.c_str(); }
 // context
 // ---------------------------------------------------------------------------
 
+// Launch scratch of one stream.  Launches queued on different streams of a context may
+// run at the same time (the sieve of batch k+1 filling the tail of batch k's sieve and its
+// deep stage), so every stream a context launches on gets its own stage-B buffer, counters,
+// statistics and prove queue; launches on one stream run in order and share them.
+struct StreamScratch {
+    hipStream_t st = nullptr;
+    MidRec* mid = nullptr;
+    u32 mid_cap = 0;
+    u32* counters = nullptr;            // [0] mid_count [1] out_count [2..3] unit counter (u64)
+    unsigned long long* stats = nullptr;   // [0] dense alive [1] stage-B tested
+    void* chain = nullptr;              // prove chain queue: [0] count, [16..] indices
+    u64 chain_bytes = 0;
+};
+
 struct ccs_ctx {
     int device = 0;
     int n_cu = 256;
     u32 max_lds_words = 16384;   // LDS per workgroup, in dwords
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
-    // scratch
-    MidRec* mid = nullptr;
-    u32 mid_cap = 0;
-    u32* counters = nullptr;            // [0] mid_count [1] out_count [2..3] unit counter (u64)
-    unsigned long long* stats = nullptr;   // [0] dense alive [1] stage-B tested
+    std::vector<StreamScratch> scratch;   // [0] belongs to `stream`
     void* out = nullptr;                // host-form output staging
     u64 out_bytes = 0;
     void* res = nullptr;                // prove result staging
     u64 res_bytes = 0;
-    void* chain = nullptr;              // prove chain queue: [0] count, [16..] indices
-    u64 chain_bytes = 0;
     JobRec* jobs = nullptr;
     u64* unit_prefix = nullptr;
     u32 jobs_cap = 0;
@@ -109,6 +117,29 @@ static int ctx_check(ccs_ctx* c) {
     if (!c) return fail(CCS_ERR_INVALID, "null context");
     HIPCHK(hipSetDevice(c->device));
     return CCS_OK;
+}
+
+static void free_scratch(StreamScratch& x) {
+    (void)hipFree(x.mid);
+    (void)hipFree(x.counters);
+    (void)hipFree(x.stats);
+    (void)hipFree(x.chain);
+    x = StreamScratch();
+}
+
+// the scratch of stream st, made on the first launch on it (nullptr after an error)
+static StreamScratch* stream_scratch(ccs_ctx* c, hipStream_t st) {
+    for (auto& x : c->scratch)
+        if (x.st == st) return &x;
+    StreamScratch x;
+    x.st = st;
+    if (hipMalloc((void**)&x.counters, 64) != hipSuccess || hipMalloc((void**)&x.stats, 64) != hipSuccess) {
+        free_scratch(x);
+        fail(CCS_ERR_NOMEM, "launch counters");
+        return nullptr;
+    }
+    c->scratch.push_back(x);
+    return &c->scratch.back();
 }
 
 static u64 mod_pow(u64 b, u64 e, u64 m) {
@@ -186,10 +217,10 @@ extern "C" int ccs_ctx_create(int device, ccs_ctx** out) {
         return fail(CCS_ERR_HIP, "stream create");
     }
     for (auto& ev : c->ev) (void)hipEventCreate(&ev);
-    if (hipMalloc((void**)&c->counters, 64) != hipSuccess ||
-        hipMalloc((void**)&c->stats, 64) != hipSuccess) {
+    if (!stream_scratch(c, c->stream)) {
+        const std::string msg = g_err;
         ccs_ctx_destroy(c);
-        return fail(CCS_ERR_NOMEM, "counters");
+        return fail(CCS_ERR_NOMEM, msg);
     }
     // prove constants: small primes 2..61, trial primes 67..409
     static const u32 small[18] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61};
@@ -213,13 +244,10 @@ extern "C" int ccs_ctx_create(int device, ccs_ctx** out) {
 extern "C" void ccs_ctx_destroy(ccs_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(c->mid);
-    (void)hipFree(c->counters);
-    (void)hipFree(c->stats);
+    (void)hipDeviceSynchronize();
+    for (auto& x : c->scratch) free_scratch(x);
     (void)hipFree(c->out);
     (void)hipFree(c->res);
-    (void)hipFree(c->chain);
     (void)hipFree(c->jobs);
     (void)hipFree(c->unit_prefix);
     for (auto& ev : c->ev)
@@ -285,7 +313,14 @@ extern "C" int ccs_stream_destroy(ccs_ctx* c, void* stream) {
     int rc = ctx_check(c);
     if (rc) return rc;
     if (!stream) return fail(CCS_ERR_INVALID, "null stream");
+    if ((hipStream_t)stream == c->stream) return fail(CCS_ERR_INVALID, "the context's own stream");
     HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    for (size_t i = 1; i < c->scratch.size(); ++i)
+        if (c->scratch[i].st == (hipStream_t)stream) {
+            free_scratch(c->scratch[i]);
+            c->scratch.erase(c->scratch.begin() + (long)i);
+            break;
+        }
     HIPCHK(hipStreamDestroy((hipStream_t)stream));
     return CCS_OK;
 }
@@ -874,15 +909,17 @@ extern "C" uint64_t ccs_count_candidates(const ccs_shape* s, ccs_u128 ts, uint32
 // launches
 // ---------------------------------------------------------------------------
 
-static int ensure_mid(ccs_ctx* c, u64 want) {
+static int ensure_mid(StreamScratch* x, u64 want) {
     if (want < (1u << 16)) want = 1u << 16;
     if (want > (1u << 28)) want = 1u << 28;
-    if (c->mid_cap >= want) return CCS_OK;
-    (void)hipFree(c->mid);
-    c->mid = nullptr;
-    c->mid_cap = 0;
-    HIPCHK(hipMalloc((void**)&c->mid, want * sizeof(MidRec)));
-    c->mid_cap = (u32)want;
+    if (x->mid_cap >= want) return CCS_OK;
+    // the stream's earlier launches may still read the old buffer
+    HIPCHK(hipStreamSynchronize(x->st));
+    (void)hipFree(x->mid);
+    x->mid = nullptr;
+    x->mid_cap = 0;
+    HIPCHK(hipMalloc((void**)&x->mid, want * sizeof(MidRec)));
+    x->mid_cap = (u32)want;
     return CCS_OK;
 }
 
@@ -894,7 +931,7 @@ struct Scratch {
     unsigned long long* stats;
 };
 
-static Scratch ctx_scratch(ccs_ctx* c) { return Scratch{c->mid, c->mid_cap, c->counters, c->stats}; }
+static Scratch to_scratch(const StreamScratch* x) { return Scratch{x->mid, x->mid_cap, x->counters, x->stats}; }
 
 static ShapeTab shape_tab(const ccs_shape* s) {
     ShapeTab t;
@@ -988,9 +1025,11 @@ static int filter_launch_impl(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint3
     const u64 cpc = (jmax + s->tune.unit_j - 1) / s->tune.unit_j;
     const u64 n_units = cpc * s->classes.size();
     const u64 cand = ccs_count_candidates(s, ts, tile_count);
-    int rc = ensure_mid(c, (u64)(cand * s->est_survival_b * 2.0) + (1u << 20));
+    StreamScratch* ss = stream_scratch(c, st);
+    if (!ss) return CCS_ERR_NOMEM;
+    int rc = ensure_mid(ss, (u64)(cand * s->est_survival_b * 2.0) + (1u << 20));
     if (rc) return rc;
-    const Scratch x = ctx_scratch(c);
+    const Scratch x = to_scratch(ss);
     SieveArgs sa;
     fill_sieve_common(s, x, sa);
     sa.mode = 0;
@@ -1073,7 +1112,7 @@ extern "C" int ccs_filter(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint32_t 
     const u32 keep = std::min(n, cap);
     if (keep && h_out) HIPCHK(hipMemcpy(h_out, c->out, (size_t)keep * sizeof(ccs_survivor), hipMemcpyDeviceToHost));
     *h_count = n;
-    rc = collect_stats(c, n, ctx_scratch(c));
+    rc = collect_stats(c, n, to_scratch(stream_scratch(c, c->stream)));
     if (stats) *stats = c->last;
     if (rc) return rc;
     return n > cap ? CCS_OVERFLOW : CCS_OK;
@@ -1090,8 +1129,10 @@ extern "C" int ccs_last_stats(ccs_ctx* c, ccs_filter_stats* out) {
 // ---------------------------------------------------------------------------
 
 // point a prove launch at a chain-queue scratch buffer of at least cap entries
-static int prove_scratch(void** buf, u64* have, u32 cap, ProveArgs& a) {
-    int rc = ensure_buf(buf, have, ((u64)cap + 64) * 4);
+static int prove_scratch(void** buf, u64* have, u32 cap, ProveArgs& a, hipStream_t st) {
+    const u64 want = ((u64)cap + 64) * 4;
+    if (*have < want) HIPCHK(hipStreamSynchronize(st));   // earlier proves may still use it
+    int rc = ensure_buf(buf, have, want);
     if (rc) return rc;
     a.chain_count = (u32*)*buf;
     a.chain_queue = (u32*)*buf + 16;
@@ -1134,9 +1175,11 @@ extern "C" int ccs_prove_launch(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, con
     a.surv_count = d_count;
     a.surv_cap = cap;
     a.out = d_out;
-    rc = prove_scratch(&c->chain, &c->chain_bytes, cap, a);
-    if (rc) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    StreamScratch* ss = stream_scratch(c, st);
+    if (!ss) return CCS_ERR_NOMEM;
+    rc = prove_scratch(&ss->chain, &ss->chain_bytes, cap, a, st);
+    if (rc) return rc;
     if (c->timed) HIPCHK(hipEventRecord(c->ev[3], st));
     prof_begin(c, 2, st);
     hipError_t e = launch_prove(a, cap, st);
@@ -1161,7 +1204,9 @@ static int prove_host_common(ccs_ctx* c, ProveArgs& a, const void* h_surv, size_
     a.surv_count = d_count;
     a.surv_cap = n;
     a.out = d_res;
-    rc = prove_scratch(&c->chain, &c->chain_bytes, n, a);
+    StreamScratch* ss = stream_scratch(c, c->stream);
+    if (!ss) return CCS_ERR_NOMEM;
+    rc = prove_scratch(&ss->chain, &ss->chain_bytes, n, a, c->stream);
     if (rc) return rc;
     HIPCHK(hipEventRecord(c->ev[3], c->stream));
     hipError_t e = launch_prove(a, n, c->stream);
@@ -1874,9 +1919,9 @@ extern "C" int ccs_prove_jobset_launch(ccs_ctx* c, ccs_jobset* js, const ccs_job
     a.surv_count = d_count;
     a.surv_cap = cap;
     a.out = d_out;
-    rc = prove_scratch(&js->chain, &js->chain_bytes, cap, a);
-    if (rc) return rc;
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    rc = prove_scratch(&js->chain, &js->chain_bytes, cap, a, st);
+    if (rc) return rc;
     prof_begin(c, 2, st);
     hipError_t e = launch_prove(a, cap, st);
     if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("prove launch: ") + hipGetErrorString(e));

@@ -184,7 +184,10 @@ int ccs_synchronize(ccs_ctx* ctx);   /* waits for every stream of the device */
  *      no HIP runtime of their own: the `stream` argument of the launch entry points takes
  *      these (NULL = the context's stream).  Lets the filter of one batch overlap the
  *      prove of the previous one, as the reference overlaps its GPU filter with the CPU
- *      prover threads (v13.cu:1967-2068, :3769-3960). ---- */
+ *      prover threads (v13.cu:1967-2068, :3769-3960).  Launches on different streams of
+ *      one context may run at the same time: each stream the context launches on gets its
+ *      own launch scratch (stage-B buffer, counters, prove queue), held until
+ *      ccs_stream_destroy or ccs_ctx_destroy; launches on one stream share it in order. ---- */
 int ccs_stream_create(ccs_ctx* ctx, void** stream);
 int ccs_stream_destroy(ccs_ctx* ctx, void* stream);
 int ccs_event_create(ccs_ctx* ctx, void** event);

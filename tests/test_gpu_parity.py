@@ -381,6 +381,44 @@ def test_pipelined_streams_match_sequential(ctx, ref):
     ctx.stream_destroy(sp)
 
 
+def test_concurrent_streams_keep_their_own_scratch(ctx, ref):
+    """Filter and prove launches queued on four streams of one context with no ordering
+    between them (each stream has its own stage-B buffer, counters and prove queue in the
+    library): every batch equals its sequential run, and the synchronous host form on the
+    context's own stream still reports its statistics."""
+    say("[plain] four concurrent streams")
+    cap, pcap = 1 << 18, 1 << 14
+    ranges = [WL.reference_range(seed=0x900 + i, bits=89)[0] for i in range(4)]
+    want = []
+    for t0 in ranges:
+        surv, st = ref.filter(t0, 30000, cap=cap)
+        want.append(sorted(zip(surv["tile_idx"].tolist(), surv["base_idx"].tolist(), surv["k_offset"].tolist())))
+    streams = [ctx.stream() for _ in ranges]
+    bufs = [(ctx.dalloc(cap * 16), ctx.dalloc(64), ctx.dalloc(pcap * 32)) for _ in ranges]
+    for t0, st, (d_s, d_c, d_r) in zip(ranges, streams, bufs):
+        ref.filter_launch(t0, 30000, d_s, cap, d_c, stream=st)
+        ref.prove_launch(t0, d_s, d_c, pcap, d_r, bits=89, stream=st)
+    ctx.synchronize()
+    for k, (d_s, d_c, d_r) in enumerate(bufs):
+        n = np.zeros(1, np.uint32)
+        ctx.d2h(n, d_c)
+        surv = np.zeros(int(n[0]), ccs.SURVIVOR_DTYPE)
+        res = np.zeros(int(n[0]), ccs.CHAIN_RESULT_DTYPE)
+        if len(surv):
+            ctx.d2h(surv, d_s)
+            ctx.d2h(res, d_r)
+        got = sorted(zip(surv["tile_idx"].tolist(), surv["base_idx"].tolist(), surv["k_offset"].tolist()))
+        assert got == want[k] and len(got) > 50
+        assert sorted(res["survivor_idx"].tolist()) == list(range(len(surv)))
+    _, st = ref.filter(ranges[0], 30000, cap=cap)
+    assert st["survivors"] == len(want[0]) and st["alive_dense"] > st["alive_tierb"] > 0
+    for st in streams:
+        ctx.stream_destroy(st)
+    for b in bufs:
+        for p in b:
+            ctx.dfree(p)
+
+
 def test_host_confirm_equals_gpu_prove(ctx, ref):
     """The host confirmation engine (CPU threads) and the GPU prove kernels give the same
     result for every survivor of a reference-shape range, and the same stats."""
