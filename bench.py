@@ -244,8 +244,8 @@ def measure_families(ccs, ctx, depth=20, q_per_family=1 << 28, exp_start=1, bits
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="c2", choices=["c2", "reference", "c3"])
@@ -294,7 +294,8 @@ def main():
         workload = "v13 campaign shape (primes 37..863), 89-bit, 3337446 tiles"
     bits = n_start.bit_length()
     cand = shape.count_candidates(t0, tc)
-    cap, pcap = 1 << 20, 1 << 16
+    # prove cap 4096 (~193 survivors per step): the prove of a step is one kernel
+    cap, pcap = 1 << 20, 1 << 12
     # The prove of step k (stream B) overlaps the filters of the next steps (stream A), as
     # the reference's prover threads consume one batch while the GPU filters the next.
     # Three survivor buffers: filter k reuses the buffer of step k-3 once that step's

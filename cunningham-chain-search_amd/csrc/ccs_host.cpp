@@ -981,9 +981,9 @@ static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, 
                               u64 n_units, u32* d_count, hipStream_t st, bool zero_out = true,
                               bool zero_stats = true, u32 nb_max = 0) {
     // fill kernels rather than memset nodes, so the sequence can be graph-captured
-    if (launch_zero(x.counters, 16, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
-    if (zero_stats && launch_zero((u32*)x.stats, 16, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
-    if (zero_out && launch_zero(d_count, 1, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
+    if (launch_zero3(x.counters, 16, zero_stats ? (u32*)x.stats : nullptr, 16, zero_out ? d_count : nullptr, 1,
+                     st) != hipSuccess)
+        return fail(CCS_ERR_HIP, "zero");
     sa.n_units = n_units;
     // a group launch takes its stage-B register width from its widest shape (nb_max)
     const u32 nbm = nb_max > s->nb ? nb_max : s->nb;
@@ -1738,8 +1738,8 @@ extern "C" int ccs_jobset_launch_each(ccs_ctx* c, ccs_jobset* js, ccs_job_surviv
         js->each_valid = true;
     }
     jobset_stats_begin(c, js);
-    if (launch_zero(d_count, 1, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
-    if (launch_zero((u32*)js->x.stats, 16, st) != hipSuccess) return fail(CCS_ERR_HIP, "zero");
+    if (launch_zero3(d_count, 1, (u32*)js->x.stats, 16, nullptr, 0, st) != hipSuccess)
+        return fail(CCS_ERR_HIP, "zero");
     for (u32 i = 0; i < js->n_jobs; ++i) {
         if (!js->counts[i]) continue;
         const u64 units = (js->counts[i] + js->each_unit[i] - 1) / js->each_unit[i];

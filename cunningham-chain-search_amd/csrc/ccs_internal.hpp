@@ -21,6 +21,10 @@ constexpr int kMaxHashSlots = 64;   // forbidden-residue hash per deep prime
 constexpr int kTierBGroup = 64;     // stage-B primes tested per wave step (one per lane)
 constexpr int kMaxTierB = 512;      // stage-B primes: at most 8 groups of 64
 constexpr unsigned kLdsCtlWords = 8;   // sieve scheduler words after the LDS tables
+// prove launches over at most this many survivors run as one kernel with a wave per
+// survivor (every chain position at once); larger ones as a gate kernel (a lane per
+// survivor) and a chain kernel (a wave per prime survivor)
+constexpr unsigned kProveWaveMax = 4096;
 
 // Per-modulus constants for exact residues of u64/u128 values: y = sum limb_k*2^(32k)
 // folded with c_k = 2^(32k) mod q (< 2^52 for q < 2^18), then reduced with a double
@@ -212,6 +216,8 @@ hipError_t launch_sieve(const SieveArgs& a, int rg, int nbv, int grid, int block
 hipError_t launch_deep(const DeepArgs& a, u32 max_entries, hipStream_t s);
 hipError_t launch_prove(const ProveArgs& a, u32 max_entries, hipStream_t s);
 hipError_t launch_zero(u32* p, size_t n_words, hipStream_t s);
+// zeroes up to three ranges of at most 256 words each (null pointers are skipped)
+hipError_t launch_zero3(u32* a, u32 na, u32* b, u32 nb, u32* c, u32 nc, hipStream_t s);
 hipError_t launch_top_gate(const TopGateArgs& a, hipStream_t s);
 
 }  // namespace ccs

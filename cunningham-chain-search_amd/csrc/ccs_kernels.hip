@@ -553,17 +553,38 @@ __device__ __forceinline__ U2 cand_n(u64 W, u64 W_hi, u64 a, u64 a_hi, u64 u_lo,
     return add(n, mk(a, a_hi));
 }
 
-// n mod q for a deep prime: limb folding with 2^(32k) mod q, then the double-reciprocal
-// reduction of mod52 with 1/q rebuilt from the f32 estimate by two Newton steps (relative
-// error below 2^-52, so the truncated quotient is within the reduction's +-2 correction)
-__device__ __forceinline__ u32 deep_mod(U2 n, const DeepPrime& d) {
-    const u64 y = (n.lo & 0xffffffffull) + (n.lo >> 32) * (u64)d.c32 + (n.hi & 0xffffffffull) * (u64)d.c64 +
-                  (n.hi >> 32) * (u64)d.c96;
+// n's four 32-bit limbs as doubles (exact), converted once per candidate
+struct Limbs {
+    double l0, l1, l2, l3;
+};
+
+__device__ __forceinline__ Limbs limbs_of(U2 n) {
+    Limbs L;
+    L.l0 = (double)(u32)n.lo;
+    L.l1 = (double)(u32)(n.lo >> 32);
+    L.l2 = (double)(u32)n.hi;
+    L.l3 = (double)(u32)(n.hi >> 32);
+    return L;
+}
+
+// n mod q for a deep prime (q < 2^18), all in f64: the limb fold y = l0 + l1*c32 + l2*c64 +
+// l3*c96 (c_k = 2^(32k) mod q) is an integer below 2^52, so every FMA of it is exact;
+// 1/q is rebuilt from the f32 estimate by two Newton steps (relative error ~2^-52), so the
+// truncated quotient is floor(y/q) or one off, the FMA remainder y - qt*q is exact, and it
+// lies in [-q, 2q) before the two corrections (checked on the host for every prime below
+// 2^18, f32 estimates up to 4 ulp off, and y at the ends of its range and next to multiples
+// of q).
+__device__ __forceinline__ u32 deep_mod(const Limbs& L, const DeepPrime& d) {
     const double q = (double)d.q;
+    const double y = __fma_rn(L.l3, (double)d.c96, __fma_rn(L.l2, (double)d.c64, __fma_rn(L.l1, (double)d.c32, L.l0)));
     double r = (double)__builtin_amdgcn_rcpf((float)d.q);
     r = __fma_rn(r, __fma_rn(-q, r, 1.0), r);
     r = __fma_rn(r, __fma_rn(-q, r, 1.0), r);
-    return mod52(y, d.q, r);
+    const double qt = __builtin_trunc(y * r);
+    int rem = (int)__fma_rn(-qt, q, y);
+    rem += (rem >> 31) & (int)d.q;
+    const u32 ur = (u32)rem;
+    return ur >= d.q ? ur - d.q : ur;
 }
 
 // one wave per stage-B survivor; the 64 lanes test 64 deep primes at a time, so the rare
@@ -604,7 +625,7 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
             }
         }
         add128(ulo, uhi, j);
-        const U2 n = cand_n(W, W_hi, a, a_hi, ulo, uhi);
+        const Limbs n = limbs_of(cand_n(W, W_hi, a, a_hi, ulo, uhi));
         bool dead = false;
         for (u32 p0 = 0; p0 < nc; p0 += 64) {
             const u32 p = p0 + lane;
@@ -700,6 +721,79 @@ __global__ __launch_bounds__(128) void prove_kernel(ProveArgs A) {
     }
 }
 
+// this lane's chain value: position lane - 16 of the chain through n (lanes 0..15 the
+// backward values (v-1)/2, lanes 16..63 the forward values 2v+1); false when the value is
+// undefined (the backward walk reaches 1) or not below 2^127
+__device__ __forceinline__ bool chain_value(u32 lane, U2& v) {
+    if (lane < 16) {
+        const int k = 16 - (int)lane;          // steps back
+        for (int t = 0; t < k; ++t) {
+            if (!ge(v, mk(2, 0))) return false;
+            v = shr1(sub(v, mk(1, 0)));
+        }
+        return ge(v, mk(2, 0));
+    }
+    const int i = (int)lane - 16;              // steps forward
+    for (int t = 0; t < i; ++t) {
+        if (v.hi >> 62) return false;
+        v = add(shl1(v), mk(1, 0));
+    }
+    return true;
+}
+
+// the chain of a prime n from the lane masks of its window (pm: prime, vm: value defined,
+// okm: test supported; lane l = position l - 16), continued one value at a time where the
+// walk leaves the window
+__device__ __forceinline__ void chain_readoff(const ProveArgs& A, U2 n, u64 pm, u64 vm, u64 okm,
+                                       ccs_chain_result& res) {
+    bool ok = true;
+    // backward: consecutive primes at 1, 2, ... steps back (lanes 15, 14, ...)
+    u32 back = 0;
+    while (back < 16 && ((pm >> (15 - back)) & 1)) ++back;
+    U2 root = n;
+    for (u32 t = 0; t < back; ++t) root = shr1(sub(root, mk(1, 0)));
+    if (back == 16) {   // longer walk: continue one value at a time (cap: 65 steps)
+        for (;;) {
+            if (back >= 65) break;
+            const U2 pred = shr1(sub(root, mk(1, 0)));
+            if (!ge(pred, mk(2, 0))) break;
+            if (!test_prime(pred, A, ok)) break;
+            root = pred;
+            ++back;
+        }
+    }
+    // forward from n: consecutive primes at 0, 1, ... steps (lanes 16, 17, ...)
+    u32 fwd = 0;
+    bool over = false;
+    while (fwd < 48 && ((pm >> (16 + fwd)) & 1)) ++fwd;
+    u32 len = back + fwd;
+    if (len > 64) len = 64;
+    if (fwd < 48 && len < 64 && !((vm >> (16 + fwd)) & 1)) over = true;   // next value >= 2^127
+    if (fwd == 48 && len < 64) {   // longer chain: continue one value at a time
+        U2 cur = n;
+        for (u32 t = 0; t < 47; ++t) cur = add(shl1(cur), mk(1, 0));
+        while (len < 64) {
+            if (cur.hi >> 62) { over = true; break; }
+            const U2 nx = add(shl1(cur), mk(1, 0));
+            if (!test_prime(nx, A, ok)) break;
+            ++len;
+            cur = nx;
+        }
+    }
+    // lanes on the walk: the backward run and its first composite, n's forward run and its
+    // first composite; only their tests count for the unsupported flag
+    const u32 blo = back < 16 ? 15 - back : 0;
+    const u32 fhi = 16 + fwd < 63 ? 16 + fwd : 63;
+    const u64 walk = (fhi >= 63 ? ~0ull : ((1ull << (fhi + 1)) - 1)) & ~((1ull << blo) - 1);
+    if (back) res.flags |= CCS_F_NONROOT;
+    if (over) res.flags |= CCS_F_OVERFLOW;
+    if ((~okm & walk) || !ok) res.flags |= CCS_F_UNSUPPORTED;
+    res.root_lo = root.lo;
+    res.root_hi = root.hi;
+    res.chain_len = len;
+    res.back_steps = back;
+}
+
 // K2b: one wave per prime survivor.  Lane l tests chain position l - 16: lanes 0..15 the
 // backward values (n-1)/2, ((n-1)/2 - 1)/2, ... (16..1 steps back), lanes 16..63 the forward
 // values 2^i (n+1) - 1, i = 0..47, all at once; the backward walk and forward follow of
@@ -715,77 +809,71 @@ __global__ __launch_bounds__(256) void chain_kernel(ProveArgs A) {
         const u32 idx = A.chain_queue[e];
         const U2 n = survivor_n(A, idx);
         bool ok = true;
-        // this lane's chain value
         U2 v = n;
-        bool valid = true;        // value defined and below 2^127
-        if (lane < 16) {
-            const int k = 16 - (int)lane;          // steps back
-            for (int t = 0; t < k; ++t) {
-                if (!ge(v, mk(2, 0))) { valid = false; break; }
-                v = shr1(sub(v, mk(1, 0)));
-            }
-            if (valid && !ge(v, mk(2, 0))) valid = false;
-        } else {
-            const int i = (int)lane - 16;          // steps forward
-            for (int t = 0; t < i; ++t) {
-                if (v.hi >> 62) { valid = false; break; }
-                v = add(shl1(v), mk(1, 0));
-            }
-        }
+        const bool valid = chain_value(lane, v);
         bool prime = valid && (lane == 16 || test_prime(v, A, ok));   // n itself is prime
         const u64 pm = ballot(prime);
         const u64 vm = ballot(valid);
         const u64 okm = ballot(ok);
         if (lane != 0) continue;
         ccs_chain_result res = ((ccs_chain_result*)A.out)[idx];
-        // backward: consecutive primes at 1, 2, ... steps back (lanes 15, 14, ...)
-        u32 back = 0;
-        while (back < 16 && ((pm >> (15 - back)) & 1)) ++back;
-        U2 root = n;
-        for (u32 t = 0; t < back; ++t) root = shr1(sub(root, mk(1, 0)));
-        if (back == 16) {   // longer walk: continue one value at a time (cap: 65 steps)
-            for (;;) {
-                if (back >= 65) break;
-                const U2 pred = shr1(sub(root, mk(1, 0)));
-                if (!ge(pred, mk(2, 0))) break;
-                if (!test_prime(pred, A, ok)) break;
-                root = pred;
-                ++back;
-            }
-        }
-        // forward from n: consecutive primes at 0, 1, ... steps (lanes 16, 17, ...)
-        u32 fwd = 0;
-        bool over = false;
-        while (fwd < 48 && ((pm >> (16 + fwd)) & 1)) ++fwd;
-        u32 len = back + fwd;
-        if (len > 64) len = 64;
-        if (fwd < 48 && len < 64 && !((vm >> (16 + fwd)) & 1)) over = true;   // next value >= 2^127
-        if (fwd == 48 && len < 64) {   // longer chain: continue one value at a time
-            U2 cur = n;
-            for (u32 t = 0; t < 47; ++t) cur = add(shl1(cur), mk(1, 0));
-            while (len < 64) {
-                if (cur.hi >> 62) { over = true; break; }
-                const U2 nx = add(shl1(cur), mk(1, 0));
-                if (!test_prime(nx, A, ok)) break;
-                ++len;
-                cur = nx;
-            }
-        }
-        // lanes on the walk: the backward run and its first composite, n's forward run and its
-        // first composite; only their tests count for the unsupported flag
-        const u32 blo = back < 16 ? 15 - back : 0;
-        const u32 fhi = 16 + fwd < 63 ? 16 + fwd : 63;
-        const u64 walk = (fhi >= 63 ? ~0ull : ((1ull << (fhi + 1)) - 1)) & ~((1ull << blo) - 1);
-        if (back) res.flags |= CCS_F_NONROOT;
-        if (over) res.flags |= CCS_F_OVERFLOW;
-        if ((~okm & walk) || !ok) res.flags |= CCS_F_UNSUPPORTED;
-        res.root_lo = root.lo;
-        res.root_hi = root.hi;
-        res.chain_len = len;
-        res.back_steps = back;
+        chain_readoff(A, n, pm, vm, okm, res);
         ((ccs_chain_result*)A.out)[idx] = res;
     }
 }
+
+// K2 for few survivors, as one kernel: one wave per survivor does the gates of K2a (the
+// trial primes spread over the lanes) and, with every lane testing its chain position at
+// once (lane 16 testing n itself), the walk of K2b.  Its latency is one primality test,
+// and the prove of a batch is a single launch, so it fits beside the next filter launch.
+__global__ __launch_bounds__(1024) void prove_wave_kernel(ProveArgs A) {
+    const u32 lane = lane_id();
+    u32 total = *A.surv_count;
+    if (total > A.surv_cap) total = A.surv_cap;
+    const u32 waves = gridDim.x * (blockDim.x >> 6);
+    const u32 wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    for (u32 idx = wid; idx < total; idx += waves) {
+        const U2 n = survivor_n(A, idx);
+        ccs_chain_result res;
+        res.survivor_idx = idx;
+        res.chain_len = 0;
+        res.back_steps = 0;
+        res.flags = 0;
+        res.root_lo = 0;
+        res.root_hi = 0;
+        bool pass = false;
+        if (!A.bits || bitlen(n) == A.bits) {
+            res.flags |= CCS_F_BITS_OK;
+            const bool div = lane < 62 && mod128(n.lo, n.hi, c_trial[lane]) == 0;
+            if (!ballot(div)) {
+                res.flags |= CCS_F_TRIAL_OK;
+                if (n.hi >> 63) res.flags |= CCS_F_OVERFLOW;
+                else pass = true;
+            }
+        }
+        if (!pass) {   // uniform
+            if (lane == 0) ((ccs_chain_result*)A.out)[idx] = res;
+            continue;
+        }
+        U2 v = n;
+        const bool valid = chain_value(lane, v);
+        bool ok = true;
+        const bool prime = valid && test_prime(v, A, ok);
+        const u64 pm = ballot(prime);
+        const u64 vm = ballot(valid);
+        const u64 okm = ballot(ok);
+        if (lane != 0) continue;
+        if (!((pm >> 16) & 1)) {   // n composite: the gate's answer
+            if (!((okm >> 16) & 1)) res.flags |= CCS_F_UNSUPPORTED;
+            ((ccs_chain_result*)A.out)[idx] = res;
+            continue;
+        }
+        res.flags |= CCS_F_PRIME;
+        chain_readoff(A, n, pm, vm, okm, res);
+        ((ccs_chain_result*)A.out)[idx] = res;
+    }
+}
+
 
 // K1.5: top-position gate of family survivors (v16_k15_topmr_kernel, v16...cu:2185-2296):
 // p_top = (W*Q) << top_pos - 1 with W = D*2^E must pass strong MR to the first `reps` bases
@@ -818,6 +906,15 @@ __global__ void zero_kernel(u32* p, size_t n) {
         p[i] = 0;
 }
 
+// up to three short word ranges in one launch (a launch's counters, statistics and output
+// count: one fill kernel instead of three in front of every sieve)
+__global__ void zero3_kernel(u32* a, u32 na, u32* b, u32 nb, u32* c, u32 nc) {
+    const u32 i = threadIdx.x;
+    if (a && i < na) a[i] = 0;
+    if (b && i < nb) b[i] = 0;
+    if (c && i < nc) c[i] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -847,15 +944,23 @@ hipError_t launch_sieve(const SieveArgs& a, int rg, int nbv, int grid, int block
 
 hipError_t launch_deep(const DeepArgs& a, u32 max_entries, hipStream_t s) {
     (void)hipGetLastError();
+    // up to 65536 waves: with a few entries per wave the dispatcher keeps CUs that are
+    // shared with other work (a concurrent prove) from holding back the end of the launch
     u32 blocks = (max_entries + 3) / 4;
     if (blocks < 1) blocks = 1;
-    if (blocks > 2048) blocks = 2048;
+    if (blocks > 16384) blocks = 16384;
     hipLaunchKernelGGL(deep_kernel, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_prove(const ProveArgs& a, u32 max_entries, hipStream_t s) {
     (void)hipGetLastError();
+    if (max_entries <= kProveWaveMax) {
+        // one wave per survivor (grid-stride beyond 1024 waves), blocks of 16 waves
+        const u32 waves = max_entries < 1024u ? (max_entries ? max_entries : 1u) : 1024u;
+        hipLaunchKernelGGL(prove_wave_kernel, dim3((waves + 15) / 16), dim3(1024), 0, s, a);
+        return hipGetLastError();
+    }
     u32 blocks = (max_entries + 127) / 128;
     if (blocks < 1) blocks = 1;
     if (blocks > 8192) blocks = 8192;
@@ -884,6 +989,13 @@ hipError_t launch_zero(u32* p, size_t n_words, hipStream_t s) {
     if (blocks < 1) blocks = 1;
     if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(zero_kernel, dim3(blocks), dim3(256), 0, s, p, n_words);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero3(u32* a, u32 na, u32* b, u32 nb, u32* c, u32 nc, hipStream_t s) {
+    if (na > 256 || nb > 256 || nc > 256) return hipErrorInvalidValue;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(zero3_kernel, dim3(1), dim3(256), 0, s, a, na, b, nb, c, nc);
     return hipGetLastError();
 }
 

@@ -196,6 +196,36 @@ def test_prove_reference_survivors(ctx, oracle, ref, oref):
         assert np.any(g["flags"] & ccs.F_PRIME)
 
 
+def test_prove_both_launch_forms_equal_oracle(ctx, oracle, ref, oref):
+    """A prove launch over at most 4096 survivors runs as one kernel (a wave per survivor,
+    every chain position at once), a larger one as the gate kernel and the chain kernel:
+    the same device survivors through both forms give the oracle's results."""
+    t0, _ = WL.reference_range(seed=0x9A, bits=89)
+    o_surv, _ = oref.filter(t0, 3000, cap=1 << 20)
+    n = len(o_surv)
+    assert 100 < n <= 4096
+    surv = np.zeros(n, ccs.SURVIVOR_DTYPE)
+    surv["base_idx"], surv["k_offset"], surv["tile_idx"] = o_surv[:, 0], o_surv[:, 1], o_surv[:, 2]
+    d_s, d_c = ctx.dalloc(n * 16), ctx.dalloc(64)
+    ctx.h2d(d_s, surv)
+    ctx.h2d(d_c, np.array([n], np.uint32))
+    for bits in (89, 0):
+        want = oref.prove(t0, o_surv, bits=bits)
+        for cap in (n, 4096, 4097, 1 << 16):
+            say(f"[prove] {n} survivors, launch cap {cap}, bits={bits}")
+            d_r = ctx.dalloc(cap * 32)
+            ref.prove_launch(t0, d_s, d_c, cap, d_r, bits=bits)
+            ctx.synchronize()
+            g = np.zeros(n, ccs.CHAIN_RESULT_DTYPE)
+            ctx.d2h(g, d_r)
+            ctx.dfree(d_r)
+            assert list(g["survivor_idx"]) == list(range(n))
+            compare_prove(g, want)
+            assert np.any(g["flags"] & ccs.F_PRIME)
+    ctx.dfree(d_s)
+    ctx.dfree(d_c)
+
+
 def test_prove_mr_base2_config1(ctx, oracle):
     s = WL.make_shape(ctx, "c1")
     o = oracle_shape(oracle, "c1")
