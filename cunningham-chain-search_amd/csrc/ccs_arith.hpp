@@ -163,12 +163,22 @@ CCS_HD U2 mont_pow(const Mont& c, U2 bm, U2 e) {
     return r;
 }
 
+// 2^e in Montgomery form: the multiply by the base of each set bit is a doubling (an add)
+CCS_HD U2 mont_pow2(const Mont& c, U2 e) {
+    U2 r = c.one;
+    for (int i = bitlen(e) - 1; i >= 0; --i) {
+        r = mont_mul(c, r, r);
+        if (bit(e, i)) r = madd(c, r, r);
+    }
+    return r;
+}
+
 CCS_HDN bool strong_mr(const Mont& c, u32 a) {
     U2 d = sub(c.n, mk(1, 0));
     int s = 0;
     while (!(d.lo & 1)) { d = shr1(d); ++s; }
     const U2 one = c.one, mone = sub(c.n, c.one);
-    U2 x = mont_pow(c, mont_small(c, a), d);
+    U2 x = a == 2 ? mont_pow2(c, d) : mont_pow(c, mont_small(c, a), d);
     if (eq(x, one) || eq(x, mone)) return true;
     for (int i = 1; i < s; ++i) {
         x = mont_mul(c, x, x);

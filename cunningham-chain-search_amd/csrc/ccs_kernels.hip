@@ -721,9 +721,14 @@ __global__ __launch_bounds__(128) void prove_kernel(ProveArgs A) {
     }
 }
 
+// forward chain positions tested at once (lanes 16 .. 16 + kFwdWindow - 1): a test's
+// latency grows with the value's bit length, so the window stops at position 23 (chains
+// longer than that, from n, continue one value at a time)
+constexpr int kFwdWindow = 24;
+
 // this lane's chain value: position lane - 16 of the chain through n (lanes 0..15 the
-// backward values (v-1)/2, lanes 16..63 the forward values 2v+1); false when the value is
-// undefined (the backward walk reaches 1) or not below 2^127
+// backward values (v-1)/2, lanes 16..39 the forward values 2v+1, lanes 40..63 none); false
+// when the value is undefined (the backward walk reaches 1), not below 2^127 or not tested
 __device__ __forceinline__ bool chain_value(u32 lane, U2& v) {
     if (lane < 16) {
         const int k = 16 - (int)lane;          // steps back
@@ -734,6 +739,7 @@ __device__ __forceinline__ bool chain_value(u32 lane, U2& v) {
         return ge(v, mk(2, 0));
     }
     const int i = (int)lane - 16;              // steps forward
+    if (i >= kFwdWindow) return false;
     for (int t = 0; t < i; ++t) {
         if (v.hi >> 62) return false;
         v = add(shl1(v), mk(1, 0));
@@ -765,13 +771,13 @@ __device__ __forceinline__ void chain_readoff(const ProveArgs& A, U2 n, u64 pm, 
     // forward from n: consecutive primes at 0, 1, ... steps (lanes 16, 17, ...)
     u32 fwd = 0;
     bool over = false;
-    while (fwd < 48 && ((pm >> (16 + fwd)) & 1)) ++fwd;
+    while (fwd < (u32)kFwdWindow && ((pm >> (16 + fwd)) & 1)) ++fwd;
     u32 len = back + fwd;
     if (len > 64) len = 64;
-    if (fwd < 48 && len < 64 && !((vm >> (16 + fwd)) & 1)) over = true;   // next value >= 2^127
-    if (fwd == 48 && len < 64) {   // longer chain: continue one value at a time
+    if (fwd < (u32)kFwdWindow && len < 64 && !((vm >> (16 + fwd)) & 1)) over = true;   // next value >= 2^127
+    if (fwd == (u32)kFwdWindow && len < 64) {   // longer chain: continue one value at a time
         U2 cur = n;
-        for (u32 t = 0; t < 47; ++t) cur = add(shl1(cur), mk(1, 0));
+        for (u32 t = 0; t + 1 < (u32)kFwdWindow; ++t) cur = add(shl1(cur), mk(1, 0));
         while (len < 64) {
             if (cur.hi >> 62) { over = true; break; }
             const U2 nx = add(shl1(cur), mk(1, 0));
@@ -795,8 +801,8 @@ __device__ __forceinline__ void chain_readoff(const ProveArgs& A, U2 n, u64 pm, 
 }
 
 // K2b: one wave per prime survivor.  Lane l tests chain position l - 16: lanes 0..15 the
-// backward values (n-1)/2, ((n-1)/2 - 1)/2, ... (16..1 steps back), lanes 16..63 the forward
-// values 2^i (n+1) - 1, i = 0..47, all at once; the backward walk and forward follow of
+// backward values (n-1)/2, ((n-1)/2 - 1)/2, ... (16..1 steps back), lanes 16..39 the forward
+// values 2^i (n+1) - 1, i = 0..23, all at once; the backward walk and forward follow of
 // prover_worker (v13.cu:1849-1881) are then read off the ballot.  Walks longer than the
 // lanes cover continue sequentially, so the result is the same for any chain.
 __global__ __launch_bounds__(256) void chain_kernel(ProveArgs A) {
