@@ -260,8 +260,10 @@ def test_prove_known_chains_as_jobs(ctx, oracle):
     W = s.W
     roots = known_roots()
     root17 = 2759832934171386593519
-    members = [((root17 + 1) * 32 - 1, 17)]   # chain position 5 of the CC17: walks back
-    cases = roots + members
+    root18 = 0xb149165114f43bf4f72249
+    members = [((root17 + 1) * 32 - 1, 17),   # chain position 5 of the CC17: walks back
+               ((root18 + 1) * 2 ** 17 - 1, 18)]   # last member of a CC18: 17 steps back,
+    cases = roots + members                      # past the 16 backward lanes
     jobs = [(n % W, n // W, 1) for n, _ in cases]
     surv = np.zeros(len(jobs), ccs.JOB_SURVIVOR_DTYPE)
     surv["job_idx"] = np.arange(len(jobs))
@@ -273,6 +275,9 @@ def test_prove_known_chains_as_jobs(ctx, oracle):
     m = g[len(roots)]
     assert m["chain_len"] == 17 and m["back_steps"] == 5 and m["flags"] & ccs.F_NONROOT
     assert (int(m["root_hi"]) << 64 | int(m["root_lo"])) == root17
+    m = g[len(roots) + 1]
+    assert m["chain_len"] == 18 and m["back_steps"] == 17 and m["flags"] & ccs.F_NONROOT
+    assert (int(m["root_hi"]) << 64 | int(m["root_lo"])) == root18
     # the same numbers through the oracle
     for (n, _), r in zip(cases, g):
         f, root, ln, back = oracle.prove_one(n)
