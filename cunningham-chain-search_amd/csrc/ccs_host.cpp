@@ -155,7 +155,7 @@ static u64 mod_pow(u64 b, u64 e, u64 m) {
 // Smallest float >= (1/q)(1 + 2^-23).  For x < 2^21 the f32 product x * rcp, rounded
 // to nearest, then truncated, is exactly floor(x / q): the relative excess of rcp plus
 // the product's rounding (at most 2^-22 + 2^-24) stays below 1/x, too small to reach the
-// next integer, and it is never below x / q.  Checked for every prime < 4096 and every
+// next integer, and it is never below x / q.  Checked for every prime < 16384 and every
 // x < 2^21 by the test suite.
 static float rcp_up(u32 q) {
     const double d = (1.0 / (double)q) * (1.0 + std::ldexp(1.0, -23));
@@ -494,14 +494,22 @@ static u32 tierb_in_lds(const ccs_shape* s, const ccs_tuning* t) {
 }
 
 // Starting point measured on the 31# depth-18 workload: dense primes below 200 folded in
-// pairs up to period 2^16, eight rows per lane, stage B over the next 256 primes below
-// 4096, 2^20-candidate units, one block of 16 waves per CU.  Shrunk until the tables fit
-// the device's LDS.
+// pairs up to period 2^16, eight rows per lane, stage B over the next 512 primes below
+// 4096 (1024 below 16384 at shallow depths), 2^20-candidate units, one block of 16 waves
+// per CU.  Shrunk until the tables fit the device's LDS.
 static void default_tuning(const ccs_shape* s, ccs_tuning* t) {
     u32 nd = 0;
     while (nd < s->fp.size() && nd < 2u * (u32)kMaxDense && s->fp[nd] < 200) ++nd;
     u32 nb = 0;
-    while (nd + nb < s->fp.size() && nb < (u32)kMaxTierB && s->fp[nd + nb] < 4096) ++nb;
+    while (nd + nb < s->fp.size() && nb < (u32)kDefaultTierB && s->fp[nd + nb] < 4096) ++nb;
+    // A shallow depth sends many candidates on to the deep stage, whose test of one prime
+    // costs several times a stage-B probe: when more than 2e-6 of the candidates are
+    // expected to pass this stage B, widen it to 1024 primes (measured on the 31# depth
+    // profiles: depth 12 -13 %, depth 14 -6 %, depth 16 even, depths 18-20 +1-2 %).
+    double surv = 1.0;
+    for (u32 i = 0; i < nd + nb; ++i) surv *= 1.0 - (double)s->forb[i].size() / s->fp[i];
+    if (surv > 2e-6)
+        while (nd + nb < s->fp.size() && nb < (u32)kMaxTierB && s->fp[nd + nb] < kTierBMaxQ) ++nb;
     t->n_dense = nd;
     t->n_tierb = nb;
     t->rows_per_group = 8;
@@ -527,13 +535,13 @@ static int validate_tuning(const ccs_shape* s, ccs_tuning* t) {
     if (t->pair_max > (1u << 16)) return fail(CCS_ERR_INVALID, "pair_max > 65536");
     const auto slots = dense_slots(s, t);
     if (slots.size() > (size_t)kMaxDense) return fail(CCS_ERR_INVALID, "more than 64 dense patterns");
-    if (t->n_tierb > (u32)kMaxTierB) return fail(CCS_ERR_INVALID, "n_tierb > 512");
+    if (t->n_tierb > (u32)kMaxTierB) return fail(CCS_ERR_INVALID, "n_tierb > 1024");
     if (t->n_dense + t->n_tierb > s->fp.size())
         return fail(CCS_ERR_INVALID, "n_dense + n_tierb exceeds the filter primes");
     if (t->unit_j == 0 || t->unit_j % kRowBits || t->unit_j > (1u << 20))
         return fail(CCS_ERR_INVALID, "unit_j must be a multiple of 2048, <= 2^20");
     for (u32 i = 0; i < t->n_dense + t->n_tierb; ++i)
-        if (s->fp[i] >= 4096) return fail(CCS_ERR_INVALID, "dense/stage-B primes must be < 4096");
+        if (s->fp[i] >= kTierBMaxQ) return fail(CCS_ERR_INVALID, "dense/stage-B primes must be < 16384");
     if (t->blocks_per_cu < 1 || t->blocks_per_cu > 16) return fail(CCS_ERR_INVALID, "blocks_per_cu");
     if (t->waves_per_block == 0) t->waves_per_block = 4;
     if (t->waves_per_block != 1 && t->waves_per_block != 2 && t->waves_per_block != 4 &&
@@ -614,6 +622,9 @@ static int upload_tables(ccs_shape* s) {
     image.push_back(0);
     s->zero_far = (u32)far_bits.size();
     far_bits.push_back(0);
+    // the sieve packs a stage-B prime's bitset offset into kTierBOffBits bits beside q
+    if (far_bits.size() > (1u << kTierBOffBits) || image.size() > (1u << kTierBOffBits))
+        return fail(CCS_ERR_INVALID, "stage-B bitsets exceed 2^18 words: fewer stage-B primes");
     std::vector<DeepPrime> deep(nc ? nc : 1);
     std::vector<u32> hash((size_t)(nc ? nc : 1) * kMaxHashSlots, 0xffffffffu);
     for (u32 i = 0; i < nc; ++i) {
@@ -2087,11 +2098,12 @@ static int autotune_impl(ccs_ctx* c, ccs_shape* s, const std::function<int(float
         while (n < s->fp.size() && n < 2u * kMaxDense && s->fp[n] < lim) ++n;
         return n;
     };
-    // time one tuning; n_tierb is kept (clamped to the primes below 4096 and to LDS)
+    // time one tuning; n_tierb is kept (clamped to the primes below 16384 and to LDS)
     auto try_tuning_nb = [&](ccs_tuning t) -> int {
         if (trials >= max_trials) return CCS_OK;
         u32 avail = 0;
-        while (t.n_dense + avail < s->fp.size() && avail < (u32)kMaxTierB && s->fp[t.n_dense + avail] < 4096) ++avail;
+        while (t.n_dense + avail < s->fp.size() && avail < (u32)kMaxTierB && s->fp[t.n_dense + avail] < kTierBMaxQ)
+            ++avail;
         if (t.n_tierb > avail) t.n_tierb = avail;
         for (;;) {
             ccs_tuning tt = t;
@@ -2114,7 +2126,7 @@ static int autotune_impl(ccs_ctx* c, ccs_shape* s, const std::function<int(float
         // stage B keeps its size but follows the dense cutoff
         const u32 keep = t.n_tierb;
         t.n_tierb = 0;
-        while (t.n_dense + t.n_tierb < s->fp.size() && t.n_tierb < keep && s->fp[t.n_dense + t.n_tierb] < 4096)
+        while (t.n_dense + t.n_tierb < s->fp.size() && t.n_tierb < keep && s->fp[t.n_dense + t.n_tierb] < kTierBMaxQ)
             ++t.n_tierb;
         for (;;) {
             ccs_tuning tt = t;
@@ -2158,7 +2170,7 @@ static int autotune_impl(ccs_ctx* c, ccs_shape* s, const std::function<int(float
             t.unit_j = uj;
             if ((rc = try_tuning(t))) return rc;
         }
-        for (u32 nb : {64u, 128u, 256u, 384u, 512u}) {
+        for (u32 nb : {64u, 128u, 256u, 384u, 512u, 768u, 1024u}) {
             ccs_tuning t = best;
             t.n_tierb = nb;
             if ((rc = try_tuning_nb(t))) return rc;

@@ -19,7 +19,10 @@ constexpr int kRowBits = 2048;      // one row = 64 lanes x 32 candidates
 constexpr int kMaxDense = 64;       // dense primes held one per lane
 constexpr int kMaxHashSlots = 64;   // forbidden-residue hash per deep prime
 constexpr int kTierBGroup = 64;     // stage-B primes tested per wave step (one per lane)
-constexpr int kMaxTierB = 512;      // stage-B primes: at most 8 groups of 64
+constexpr int kMaxTierB = 1024;     // stage-B primes: at most 16 groups of 64
+constexpr int kDefaultTierB = 512;  // the default tuning's stage B (the autotuner may widen it)
+constexpr unsigned kTierBMaxQ = 16384;   // stage-B primes are below this (14 bits)
+constexpr unsigned kTierBOffBits = 18;   // bitset word offsets of stage-B primes (< 2^18)
 constexpr unsigned kLdsCtlWords = 8;   // sieve scheduler words after the LDS tables
 // prove launches over at most this many survivors run as one kernel with a wave per
 // survivor (every chain position at once); larger ones as a gate kernel (a lane per
@@ -55,7 +58,7 @@ struct TierBPrime {
     u32 q;
     u32 lds_off;    // dword offset of the q-bit bitset: in LDS for the first
                     // SieveArgs::nb_lds primes, in SieveArgs::tierb_far for the others
-    float rcp;      // 1.0f / q, for x mod q with x < 2^18
+    float rcp;      // 1/q rounded up, for x mod q with x < 2^21
     u32 pad;
 };
 

@@ -143,16 +143,18 @@ __device__ __forceinline__ bool resolve(const SieveArgs& A, u64 unit, u32 lane, 
 
 // per-lane copies of the stage-B prime constants: lane l holds prime 64*i + l in slot i
 // (group 0, probed for every dense survivor, keeps q and the offset apart; the rarely
-// probed later groups pack them into one register: offset | q << 20, q < 4096)
+// probed later groups pack them into one register: offset | q << 18, q < 2^14)
 template <int NBV>
 struct TierBRegs {
     u32 s[NBV];      // phase of the current unit
     int nq0;         // group 0: -q
     u32 off0;        // group 0: dword offset of the bitset (LDS)
-    u32 qo[NBV];     // groups >= 1: offset | q << 20 (LDS or device memory); [0] unused
+    u32 qo[NBV];     // groups >= 1: offset | q << 18 (LDS or device memory); [0] unused
     u32 rcp[NBV];    // f32 bits of the rounded-up 1/q
-    __device__ __forceinline__ int nq(int i) const { return i == 0 ? nq0 : -(int)(qo[i] >> 20); }
-    __device__ __forceinline__ u32 off(int i) const { return i == 0 ? off0 : (qo[i] & 0xfffffu); }
+    __device__ __forceinline__ int nq(int i) const { return i == 0 ? nq0 : -(int)(qo[i] >> kTierBOffBits); }
+    __device__ __forceinline__ u32 off(int i) const {
+        return i == 0 ? off0 : (qo[i] & ((1u << kTierBOffBits) - 1u));
+    }
 };
 
 // Stage B for one candidate j (uniform across the wave): the stage-B primes are held one
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                 R.nq0 = -(int)tp.q;
                 R.off0 = tp.lds_off;
             }
-            R.qo[i] = tp.lds_off | (tp.q << 20);
+            R.qo[i] = tp.lds_off | (tp.q << kTierBOffBits);
             R.rcp[i] = __float_as_uint(tp.rcp);
         }
     };
@@ -941,7 +943,10 @@ hipError_t launch_sieve(const SieveArgs& a, int rg, int nbv, int grid, int block
     if (rg == R && nbv == B) return launch_sieve_t<R, B>(a, grid, block, lds_bytes, s);
 #define CCS_SIEVE_ROW(R)                                                                      \
     CCS_SIEVE_CASE(R, 1) CCS_SIEVE_CASE(R, 2) CCS_SIEVE_CASE(R, 3) CCS_SIEVE_CASE(R, 4)      \
-    CCS_SIEVE_CASE(R, 5) CCS_SIEVE_CASE(R, 6) CCS_SIEVE_CASE(R, 7) CCS_SIEVE_CASE(R, 8)
+    CCS_SIEVE_CASE(R, 5) CCS_SIEVE_CASE(R, 6) CCS_SIEVE_CASE(R, 7) CCS_SIEVE_CASE(R, 8)      \
+    CCS_SIEVE_CASE(R, 12) CCS_SIEVE_CASE(R, 16)
+    // more than 8 stage-B groups: the next instance up (lanes past the last prime are padding)
+    if (nbv > 8) nbv = nbv <= 12 ? 12 : 16;
     CCS_SIEVE_ROW(1) CCS_SIEVE_ROW(2) CCS_SIEVE_ROW(4) CCS_SIEVE_ROW(8)
 #undef CCS_SIEVE_ROW
 #undef CCS_SIEVE_CASE

@@ -163,6 +163,55 @@ def test_deep_stage_with_survivors(ctx, oracle):
     assert len(g) > 100
 
 
+@pytest.mark.parametrize("tuning", [dict(n_tierb=1024), dict(n_tierb=700, rows_per_group=4),
+                                    dict(n_dense=20, n_tierb=1000)])
+def test_wide_stage_b_vs_oracle(ctx, oracle, tuning):
+    """Stage B over up to 1024 primes (16 groups of 64, primes up to ~8700, most bitsets in
+    device memory; 700 primes run on the 12-group kernel with padding lanes): at depth 12
+    the stage counters and the survivors equal the oracle's."""
+    s = WL.make_shape(ctx, "c2", depth=12)
+    o = oracle_shape(oracle, "c2", depth=12)
+    t0, _, _ = WL.c2_range(seed=0x12B)
+    g, og, st = check_plain(ctx, oracle, "c2", t0, 1000, shape=s, oshape=o, **tuning)
+    assert st["tierb_end"] - st["dense_end"] == tuning["n_tierb"]
+    assert len(g) > 20 and st["alive_tierb"] > st["survivors"]
+
+
+def test_wide_stage_b_jobs_and_group_vs_oracle(ctx, oracle):
+    """Job sets with a 1024-prime stage B, alone and in a group launch beside a shape with
+    the default stage B (the group runs the 16-group kernel for both), equal the oracle."""
+    jobs, W = WL.batched_jobs(n_jobs=12, seed=0xB16, lo_exp=18, hi_exp=22)
+    wide = WL.make_shape(ctx, "c4", classes=False, depth=11, target=18)
+    wide.set_tuning(n_tierb=1024)
+    narrow = WL.make_shape(ctx, "c4", classes=False, depth=13, target=18)
+    want = {}
+    for k, (sh, depth) in enumerate(((wide, 11), (narrow, 13))):
+        o = oracle_shape(oracle, "c4", depth=depth, target=18)
+        want[k] = []
+        for ji, (a, u0, count) in enumerate(jobs):
+            js, _ = o.filter_class(a, W, u0, count)
+            want[k] += [(ji, int(x)) for x in js]
+    say(f"[batch] wide stage B: {len(want[0])} and {len(want[1])} survivors")
+    g, st = wide.batch(jobs, cap=1 << 20)
+    assert sorted(zip(g["job_idx"].tolist(), g["u_off"].tolist())) == sorted(want[0])
+    assert st["tierb_end"] - st["dense_end"] == 1024
+    gs = ccs.GroupSet([wide, narrow], [(0, j) for j in jobs] + [(1, j) for j in jobs])
+    cap = 1 << 20
+    d_out, d_cnt = ctx.dalloc(cap * 8), ctx.dalloc(64)
+    gs.launch(d_out, cap, d_cnt)
+    ctx.synchronize()
+    n = np.zeros(1, np.uint32)
+    ctx.d2h(n, d_cnt)
+    out = np.zeros(int(n[0]), ccs.JOB_SURVIVOR_DTYPE)
+    ctx.d2h(out, d_out)
+    got = sorted(zip(out["job_idx"].tolist(), out["u_off"].tolist()))
+    assert got == sorted(want[0] + [(len(jobs) + ji, u) for ji, u in want[1]])
+    assert len(want[0]) > 100 and len(want[1]) > 10
+    gs.close()
+    ctx.dfree(d_out)
+    ctx.dfree(d_cnt)
+
+
 def test_config3_shape_counters(ctx, oracle):
     s = WL.make_shape(ctx, "c3")
     o = oracle_shape(oracle, "c3")

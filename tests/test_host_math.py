@@ -1,6 +1,6 @@
 """Arithmetic facts the kernels rely on, checked exhaustively on the CPU.
 
-Stage B reduces x = j + s (x < 2^21) modulo a prime q < 4096 with one f32 multiply by a
+Stage B reduces x = j + s (x < 2^21) modulo a prime q < 16384 with one f32 multiply by a
 reciprocal rounded up to the smallest float >= (1/q)(1 + 2^-23) and a truncation, with no
 correction step.  numpy's float32 multiply rounds to nearest like the GPU's v_mul_f32.
 
@@ -29,7 +29,7 @@ def test_rounded_up_reciprocal_gives_exact_quotients(oracle):
     O = oracle
     x = np.arange(0, 1 << 21, dtype=np.uint32)
     xf = x.astype(np.float32)
-    for q in (int(v) for v in O.primes_between(3, 4096)):
+    for q in (int(v) for v in O.primes_between(3, 16384)):
         qt = (xf * rcp_up(q)).astype(np.uint32)
         assert np.array_equal(qt, x // q), q
 
