@@ -493,13 +493,21 @@ static u32 tierb_in_lds(const ccs_shape* s, const ccs_tuning* t) {
     return n;
 }
 
-// Starting point measured on the 31# depth-18 workload: dense primes below 200 folded in
-// pairs up to period 2^16, eight rows per lane, stage B over the next 512 primes below
-// 4096 (1024 below 16384 at shallow depths), 2^20-candidate units, one block of 16 waves
-// per CU.  Shrunk until the tables fit the device's LDS.
+// Starting point measured on the 31# depth-18 workload: dense primes below 200 (320 at
+// shallow depths) folded in pairs up to period 2^16, eight rows per lane, stage B over the
+// next 512 primes below 4096 (1024 below 16384 at shallow depths), 2^20-candidate units,
+// one block of 16 waves per CU.  Shrunk until the tables fit the device's LDS.
 static void default_tuning(const ccs_shape* s, ccs_tuning* t) {
     u32 nd = 0;
     while (nd < s->fp.size() && nd < 2u * (u32)kMaxDense && s->fp[nd] < 200) ++nd;
+    // At a shallow depth many candidates survive the dense primes, and stage B's cost per
+    // survivor dwarfs a dense pattern's per candidate: when more than 0.25 % survive the
+    // primes below 200, the dense stage takes the primes below 320 as well (measured on
+    // the 31# profiles: depths 12 and 14 -24 %, depth 16 better with 200).
+    double sd = 1.0;
+    for (u32 i = 0; i < nd; ++i) sd *= 1.0 - (double)s->forb[i].size() / s->fp[i];
+    if (sd > 2.5e-3)
+        while (nd < s->fp.size() && nd < 2u * (u32)kMaxDense && s->fp[nd] < 320) ++nd;
     u32 nb = 0;
     while (nd + nb < s->fp.size() && nb < (u32)kDefaultTierB && s->fp[nd + nb] < 4096) ++nb;
     // A shallow depth sends many candidates on to the deep stage, whose test of one prime
