@@ -834,7 +834,7 @@ __global__ __launch_bounds__(256) void chain_kernel(ProveArgs A) {
 // trial primes spread over the lanes) and, with every lane testing its chain position at
 // once (lane 16 testing n itself), the walk of K2b.  Its latency is one primality test,
 // and the prove of a batch is a single launch, so it fits beside the next filter launch.
-__global__ __launch_bounds__(1024) void prove_wave_kernel(ProveArgs A) {
+__global__ __launch_bounds__(256) void prove_wave_kernel(ProveArgs A) {
     const u32 lane = lane_id();
     u32 total = *A.surv_count;
     if (total > A.surv_cap) total = A.surv_cap;
@@ -967,9 +967,12 @@ hipError_t launch_deep(const DeepArgs& a, u32 max_entries, hipStream_t s) {
 hipError_t launch_prove(const ProveArgs& a, u32 max_entries, hipStream_t s) {
     (void)hipGetLastError();
     if (max_entries <= kProveWaveMax) {
-        // one wave per survivor (grid-stride beyond 1024 waves), blocks of 16 waves
+        // one wave per survivor (grid-stride beyond 1024 waves), blocks of 4 waves: beside
+        // a running filter launch, a CU that holds prove waves slows what else runs there
+        // (the deep stage), so they are spread thin (16-wave blocks: 5.64 ms per config-2
+        // step, 4-wave blocks: 5.46 ms)
         const u32 waves = max_entries < 1024u ? (max_entries ? max_entries : 1u) : 1024u;
-        hipLaunchKernelGGL(prove_wave_kernel, dim3((waves + 15) / 16), dim3(1024), 0, s, a);
+        hipLaunchKernelGGL(prove_wave_kernel, dim3((waves + 3) / 4), dim3(256), 0, s, a);
         return hipGetLastError();
     }
     u32 blocks = (max_entries + 127) / 128;
