@@ -27,6 +27,11 @@ PUBLISHED_GPU_CAND_PER_S = 65e9
 # 524308 KiB for 1 GiB fetched (x2.0), WRITE_SIZE 1048576 KiB for 1 GiB written (x1.0).
 SIEVE_TRAFFIC = {"c2": {"fetch_kib": 2606.25, "write_kib": 377.45, "fetch_scale": 2.0, "write_scale": 1.0,
                         "profile": "profiles/sieve_kernel_cc18_31primorial_pmc.md"}}
+# hardware utilization of the sieve kernel from the same counter profile (SQ counters over
+# 256 CUs x kernel cycles): wave64 VALU issue, LDS port bytes against 128 B/clk per CU,
+# LDS bank-conflict cycles
+SIEVE_ISSUE = {"c2": {"valu_issue_frac": 0.56, "lds_port_frac": 0.74, "lds_bank_conflict_frac": 0.063,
+                      "profile": "profiles/sieve_kernel_cc18_31primorial_pmc.md"}}
 
 
 def log(*a):
@@ -381,7 +386,7 @@ def main():
                      "unit": "Tops/s (int32 VALU)", "frac": achieved / WL.INT32_PEAK_TOPS,
                      "traffic": traffic, "kernel": "sieve_kernel",
                      "kernel_ms": ms_sieve, "e_tests_per_candidate": e_tests,
-                     "ops_per_launch": ops},
+                     "ops_per_launch": ops, "hardware": SIEVE_ISSUE.get(name)},
         "kernels_ms": {"sieve": ms_sieve, "deep": kt["ms_deep"] / max(kt["n_deep"], 1),
                        "prove": kt["ms_prove"] / max(kt["n_prove"], 1)},
         "survivors_per_step": int(n_surv[0]),
