@@ -155,6 +155,33 @@ def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_c
     while srv.poll():
         pass
     srv.shutdown()
+    # the plain comparator: the same calibration jobs with one filter launch per job (each
+    # with the unit size it would have alone; jobs above 2^32 candidates, the most one
+    # launch takes per job, in pieces of 2^32), then one prove launch per depth
+    cap = 1 << 22
+    d_out, d_cnt, d_res = ctx.dalloc(cap * 8), ctx.dalloc(64), ctx.dalloc(cap * 32)
+    by_depth = {}
+    for r in cal:
+        u0, left = r["u0"], r["count"]
+        while left > 0:
+            n = min(left, 1 << 32)
+            by_depth.setdefault(r["depth"], []).append((r["a"], u0, n))
+            u0 += n
+            left -= n
+    plain_s = 0.0
+    for d, jl in by_depth.items():
+        js = ccs.JobSet(shapes[d], jl)
+        js.launch_each(d_out, cap, d_cnt)      # warm-up (and the per-job unit sizes)
+        ctx.synchronize()
+        t = time.perf_counter()
+        js.launch_each(d_out, cap, d_cnt)
+        js.prove_launch(d_out, d_cnt, cap, d_res)
+        ctx.synchronize()
+        plain_s += time.perf_counter() - t
+        js.close()
+    for ptr in (d_out, d_cnt, d_res):
+        ctx.dfree(ptr)
+    plain = sum(r["count"] for r in cal) / plain_s
     srv = ccs.Server(ctx, shapes, slice_candidates=slice_candidates)
     reqs = WL.served_requests(n_jobs=n_jobs)
     mean = sum(r["count"] for r in reqs) / len(reqs)
@@ -183,6 +210,7 @@ def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_c
     return {"workload": "config 5: Poisson stream, depth U{12..20}, width U{64,80,96}, classes mod 31#, "
                         "lengths log-uniform in [2^24, 2^34]",
             "jobs": len(done), "load": load, "arrival_rate_per_s": lam, "capacity_candidates_per_s": capacity,
+            "plain_loop_candidates_per_s": plain, "speedup_capacity_vs_plain": capacity / plain,
             "aggregate_candidates_per_s": total / wall, "wall_s": wall,
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_ms": float(lat.mean() * 1e3), "slices": st["slices"], "launches": st["launches"],
