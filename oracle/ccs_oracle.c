@@ -1,6 +1,6 @@
 /*
  * ccs_oracle.c — CPU restatement of the reference's Cunningham-chain (first kind)
- * candidate pipeline.  TEST INFRASTRUCTURE ONLY: this file is the checker that the
+ * candidate pipeline.  TEST INFRASTRUCTURE ONLY: this file is the CPU model that the
  * tests, the smoke check and the CPU-baseline leg of the benchmark compare the HIP
  * product against.  Nothing in the product links, loads or calls it.
  *

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/ccs_oracle.c) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/ccs_oracle.c) before the GPU tests trust it as their model.
 
 Every check below comes from outside the oracle: the reference's own self-test vectors
 and data sample (tests/golden/), counts the reference prints, the mathematics of the
