@@ -388,7 +388,11 @@ int main(int argc, char** argv) {
     void *s_filter = nullptr, *s_prove = nullptr;
     CHECK(ccs_stream_create(ctx, &s_filter));
     CHECK(ccs_stream_create(ctx, &s_prove));
-    Slot slots[2];
+    // three batches in flight: the host collects batch k-3 (its prove long finished) before
+    // it launches batch k, so a prove that runs in the next filter's tail never holds the
+    // host back from queueing the filter after it
+    constexpr int kSlots = 3;
+    Slot slots[kSlots];
     for (auto& sl : slots) {
         CHECK(ccs_dmalloc(ctx, (void**)&sl.d_surv, (uint64_t)surv_cap * sizeof(ccs_survivor)));
         CHECK(ccs_dmalloc(ctx, (void**)&sl.d_count, 64));
@@ -492,8 +496,8 @@ int main(int argc, char** argv) {
             break;
         }
         CHECK(rc);
-        const int b = k & 1;
-        if (flight.size() == 2 && collect()) return 1;          // slot b's previous batch
+        const int b = k % kSlots;
+        if (flight.size() == (size_t)kSlots && collect()) return 1;   // slot b's previous batch
         Slot& sl = slots[b];
         CHECK(ccs_filter_launch(ctx, shape, ts, tc, sl.d_surv, surv_cap, sl.d_count, s_filter));
         CHECK(ccs_event_record(ctx, sl.ev_filtered, s_filter));
@@ -502,8 +506,8 @@ int main(int argc, char** argv) {
             CHECK(ccs_prove_launch(ctx, shape, ts, sl.d_surv, sl.d_count, res_cap, &pp, sl.d_res, s_prove));
             CHECK(ccs_event_record(ctx, sl.ev_proved, s_prove));
         }
-        // slot b is reused by batch k+2 only after collect() has waited for this batch's
-        // prove on the host (at most two batches are in flight)
+        // slot b is reused by batch k+3 only after collect() has waited for this batch's
+        // prove on the host (at most three batches are in flight)
         flight.push_back({b, ts, tc});
         ++k;
         if (o.cpu_prove && flight.size() == 2 && collect()) return 1;   // host proves while the GPU filters
