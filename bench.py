@@ -327,8 +327,19 @@ def main():
         workload = "v13 campaign shape (primes 37..863), 89-bit, 3337446 tiles"
     bits = n_start.bit_length()
     cand = shape.count_candidates(t0, tc)
-    # prove cap 4096 (~193 survivors per step): the prove of a step is one kernel
+    # algorithmic work: E[residue tests] per candidate under ascending-prime early exit
+    # (also the analytic survival, which sizes the prove)
+    fs = []
+    for q in shape.filter_primes:
+        lo = 0 if q <= shape.pos0_max_q else 1
+        fs.append((q, len(WL.forbidden(q, lo, shape.depth))))
+    e_tests, surv_frac = WL.expected_tests(fs)
+    # prove cap 4096 when the step's survivors fit (config 2: ~190), so the prove of a step
+    # is one kernel; the v13 campaign shape (~8·10^5 per 2^36) gets room for all of them
+    expect = cand * surv_frac
     cap, pcap = 1 << 20, 1 << 12
+    if 3 * expect > pcap:
+        pcap = min(1 << 21, 1 << int(3 * expect).bit_length())
     # The prove of step k (stream B) overlaps the filters of the next steps (stream A), as
     # the reference's prover threads consume one batch while the GPU filters the next.
     # Three survivor buffers: filter k reuses the buffer of step k-3 once that step's
@@ -382,13 +393,7 @@ def main():
         ctx.d2h(res, d_res[last])
     value = cand * args.steps / dt
     ms_sieve = kt["ms_sieve"] / max(kt["n_sieve"], 1)
-    # algorithmic work: E[residue tests] per candidate under ascending-prime early exit,
-    # 4 int32 VALU ops per test, against the int32 VALU peak
-    fs = []
-    for q in shape.filter_primes:
-        lo = 0 if q <= shape.pos0_max_q else 1
-        fs.append((q, len(WL.forbidden(q, lo, shape.depth))))
-    e_tests, surv_frac = WL.expected_tests(fs)
+    # 4 int32 VALU ops per residue test, against the int32 VALU peak
     tr = SIEVE_TRAFFIC.get(name)
     traffic = None
     if tr and tr["fetch_kib"] is not None:

@@ -599,6 +599,31 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
     if (A.stats && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.stats[2], (unsigned long long)all);
     const u32 waves = gridDim.x * (blockDim.x >> 6);
     const u32 wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    u32 q_n = 0, q_owner = 0, q_j = 0;
+    auto flush = [&]() {
+        if (!q_n) return;
+        u32 base = 0;
+        if (lane == 0) base = atomicAdd(A.out_count, q_n);
+        base = bcast32(base);
+        if (lane < q_n && base + lane < A.out_cap) {
+            const u32 slot = base + lane;
+            if (A.mode == 0) {
+                const ClassRec cr = A.plain.cls[q_owner];
+                ccs_survivor s;
+                s.base_idx = cr.base_idx;
+                s.k_offset = cr.k;
+                s.tile_idx = 6u * q_j + cr.rho - A.plain.off;
+                s._pad = 0;
+                ((ccs_survivor*)A.out)[slot] = s;
+            } else {
+                ccs_job_survivor s;
+                s.job_idx = q_owner;
+                s.u_off = q_j;
+                ((ccs_job_survivor*)A.out)[slot] = s;
+            }
+        }
+        q_n = 0;
+    };
     for (u32 e = wid; e < total; e += waves) {
         const MidRec rec = A.mid[e];
         const u32 owner = rec.owner, j = rec.j;
@@ -646,24 +671,17 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
             }
             if (ballot(kill)) { dead = true; break; }
         }
-        if (dead || lane != 0) continue;
-        const u32 slot = atomicAdd(A.out_count, 1u);
-        if (slot >= A.out_cap) continue;
-        if (A.mode == 0) {
-            const ClassRec cr = A.plain.cls[owner];
-            ccs_survivor s;
-            s.base_idx = cr.base_idx;
-            s.k_offset = cr.k;
-            s.tile_idx = 6u * j + cr.rho - A.plain.off;
-            s._pad = 0;
-            ((ccs_survivor*)A.out)[slot] = s;
-        } else {
-            ccs_job_survivor s;
-            s.job_idx = owner;
-            s.u_off = j;
-            ((ccs_job_survivor*)A.out)[slot] = s;
+        if (dead) continue;
+        // survivors wait in registers (lane k holds the k-th) and go out 64 per atomic:
+        // shallow profiles keep 10^5..10^6 survivors per launch, and one atomic each on the
+        // shared count would serialise at ~12 ns apiece
+        if (lane == q_n) {
+            q_owner = owner;
+            q_j = j;
         }
+        if (++q_n == 64u) flush();
     }
+    flush();
 }
 
 // ---------------------------------------------------------------------------
@@ -802,30 +820,48 @@ __device__ __forceinline__ void chain_readoff(const ProveArgs& A, U2 n, u64 pm, 
     res.back_steps = back;
 }
 
-// K2b: one wave per prime survivor.  Lane l tests chain position l - 16: lanes 0..15 the
-// backward values (n-1)/2, ((n-1)/2 - 1)/2, ... (16..1 steps back), lanes 16..39 the forward
-// values 2^i (n+1) - 1, i = 0..23, all at once; the backward walk and forward follow of
-// prover_worker (v13.cu:1849-1881) are then read off the ballot.  Walks longer than the
-// lanes cover continue sequentially, so the result is the same for any chain.
+// K2b for large batches: one lane per prime survivor walks its chain one value at a time
+// (backward walk to the true root, then forward follow from n), stopping at the first
+// composite on each side, as prover_worker does (v13.cu:1849-1881).  Most chains end
+// within a few values, so this does a few tests per survivor where a 64-lane window does
+// 40; the lanes of a wave diverge only in how far their chains go.
 __global__ __launch_bounds__(256) void chain_kernel(ProveArgs A) {
-    const u32 lane = lane_id();
     u32 total = *A.chain_count;
     if (total > A.surv_cap) total = A.surv_cap;
-    const u32 waves = gridDim.x * (blockDim.x >> 6);
-    const u32 wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    for (u32 e = wid; e < total; e += waves) {
+    const u32 stride = gridDim.x * blockDim.x;
+    for (u32 e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
         const u32 idx = A.chain_queue[e];
         const U2 n = survivor_n(A, idx);
-        bool ok = true;
-        U2 v = n;
-        const bool valid = chain_value(lane, v);
-        bool prime = valid && (lane == 16 || test_prime(v, A, ok));   // n itself is prime
-        const u64 pm = ballot(prime);
-        const u64 vm = ballot(valid);
-        const u64 okm = ballot(ok);
-        if (lane != 0) continue;
         ccs_chain_result res = ((ccs_chain_result*)A.out)[idx];
-        chain_readoff(A, n, pm, vm, okm, res);
+        bool ok = true;
+        // backward: consecutive primes at 1, 2, ... steps back (at most 65)
+        u32 back = 0;
+        U2 root = n;
+        while (back < 65) {
+            const U2 pred = shr1(sub(root, mk(1, 0)));
+            if (!ge(pred, mk(2, 0))) break;
+            if (!test_prime(pred, A, ok)) break;
+            root = pred;
+            ++back;
+        }
+        // forward from n (prime): consecutive primes at 1, 2, ... steps, length <= 64
+        u32 len = back + 1 > 64 ? 64 : back + 1;
+        bool over = false;
+        U2 cur = n;
+        while (len < 64) {
+            if (cur.hi >> 62) { over = true; break; }   // the next value would reach 2^127
+            const U2 nx = add(shl1(cur), mk(1, 0));
+            if (!test_prime(nx, A, ok)) break;
+            ++len;
+            cur = nx;
+        }
+        if (back) res.flags |= CCS_F_NONROOT;
+        if (over) res.flags |= CCS_F_OVERFLOW;
+        if (!ok) res.flags |= CCS_F_UNSUPPORTED;
+        res.root_lo = root.lo;
+        res.root_hi = root.hi;
+        res.chain_len = len;
+        res.back_steps = back;
         ((ccs_chain_result*)A.out)[idx] = res;
     }
 }
@@ -980,7 +1016,7 @@ hipError_t launch_prove(const ProveArgs& a, u32 max_entries, hipStream_t s) {
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, a.chain_count, (size_t)1);
     hipLaunchKernelGGL(prove_kernel, dim3(blocks), dim3(128), 0, s, a);
-    u32 cblocks = (max_entries + 3) / 4;
+    u32 cblocks = (max_entries + 255) / 256;
     if (cblocks < 1) cblocks = 1;
     if (cblocks > 2048) cblocks = 2048;
     hipLaunchKernelGGL(chain_kernel, dim3(cblocks), dim3(256), 0, s, a);

@@ -331,6 +331,22 @@ def test_prove_known_chains_as_jobs(ctx, oracle):
     for (n, _), r in zip(cases, g):
         f, root, ln, back = oracle.prove_one(n)
         assert (int(r["flags"]), int(r["chain_len"]), int(r["back_steps"])) == (f, ln, back)
+    # and through a device launch whose cap (> 4096) selects the gate kernel and the
+    # one-lane-per-chain walk
+    js = ccs.JobSet(s, jobs)
+    cap = 8192
+    d_s, d_c, d_r = ctx.dalloc(cap * 8), ctx.dalloc(64), ctx.dalloc(cap * 32)
+    ctx.h2d(d_s, surv)
+    ctx.h2d(d_c, np.array([len(surv)], np.uint32))
+    js.prove_launch(d_s, d_c, cap, d_r)
+    ctx.synchronize()
+    g2 = np.zeros(len(surv), ccs.CHAIN_RESULT_DTYPE)
+    ctx.d2h(g2, d_r)
+    for f in PROVE_FIELDS + ("survivor_idx",):
+        assert np.array_equal(g2[f], g[f]), f
+    js.close()
+    for ptr in (d_s, d_c, d_r):
+        ctx.dfree(ptr)
 
 
 def test_prove_random_large_values_vs_oracle(ctx, oracle):
