@@ -69,8 +69,9 @@ CCS_HD U2 mul_u64(U2 a, u64 b) { return mk(a.lo * b, umulhi64(a.lo, b) + a.hi * 
 // ---------------------------------------------------------------------------
 
 struct Mont {
-    U2 n, ni;   // ni = -n^-1 mod 2^128
-    U2 one;     // R mod n
+    U2 n, ni;     // ni = -n^-1 mod 2^128
+    U2 one;       // R mod n
+    double ninv;  // 1/n in f64 (quotient estimates of mul_small)
 };
 
 CCS_HD void mont_init(Mont& c, U2 n) {
@@ -88,6 +89,7 @@ CCS_HD void mont_init(Mont& c, U2 n) {
         if (ge(r, n)) r = sub(r, n);
     }
     c.one = r;
+    c.ninv = 1.0 / ((double)n.hi * 18446744073709551616.0 + (double)n.lo);
 }
 
 CCS_HD U2 mont_mul(const Mont& c, U2 a, U2 b) {
@@ -143,6 +145,36 @@ CCS_HD U2 mont_mul(const Mont& c, U2 a, U2 b) {
 CCS_HD U2 madd(const Mont& c, U2 a, U2 b) { U2 r = add(a, b); if (ge(r, c.n)) r = sub(r, c.n); return r; }
 CCS_HD U2 msub(const Mont& c, U2 a, U2 b) { return ge(a, b) ? sub(a, b) : add(sub(a, b), c.n); }
 CCS_HD U2 mhalf(const Mont& c, U2 a) { if (a.lo & 1) a = add(a, c.n); return shr1(a); }
+
+// x*m mod n for x < n and a small m (< 2^16); keeps the Montgomery form of x.  t = x*m has
+// three limbs (< 2^143); its quotient by n (< 2^16) is estimated in f64 with relative error
+// ~2^-50, so the estimate is exact or one off, and t - q*n is corrected once either way.
+CCS_HD U2 mul_small(const Mont& c, U2 x, u32 m) {
+    const u64 t0 = x.lo * m, c0 = umulhi64(x.lo, m);
+    const u64 t1a = x.hi * m, c1 = umulhi64(x.hi, m);
+    const u64 t1 = t1a + c0;
+    const u64 t2 = c1 + (t1 < t1a);
+    const double tf = ((double)t2 * 18446744073709551616.0 + (double)t1) * 18446744073709551616.0 + (double)t0;
+    const u64 q = (u64)(u32)(tf * c.ninv);
+    const u64 p0 = c.n.lo * q, pc0 = umulhi64(c.n.lo, q);
+    const u64 p1a = c.n.hi * q, pc1 = umulhi64(c.n.hi, q);
+    const u64 p1 = p1a + pc0;
+    const u64 p2 = pc1 + (p1 < p1a);
+    // r = t - q*n over three limbs
+    const u64 r0 = t0 - p0, b0 = t0 < p0;
+    const u64 r1 = t1 - p1 - b0, b1 = (t1 < p1) | ((t1 - p1) < b0);
+    const u64 r2 = t2 - p2 - b1;
+    U2 r = mk(r0, r1);
+    if ((int64_t)r2 < 0) r = add(r, c.n);                      // q was one too large
+    else if (r2 != 0 || ge(r, c.n)) r = sub(r, c.n);           // q was one too small
+    return r;
+}
+
+// x*v mod n for a small signed v (|v| < 2^16)
+CCS_HD U2 mul_small_signed(const Mont& c, U2 x, int v) {
+    const U2 r = mul_small(c, x, (u32)(v < 0 ? -v : v));
+    return v < 0 ? msub(c, mk(0, 0), r) : r;
+}
 
 // Montgomery form of a small constant x: x*R mod n by double-and-add of R mod n
 CCS_HD U2 mont_small(const Mont& c, u32 x) {
@@ -244,31 +276,31 @@ CCS_HDN bool strong_lucas(const Mont& c, bool& ok) {
         if (++tries > 200) { ok = false; return false; }
     }
     const int Q = (1 - D) / 4;
-    U2 Dm = mont_small(c, (u32)(D < 0 ? -D : D));
-    if (D < 0) Dm = msub(c, mk(0, 0), Dm);
-    U2 Qm = mont_small(c, (u32)(Q < 0 ? -Q : Q));
-    if (Q < 0) Qm = msub(c, mk(0, 0), Qm);
+    // D*U and Q^k*Q multiply by small constants (mul_small, not a Montgomery product);
+    // with Q = -1 (D = 5) Q^k is +-1, so it is squared to 1 and negated without products
+    const bool qunit = Q == -1;
+    const U2 mone = msub(c, mk(0, 0), c.one);
     U2 d = add(c.n, mk(1, 0));
     int s = 0;
     while (!(d.lo & 1)) { d = shr1(d); ++s; }
-    U2 U = c.one, V = c.one, Qk = Qm;
+    U2 U = c.one, V = c.one, Qk = qunit ? mone : mul_small_signed(c, c.one, Q);
     for (int i = bitlen(d) - 2; i >= 0; --i) {
         U = mont_mul(c, U, V);
         V = msub(c, mont_mul(c, V, V), madd(c, Qk, Qk));
-        Qk = mont_mul(c, Qk, Qk);
+        Qk = qunit ? c.one : mont_mul(c, Qk, Qk);
         if (bit(d, i)) {
             const U2 nu = mhalf(c, madd(c, U, V));
-            const U2 nv = mhalf(c, madd(c, mont_mul(c, Dm, U), V));
+            const U2 nv = mhalf(c, madd(c, mul_small_signed(c, U, D), V));
             U = nu;
             V = nv;
-            Qk = mont_mul(c, Qk, Qm);
+            Qk = qunit ? msub(c, mk(0, 0), Qk) : mul_small_signed(c, Qk, Q);
         }
     }
     if (iszero(U) || iszero(V)) return true;
     for (int r = 1; r < s; ++r) {
         V = msub(c, mont_mul(c, V, V), madd(c, Qk, Qk));
         if (iszero(V)) return true;
-        Qk = mont_mul(c, Qk, Qk);
+        Qk = qunit ? c.one : mont_mul(c, Qk, Qk);
     }
     return false;
 }

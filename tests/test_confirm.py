@@ -86,3 +86,47 @@ def test_bad_parameters():
         ccs.confirm_values([7], test="mr", bases=())
     with pytest.raises(ccs.CcsError):
         ccs.confirm_values([7], bits=200)
+
+
+def strong_pseudoprimes_base2_without_small_factors(count, rng):
+    """Composites n = p(2p - 1) that pass strong MR to base 2 (p, 2p - 1 prime, n = 1 mod 8
+    makes 2 a residue often enough), none with a factor below 410, so the Lucas half of
+    BPSW is what rejects them."""
+    import sympy
+    out = []
+    while len(out) < count:
+        p = int(sympy.randprime(1 << rng.randrange(20, 50), 1 << 51))
+        q = 2 * p - 1
+        if not sympy.isprime(q):
+            continue
+        n = p * q
+        d, s = n - 1, 0
+        while d % 2 == 0:
+            d //= 2
+            s += 1
+        x = pow(2, d, n)
+        if x in (1, n - 1) or any(pow(x, 1 << r, n) == n - 1 for r in range(1, s)):
+            out.append(n)
+    return out
+
+
+def test_library_bpsw_on_primes_and_base2_pseudoprimes():
+    """The library's own Baillie-PSW (the code the GPU kernels compile too) on random primes
+    of 62..127 bits, which must pass every Lucas parameter the Selfridge search picks
+    (D = 5 with Q = -1 and the others), and on base-2 strong pseudoprimes, which only the
+    Lucas half rejects; sympy's isprime decides the expected answer."""
+    import sympy
+    rng = random.Random(0xB95)
+    primes = [int(sympy.randprime(1 << (b - 1), 1 << b)) for b in rng.choices(range(62, 128), k=300)]
+    spsp = strong_pseudoprimes_base2_without_small_factors(40, rng)
+    comps = [p * int(sympy.nextprime(p // 3 + 5)) for p in primes[:60] if p < (1 << 62)]
+    values = primes + spsp + comps
+    res, _ = ccs.confirm_values(values)
+    got = [bool(int(r["flags"]) & ccs.F_PRIME) for r in res]
+    want = [bool(sympy.isprime(v)) for v in values]
+    assert got == want
+    assert sum(want) == len(primes) and len(spsp) == 40
+    # the Selfridge D of the primes: both the Q = -1 case (D = 5) and the others occur
+    from sympy import jacobi_symbol
+    d5 = sum(1 for p in primes if jacobi_symbol(5, p) == -1)
+    assert 50 < d5 < 250
