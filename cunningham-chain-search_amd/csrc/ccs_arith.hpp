@@ -92,53 +92,51 @@ CCS_HD void mont_init(Mont& c, U2 n) {
     c.ninv = 1.0 / ((double)n.hi * 18446744073709551616.0 + (double)n.lo);
 }
 
+// a*b*2^-128 mod n by CIOS over four 32-bit limbs: each step adds a[i]*b, then m*n with
+// m = t0 * (-n^-1 mod 2^32), and drops the low limb.  The products are 32x32 -> 64-bit
+// multiply-adds (v_mad_u64_u32 on the GPU), 32 of them instead of the 58 multiplies that
+// 64-bit limbs take; t stays below 2n < 2^128, so one final subtraction reduces it.
 CCS_HD U2 mont_mul(const Mont& c, U2 a, U2 b) {
-    // t = a*b, four 64-bit limbs
-    u64 l00 = a.lo * b.lo, h00 = umulhi64(a.lo, b.lo);
-    u64 l01 = a.lo * b.hi, h01 = umulhi64(a.lo, b.hi);
-    u64 l10 = a.hi * b.lo, h10 = umulhi64(a.hi, b.lo);
-    u64 l11 = a.hi * b.hi, h11 = umulhi64(a.hi, b.hi);
-    u64 t0 = l00;
-    u64 t1 = h00 + l01;
-    u64 c1 = t1 < l01;
-    t1 += l10;
-    c1 += t1 < l10;
-    u64 t2 = h01 + h10;
-    u64 c2 = t2 < h10;
-    t2 += l11;
-    c2 += t2 < l11;
-    t2 += c1;
-    c2 += t2 < c1;
-    u64 t3 = h11 + c2;
-    // m = t * ni mod 2^128
-    u64 m0 = t0 * c.ni.lo;
-    u64 m1 = umulhi64(t0, c.ni.lo) + t0 * c.ni.hi + t1 * c.ni.lo;
-    // s = m * n, limbs 1..3 (limb 0 cancels t0)
-    l00 = m0 * c.n.lo; h00 = umulhi64(m0, c.n.lo);
-    l01 = m0 * c.n.hi; h01 = umulhi64(m0, c.n.hi);
-    l10 = m1 * c.n.lo; h10 = umulhi64(m1, c.n.lo);
-    l11 = m1 * c.n.hi; h11 = umulhi64(m1, c.n.hi);
-    u64 s1 = h00 + l01;
-    u64 d1 = s1 < l01;
-    s1 += l10;
-    d1 += s1 < l10;
-    u64 s2 = h01 + h10;
-    u64 d2 = s2 < h10;
-    s2 += l11;
-    d2 += s2 < l11;
-    s2 += d1;
-    d2 += s2 < d1;
-    u64 s3 = h11 + d2;
-    (void)s1;
-    // (t + s) / 2^128: the low 128 bits sum to 0 mod 2^128 with a carry iff t_low != 0
-    const u64 k = (t0 | t1) != 0;
-    u64 r0 = t2 + s2;
-    u64 k2 = r0 < s2;
-    r0 += k;
-    k2 += r0 < k;
-    u64 r1 = t3 + s3 + k2;
-    U2 r = mk(r0, r1);
-    if (ge(r, c.n)) r = sub(r, c.n);
+    const u32 A[4] = {(u32)a.lo, (u32)(a.lo >> 32), (u32)a.hi, (u32)(a.hi >> 32)};
+    const u32 B[4] = {(u32)b.lo, (u32)(b.lo >> 32), (u32)b.hi, (u32)(b.hi >> 32)};
+    const u32 N[4] = {(u32)c.n.lo, (u32)(c.n.lo >> 32), (u32)c.n.hi, (u32)(c.n.hi >> 32)};
+    const u32 ni = (u32)c.ni.lo;
+    u32 t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        u64 p = (u64)A[i] * B[0] + t0;
+        t0 = (u32)p;
+        u64 cc = p >> 32;
+        p = (u64)A[i] * B[1] + t1 + cc;
+        t1 = (u32)p;
+        cc = p >> 32;
+        p = (u64)A[i] * B[2] + t2 + cc;
+        t2 = (u32)p;
+        cc = p >> 32;
+        p = (u64)A[i] * B[3] + t3 + cc;
+        t3 = (u32)p;
+        cc = p >> 32;
+        u64 s = (u64)t4 + cc;
+        t4 = (u32)s;
+        const u32 t5 = (u32)(s >> 32);
+        const u32 m = t0 * ni;
+        p = (u64)m * N[0] + t0;   // low limb becomes 0
+        cc = p >> 32;
+        p = (u64)m * N[1] + t1 + cc;
+        t0 = (u32)p;
+        cc = p >> 32;
+        p = (u64)m * N[2] + t2 + cc;
+        t1 = (u32)p;
+        cc = p >> 32;
+        p = (u64)m * N[3] + t3 + cc;
+        t2 = (u32)p;
+        cc = p >> 32;
+        s = (u64)t4 + cc;
+        t3 = (u32)s;
+        t4 = t5 + (u32)(s >> 32);
+    }
+    U2 r = mk((u64)t0 | ((u64)t1 << 32), (u64)t2 | ((u64)t3 << 32));
+    if (t4 || ge(r, c.n)) r = sub(r, c.n);
     return r;
 }
 
