@@ -684,6 +684,44 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
     flush();
 }
 
+// A shape without deep primes: its stage-B survivors are final, so one lane per entry
+// writes the output record (the deep kernel would spend a wave on each, in turn); each
+// wave takes its slots with one atomic.
+__global__ __launch_bounds__(256) void emit_kernel(DeepArgs A) {
+    const u32 lane = lane_id();
+    const u32 all = *A.mid_count;
+    u32 total = all;
+    if (total > A.mid_cap) total = A.mid_cap;
+    if (A.stats && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.stats[2], (unsigned long long)all);
+    const u32 stride = gridDim.x * blockDim.x;
+    for (u32 e0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); e0 < total; e0 += stride) {
+        const u32 e = e0 + lane;
+        const bool live = e < total;
+        const u64 act = ballot(live);
+        u32 base = 0;
+        if (lane == 0) base = atomicAdd(A.out_count, (u32)__builtin_popcountll(act));
+        base = bcast32(base);
+        if (!live) continue;
+        const u32 slot = base + lanes_below(act);
+        if (slot >= A.out_cap) continue;
+        const MidRec rec = A.mid[e];
+        if (A.mode == 0) {
+            const ClassRec cr = A.plain.cls[rec.owner];
+            ccs_survivor s;
+            s.base_idx = cr.base_idx;
+            s.k_offset = cr.k;
+            s.tile_idx = 6u * rec.j + cr.rho - A.plain.off;
+            s._pad = 0;
+            ((ccs_survivor*)A.out)[slot] = s;
+        } else {
+            ccs_job_survivor s;
+            s.job_idx = rec.owner;
+            s.u_off = rec.j;
+            ((ccs_job_survivor*)A.out)[slot] = s;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // K2: prove kernel — GMP confirmation engine per survivor
 // ---------------------------------------------------------------------------
@@ -993,6 +1031,13 @@ hipError_t launch_deep(const DeepArgs& a, u32 max_entries, hipStream_t s) {
     (void)hipGetLastError();
     // up to 65536 waves: with a few entries per wave the dispatcher keeps CUs that are
     // shared with other work (a concurrent prove) from holding back the end of the launch
+    if (a.nc == 0 && !a.group) {   // no deep primes: every stage-B survivor is final
+        u32 eblocks = (max_entries + 255) / 256;
+        if (eblocks < 1) eblocks = 1;
+        if (eblocks > 4096) eblocks = 4096;
+        hipLaunchKernelGGL(emit_kernel, dim3(eblocks), dim3(256), 0, s, a);
+        return hipGetLastError();
+    }
     u32 blocks = (max_entries + 3) / 4;
     if (blocks < 1) blocks = 1;
     if (blocks > 16384) blocks = 16384;
