@@ -25,7 +25,7 @@ PUBLISHED_GPU_CAND_PER_S = 65e9
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch), each scaled by what the
 # counter reads for a 1 GiB device copy (bench.py --copy-calibration): FETCH_SIZE reads
 # 524308 KiB for 1 GiB fetched (x2.0), WRITE_SIZE 1048576 KiB for 1 GiB written (x1.0).
-SIEVE_TRAFFIC = {"c2": {"fetch_kib": 2615.56, "write_kib": 372.83, "fetch_scale": 2.0, "write_scale": 1.0,
+SIEVE_TRAFFIC = {"c2": {"fetch_kib": 2613.41, "write_kib": 373.12, "fetch_scale": 2.0, "write_scale": 1.0,
                         "profile": "profiles/sieve_kernel_cc18_31primorial_pmc.md"}}
 # hardware utilization of the sieve kernel from the same counter profile (SQ counters over
 # 256 CUs x kernel cycles): wave64 VALU issue, LDS port bytes against 128 B/clk per CU,
