@@ -335,7 +335,11 @@ int ccs_filter(ccs_ctx* ctx, const ccs_shape* shape, ccs_u128 tile_start, uint32
 int ccs_last_stats(ccs_ctx* ctx, ccs_filter_stats* out);
 
 /* ---- prove (replaces cc18_prove_kernel v13.cu:1462-1538 and the GMP prover pool
- *      prover_worker :1794-1924): one ccs_chain_result per survivor ---- */
+ *      prover_worker :1794-1924): one ccs_chain_result per survivor.  The launch reads
+ *      min(*d_count, cap) survivors; cap <= 4096 runs one kernel with a wave per survivor
+ *      (every chain position tested at once: short latency), a larger cap a gate kernel
+ *      and a chain-walk kernel with a lane per survivor (little work per survivor for
+ *      large batches).  Both give the same results. ---- */
 int ccs_prove_launch(ccs_ctx* ctx, const ccs_shape* shape, ccs_u128 tile_start,
                      const ccs_survivor* d_surv, const uint32_t* d_count, uint32_t cap,
                      const ccs_prove_params* params, ccs_chain_result* d_out, void* stream);
