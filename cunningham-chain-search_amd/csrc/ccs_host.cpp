@@ -449,23 +449,30 @@ static void free_device_tables(ccs_shape* s) {
 
 static u32 dense_ext_words(u32 q, u32 rg) { return ((q - 1) >> 5) + 64u * rg + 1u; }
 
-// dense slots: consecutive dense primes folded in pairs while q1*q2 <= pair_max
-static std::vector<std::pair<int, int>> dense_slots(const ccs_shape* s, const ccs_tuning* t) {
-    std::vector<std::pair<int, int>> out;
+// A dense slot: up to three consecutive dense primes sharing one pattern, whose period is
+// their product (at most pair_max <= 2^18, the bound of the slot's residue constants).
+struct DenseSlot {
+    u32 first, count;   // filter-prime indices [first, first + count)
+};
+
+// dense slots: consecutive dense primes folded in threes, else pairs, while the product
+// stays <= pair_max (config 2 at 2^16: 37*41*43 = 65231, then 16 pairs)
+static std::vector<DenseSlot> dense_slots(const ccs_shape* s, const ccs_tuning* t) {
+    std::vector<DenseSlot> out;
     for (u32 i = 0; i < t->n_dense;) {
-        if (i + 1 < t->n_dense && (u64)s->fp[i] * s->fp[i + 1] <= t->pair_max) {
-            out.push_back({(int)i, (int)i + 1});
-            i += 2;
-        } else {
-            out.push_back({(int)i, -1});
-            i += 1;
-        }
+        u32 c = 1;
+        u64 prod = s->fp[i];
+        while (c < 3 && i + c < t->n_dense && prod * s->fp[i + c] <= t->pair_max) prod *= s->fp[i + c++];
+        out.push_back({i, c});
+        i += c;
     }
     return out;
 }
 
-static u32 slot_period(const ccs_shape* s, std::pair<int, int> sl) {
-    return sl.second < 0 ? s->fp[sl.first] : s->fp[sl.first] * s->fp[sl.second];
+static u32 slot_period(const ccs_shape* s, DenseSlot sl) {
+    u32 p = 1;
+    for (u32 k = 0; k < sl.count; ++k) p *= s->fp[sl.first + k];
+    return p;
 }
 
 // default split: dense primes below 128 (at most 64), stage B up to 256 primes below 2^12
@@ -540,7 +547,7 @@ static int validate_tuning(const ccs_shape* s, ccs_tuning* t) {
     if (t->rows_per_group != 1 && t->rows_per_group != 2 && t->rows_per_group != 4 && t->rows_per_group != 8)
         return fail(CCS_ERR_INVALID, "rows_per_group must be 1, 2, 4 or 8");
     if (t->n_dense > 2u * kMaxDense) return fail(CCS_ERR_INVALID, "n_dense > 128");
-    if (t->pair_max > (1u << 16)) return fail(CCS_ERR_INVALID, "pair_max > 65536");
+    if (t->pair_max > (1u << 18)) return fail(CCS_ERR_INVALID, "pair_max > 2^18");
     const auto slots = dense_slots(s, t);
     if (slots.size() > (size_t)kMaxDense) return fail(CCS_ERR_INVALID, "more than 64 dense patterns");
     if (t->n_tierb > (u32)kMaxTierB) return fail(CCS_ERR_INVALID, "n_tierb > 1024");
@@ -580,11 +587,14 @@ static int upload_tables(ccs_shape* s) {
         return kill;
     };
     for (size_t k = 0; k < slots.size(); ++k) {
-        const int i1 = slots[k].first, i2 = slots[k].second;
-        const u32 q1 = s->fp[i1], q2 = i2 < 0 ? 1 : s->fp[i2];
-        const u32 Q = q1 * q2;
-        std::vector<u8> k1 = uset(i1), k2;
-        if (i2 >= 0) k2 = uset(i2);
+        const DenseSlot sl = slots[k];
+        const u32 Q = slot_period(s, sl);
+        std::vector<std::vector<u8>> ks;
+        std::vector<u32> qs;
+        for (u32 m = 0; m < sl.count; ++m) {
+            ks.push_back(uset(sl.first + m));
+            qs.push_back(s->fp[sl.first + m]);
+        }
         const u32 words = dense_ext_words(Q, rg);
         DensePrime d;
         d.q = Q;
@@ -597,7 +607,9 @@ static int upload_tables(ccs_shape* s) {
             u32 v = 0;
             for (u32 b = 0; b < 32; ++b) {
                 const u64 x = (u64)w * 32 + b;
-                if (k1[x % q1] || (i2 >= 0 && k2[x % q2])) v |= 1u << b;
+                bool kill = false;
+                for (u32 m = 0; m < sl.count; ++m) kill = kill || ks[m][x % qs[m]];
+                if (kill) v |= 1u << b;
             }
             image.push_back(v);
         }
@@ -2160,7 +2172,7 @@ static int autotune_impl(ccs_ctx* c, ccs_shape* s, const std::function<int(float
             if (t.n_dense == base.n_dense) continue;
             if ((rc = try_tuning(t))) return rc;
         }
-        for (u32 pm : {0u, 4096u, 16384u, 65536u}) {
+        for (u32 pm : {0u, 4096u, 16384u, 65536u, 262144u}) {
             ccs_tuning t = best;
             if (pm == t.pair_max) continue;
             t.pair_max = pm;

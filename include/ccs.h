@@ -161,8 +161,9 @@ typedef struct {
     uint32_t rows_per_group;   /* 1, 2, 4 or 8 rows of 2048 candidates per lane step */
     uint32_t unit_j;           /* candidates per work unit, multiple of 2048, <= 2^20 */
     uint32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu             */
-    uint32_t pair_max;         /* fold adjacent dense primes into one pattern when  */
-                               /*   q1*q2 <= pair_max (0 = one prime per pattern)   */
+    uint32_t pair_max;         /* fold up to three adjacent dense primes into one  */
+                               /*   pattern while their product <= pair_max (at    */
+                               /*   most 2^18; 0 = one prime per pattern)           */
     uint32_t waves_per_block;  /* 1, 2, 4, 8 or 16 wave64s share one LDS table copy */
 } ccs_tuning;
 
