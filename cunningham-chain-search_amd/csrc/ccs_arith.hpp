@@ -96,10 +96,14 @@ CCS_HD void mont_init(Mont& c, U2 n) {
 // m = t0 * (-n^-1 mod 2^32), and drops the low limb.  The products are 32x32 -> 64-bit
 // multiply-adds (v_mad_u64_u32 on the GPU), 32 of them instead of the 58 multiplies that
 // 64-bit limbs take; t stays below 2n < 2^128, so one final subtraction reduces it.
+// L = 3 is the same product for n < 2^96: a, b < n then have a zero top limb too, and
+// with the three top limbs written as constant zeros the compiler drops their products
+// (21 multiply-adds); R stays 2^128, so both forms give the same value.
+template <int L>
 CCS_HD U2 mont_mul(const Mont& c, U2 a, U2 b) {
-    const u32 A[4] = {(u32)a.lo, (u32)(a.lo >> 32), (u32)a.hi, (u32)(a.hi >> 32)};
-    const u32 B[4] = {(u32)b.lo, (u32)(b.lo >> 32), (u32)b.hi, (u32)(b.hi >> 32)};
-    const u32 N[4] = {(u32)c.n.lo, (u32)(c.n.lo >> 32), (u32)c.n.hi, (u32)(c.n.hi >> 32)};
+    const u32 A[4] = {(u32)a.lo, (u32)(a.lo >> 32), (u32)a.hi, L == 3 ? 0u : (u32)(a.hi >> 32)};
+    const u32 B[4] = {(u32)b.lo, (u32)(b.lo >> 32), (u32)b.hi, L == 3 ? 0u : (u32)(b.hi >> 32)};
+    const u32 N[4] = {(u32)c.n.lo, (u32)(c.n.lo >> 32), (u32)c.n.hi, L == 3 ? 0u : (u32)(c.n.hi >> 32)};
     const u32 ni = (u32)c.ni.lo;
     u32 t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0;
 #pragma unroll
@@ -184,34 +188,37 @@ CCS_HD U2 mont_small(const Mont& c, u32 x) {
     return r;
 }
 
+template <int L>
 CCS_HD U2 mont_pow(const Mont& c, U2 bm, U2 e) {
     U2 r = c.one;
     for (int i = bitlen(e) - 1; i >= 0; --i) {
-        r = mont_mul(c, r, r);
-        if (bit(e, i)) r = mont_mul(c, r, bm);
+        r = mont_mul<L>(c, r, r);
+        if (bit(e, i)) r = mont_mul<L>(c, r, bm);
     }
     return r;
 }
 
 // 2^e in Montgomery form: the multiply by the base of each set bit is a doubling (an add)
+template <int L>
 CCS_HD U2 mont_pow2(const Mont& c, U2 e) {
     U2 r = c.one;
     for (int i = bitlen(e) - 1; i >= 0; --i) {
-        r = mont_mul(c, r, r);
+        r = mont_mul<L>(c, r, r);
         if (bit(e, i)) r = madd(c, r, r);
     }
     return r;
 }
 
+template <int L>
 CCS_HDN bool strong_mr(const Mont& c, u32 a) {
     U2 d = sub(c.n, mk(1, 0));
     int s = 0;
     while (!(d.lo & 1)) { d = shr1(d); ++s; }
     const U2 one = c.one, mone = sub(c.n, c.one);
-    U2 x = a == 2 ? mont_pow2(c, d) : mont_pow(c, mont_small(c, a), d);
+    U2 x = a == 2 ? mont_pow2<L>(c, d) : mont_pow<L>(c, mont_small(c, a), d);
     if (eq(x, one) || eq(x, mone)) return true;
     for (int i = 1; i < s; ++i) {
-        x = mont_mul(c, x, x);
+        x = mont_mul<L>(c, x, x);
         if (eq(x, mone)) return true;
         if (eq(x, one)) return false;
     }
@@ -260,6 +267,7 @@ CCS_HD int jacobi_D(int D, U2 n) {
 // strong Lucas-Selfridge test (P = 1, Q = (1-D)/4, D = 5, -7, 9, ... first with (D/n) = -1).
 // `ok` is cleared when no parameter was found in range (never happens for the chain
 // values tested here: they are 2 mod 3, so never perfect squares).
+template <int L>
 CCS_HDN bool strong_lucas(const Mont& c, bool& ok) {
     int D = 5;
     int tries = 0;
@@ -283,9 +291,9 @@ CCS_HDN bool strong_lucas(const Mont& c, bool& ok) {
     while (!(d.lo & 1)) { d = shr1(d); ++s; }
     U2 U = c.one, V = c.one, Qk = qunit ? mone : mul_small_signed(c, c.one, Q);
     for (int i = bitlen(d) - 2; i >= 0; --i) {
-        U = mont_mul(c, U, V);
-        V = msub(c, mont_mul(c, V, V), madd(c, Qk, Qk));
-        Qk = qunit ? c.one : mont_mul(c, Qk, Qk);
+        U = mont_mul<L>(c, U, V);
+        V = msub(c, mont_mul<L>(c, V, V), madd(c, Qk, Qk));
+        Qk = qunit ? c.one : mont_mul<L>(c, Qk, Qk);
         if (bit(d, i)) {
             const U2 nu = mhalf(c, madd(c, U, V));
             const U2 nv = mhalf(c, madd(c, mul_small_signed(c, U, D), V));
@@ -296,11 +304,22 @@ CCS_HDN bool strong_lucas(const Mont& c, bool& ok) {
     }
     if (iszero(U) || iszero(V)) return true;
     for (int r = 1; r < s; ++r) {
-        V = msub(c, mont_mul(c, V, V), madd(c, Qk, Qk));
+        V = msub(c, mont_mul<L>(c, V, V), madd(c, Qk, Qk));
         if (iszero(V)) return true;
-        Qk = qunit ? c.one : mont_mul(c, Qk, Qk);
+        Qk = qunit ? c.one : mont_mul<L>(c, Qk, Qk);
     }
     return false;
+}
+
+// n < 2^96, so the three-limb products apply.  On the GPU the choice is made for the whole
+// wave (its active lanes): one lane above 2^96 sends the wave to the four-limb products,
+// so lanes of one wave never run both forms one after the other.
+CCS_HD bool below96(U2 n) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __all((n.hi >> 32) == 0);
+#else
+    return (n.hi >> 32) == 0;
+#endif
 }
 
 // Baillie-PSW: the decision of GMP's mpz_probab_prime_p(n, 1) (v13.cu:1841)
@@ -316,8 +335,8 @@ CCS_HDN bool is_prime_bpsw(U2 n, const PrimeMod* small18, bool& ok) {
     if (n.hi == 0 && n.lo < 61ull * 61ull) return true;
     Mont c;
     mont_init(c, n);
-    if (!strong_mr(c, 2)) return false;
-    return strong_lucas(c, ok);
+    if (below96(n)) return strong_mr<3>(c, 2) && strong_lucas<3>(c, ok);
+    return strong_mr<4>(c, 2) && strong_lucas<4>(c, ok);
 }
 
 // strong Miller-Rabin over the given bases
@@ -327,10 +346,11 @@ CCS_HDN bool is_prime_mr(U2 n, const u32* bases, int n_bases) {
     if (!(n.lo & 1)) return false;
     Mont c;
     mont_init(c, n);
+    const bool small = below96(n);
     for (int i = 0; i < n_bases; ++i) {
         const u32 b = bases[i];
         if (n.hi == 0 && (u64)b % n.lo == 0) continue;
-        if (!strong_mr(c, b)) return false;
+        if (!(small ? strong_mr<3>(c, b) : strong_mr<4>(c, b))) return false;
     }
     return true;
 }
