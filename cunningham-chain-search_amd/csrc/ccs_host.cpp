@@ -402,6 +402,7 @@ struct ccs_shape {
     u32 zero_lds = 0, zero_far = 0;          // zero words of the LDS image / device bitsets
     u32 nb_lds = 0;                          // stage-B primes with their bitsets in LDS
     double est_survival_b = 1.0;             // expected fraction alive after stage B
+    u32 deep_masked = 0;                     // the deep stage reads the forbidden masks
     double est_survival_d = 1.0;             // expected fraction alive after the dense stage
 };
 
@@ -646,7 +647,8 @@ static int upload_tables(ccs_shape* s) {
     if (far_bits.size() > (1u << kTierBOffBits) || image.size() > (1u << kTierBOffBits))
         return fail(CCS_ERR_INVALID, "stage-B bitsets exceed 2^18 words: fewer stage-B primes");
     std::vector<DeepPrime> deep(nc ? nc : 1);
-    std::vector<u32> hash((size_t)(nc ? nc : 1) * kMaxHashSlots, 0xffffffffu);
+    std::vector<u32> hash((size_t)(nc ? nc : 1) * kMaxHashSlots + (size_t)nc * 4, 0xffffffffu);
+    std::fill(hash.begin() + (size_t)nc * kMaxHashSlots, hash.end(), 0u);   // nc masks
     for (u32 i = 0; i < nc; ++i) {
         const u32 idx = nd + nb + i;
         const PrimeMod pm = make_primemod(s->fp[idx], wfull(s));
@@ -656,10 +658,12 @@ static int upload_tables(ccs_shape* s) {
         d.c64 = pm.c64;
         d.c96 = pm.c96;
         deep[i] = d;
+        u32* mask = &hash[(size_t)nc * kMaxHashSlots + (size_t)i * 4];
         for (u32 f : s->forb[idx]) {
             u32 slot = f & (kMaxHashSlots - 1);
             while (hash[(size_t)i * kMaxHashSlots + slot] != 0xffffffffu) slot = (slot + 1) & (kMaxHashSlots - 1);
             hash[(size_t)i * kMaxHashSlots + slot] = f;
+            mask[(f >> 5) & 3] |= 1u << (f & 31);
         }
     }
 
@@ -671,6 +675,7 @@ static int upload_tables(ccs_shape* s) {
     }
     if (nd == 0) s->est_survival_d = 1.0;
     s->est_survival_b = surv;
+    s->deep_masked = surv > kDeepMaskSurvival ? 1u : 0u;
 
     free_device_tables(s);
     HIPCHK(hipSetDevice(s->device));
@@ -985,7 +990,7 @@ static DeepTab deep_tab(const ccs_shape* s) {
     t.deep = s->d_deep;
     t.hash = s->d_hash;
     t.nc = s->nc;
-    t.pad = 0;
+    t.masked = s->deep_masked;
     return t;
 }
 
@@ -1078,6 +1083,7 @@ static int filter_launch_impl(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint3
     da.deep = s->d_deep;
     da.hash = s->d_hash;
     da.nc = s->nc;
+    da.masked = s->deep_masked;
     da.mode = 0;
     da.plain = sa.plain;
     da.out = d_out;
@@ -1710,6 +1716,7 @@ static void jobset_args(const ccs_jobset* js, SieveArgs& sa, DeepArgs& da, void*
     da.deep = s->d_deep;
     da.hash = s->d_hash;
     da.nc = s->nc;
+    da.masked = s->deep_masked;
     da.mode = 1;
     if (!js->group.empty()) {
         sa.unit_j = 0;                 // each job's own unit size

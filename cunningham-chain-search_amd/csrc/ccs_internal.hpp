@@ -64,12 +64,18 @@ struct TierBPrime {
 
 // Deep stage (K1c): forbidden residues of n mod q in a 64-slot open-addressing hash,
 // key r, slot (r & 63), linear probing, empty slot = 0xFFFFFFFF.
-// Deep-stage prime: 16 bytes, one dwordx4 load per lane and step (the deep kernel is bound
-// by these gathers); 1/q is rebuilt in the kernel and prime i's hash is at i * kMaxHashSlots.
+// Deep-stage prime: 16 bytes, one dwordx4 load per lane and step; 1/q is rebuilt in the
+// kernel and prime i's hash is at i * kMaxHashSlots.  After the nc hashes come nc 128-bit
+// masks (4 words each): bit (r & 127) is set for every forbidden residue r, so a clear
+// bit proves r allowed without the hash probe (a scattered L2 line per lane).  Shapes
+// whose stage-B survivors are many (expected survival above kDeepMaskSurvival) read the
+// masks and probe only where the bit is set; the others probe directly, as the extra
+// load costs more than it saves when the survivors die within a few steps.
 struct DeepPrime {
     u32 q;
     u32 c32, c64, c96;   // 2^(32k) mod q
 };
+constexpr double kDeepMaskSurvival = 1e-6;
 
 struct ClassRec {   // plain lattice class (base_idx, k, rho) and its residue a mod W
     u64 a;
@@ -155,7 +161,7 @@ struct DeepTab {
     const DeepPrime* deep;
     const u32* hash;
     u32 nc;
-    u32 pad;
+    u32 masked;               // read the masks after the hashes
 };
 
 struct DeepArgs {
@@ -163,6 +169,7 @@ struct DeepArgs {
     const DeepPrime* deep;
     const u32* hash;
     u32 nc;
+    u32 masked;               // read the masks after the hashes (plain / one-shape launches)
     const DeepTab* group;     // per-shape deep tables of a group launch (null: one shape)
     int mode;
     PlainSrc plain;

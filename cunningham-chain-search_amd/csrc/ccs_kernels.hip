@@ -631,6 +631,7 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
         const DeepPrime* deep = A.deep;
         const u32* hash = A.hash;
         u32 nc = A.nc;
+        u32 masked = A.masked;
         if (A.mode == 0) {
             const ClassRec cr = A.plain.cls[owner];
             a = cr.a;
@@ -649,6 +650,7 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
                 deep = t.deep;
                 hash = t.hash;
                 nc = t.nc;
+                masked = t.masked;
             }
         }
         add128(ulo, uhi, j);
@@ -659,14 +661,24 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
             bool kill = false;
             if (p < nc) {
                 const DeepPrime dp = deep[p];
+                uint4 m = make_uint4(0u, 0u, 0u, 0u);
+                if (masked) m = ((const uint4*)(hash + (size_t)nc * kMaxHashSlots))[p];   // wave-uniform
                 const u32 r = deep_mod(n, dp);
-                const u32* h = hash + (size_t)p * kMaxHashSlots;
-                u32 slot = r & (kMaxHashSlots - 1);
-                for (int probe = 0; probe < kMaxHashSlots; ++probe) {
-                    const u32 v = h[slot];
-                    if (v == r) { kill = true; break; }
-                    if (v == 0xffffffffu) break;
-                    slot = (slot + 1) & (kMaxHashSlots - 1);
+                bool probe = true;
+                if (masked) {
+                    const u32 w = (r >> 5) & 3;
+                    const u32 mw = w == 0 ? m.x : w == 1 ? m.y : w == 2 ? m.z : m.w;
+                    probe = (mw >> (r & 31)) & 1u;
+                }
+                if (probe) {
+                    const u32* h = hash + (size_t)p * kMaxHashSlots;
+                    u32 slot = r & (kMaxHashSlots - 1);
+                    for (int probe = 0; probe < kMaxHashSlots; ++probe) {
+                        const u32 v = h[slot];
+                        if (v == r) { kill = true; break; }
+                        if (v == 0xffffffffu) break;
+                        slot = (slot + 1) & (kMaxHashSlots - 1);
+                    }
                 }
             }
             if (ballot(kill)) { dead = true; break; }
