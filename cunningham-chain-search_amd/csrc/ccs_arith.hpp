@@ -322,21 +322,34 @@ CCS_HD bool below96(U2 n) {
 #endif
 }
 
-// Baillie-PSW: the decision of GMP's mpz_probab_prime_p(n, 1) (v13.cu:1841)
-// small18: PrimeMod records of the 18 primes 2..61
-CCS_HDN bool is_prime_bpsw(U2 n, const PrimeMod* small18, bool& ok) {
-    if (n.hi == 0 && n.lo < 2) return false;
+// Baillie-PSW in two parts, for callers that gather the Lucas tests of many values:
+// bpsw_pre divides by the primes 2..61 (small18: their PrimeMod records) and runs the
+// strong MR to base 2, returning 0 (composite), 1 (prime: n is small) or 2 (a base-2
+// strong probable prime, decided by bpsw_lucas).
+CCS_HDN int bpsw_pre(U2 n, const PrimeMod* small18) {
+    if (n.hi == 0 && n.lo < 2) return 0;
 #pragma unroll 1
     for (int i = 0; i < 18; ++i) {
         const u32 p = small18[i].q;
-        if (n.hi == 0 && n.lo == p) return true;
-        if (mod128(n.lo, n.hi, small18[i]) == 0) return false;
+        if (n.hi == 0 && n.lo == p) return 1;
+        if (mod128(n.lo, n.hi, small18[i]) == 0) return 0;
     }
-    if (n.hi == 0 && n.lo < 61ull * 61ull) return true;
+    if (n.hi == 0 && n.lo < 61ull * 61ull) return 1;
     Mont c;
     mont_init(c, n);
-    if (below96(n)) return strong_mr<3>(c, 2) && strong_lucas<3>(c, ok);
-    return strong_mr<4>(c, 2) && strong_lucas<4>(c, ok);
+    return (below96(n) ? strong_mr<3>(c, 2) : strong_mr<4>(c, 2)) ? 2 : 0;
+}
+
+CCS_HDN bool bpsw_lucas(U2 n, bool& ok) {
+    Mont c;
+    mont_init(c, n);
+    return below96(n) ? strong_lucas<3>(c, ok) : strong_lucas<4>(c, ok);
+}
+
+// Baillie-PSW: the decision of GMP's mpz_probab_prime_p(n, 1) (v13.cu:1841)
+CCS_HDN bool is_prime_bpsw(U2 n, const PrimeMod* small18, bool& ok) {
+    const int pre = bpsw_pre(n, small18);
+    return pre == 1 || (pre == 2 && bpsw_lucas(n, ok));
 }
 
 // strong Miller-Rabin over the given bases

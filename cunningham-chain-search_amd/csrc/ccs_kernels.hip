@@ -753,8 +753,16 @@ __device__ __forceinline__ U2 survivor_n(const ProveArgs& A, u32 idx) {
     return cand_n(jr.W, jr.W_hi, jr.a, jr.a_hi, ulo, uhi);
 }
 
+// private flag of a K2a record whose n passed the strong MR to base 2: K2b runs its Lucas
+// test (lanes that all need one) and clears the flag, or the prime flag with it
+constexpr u32 kLucasPending = 1u << 31;
+
 // K2a: per survivor the gates of prover_worker (v13.cu:1828-1842): bit size, trial division
 // by 67..409, primality of n.  Composites are final here; primes go to the chain queue.
+// Under Baillie-PSW this kernel runs the strong MR to base 2 only: a wave of survivors holds
+// a few primes (~1 in 7 at 89 bits), and running their Lucas tests here would hold every
+// lane of the wave for them; the base-2 probable primes are queued and their Lucas tests
+// run in K2b, one per lane.
 __global__ __launch_bounds__(128) void prove_kernel(ProveArgs A) {
     u32 total = *A.surv_count;
     if (total > A.surv_cap) total = A.surv_cap;
@@ -778,8 +786,14 @@ __global__ __launch_bounds__(128) void prove_kernel(ProveArgs A) {
             if (div) break;
             res.flags |= CCS_F_TRIAL_OK;
             if (n.hi >> 63) { res.flags |= CCS_F_OVERFLOW; break; }
-            if (!test_prime(n, A, ok)) break;
-            res.flags |= CCS_F_PRIME;
+            if (A.test_kind == 0) {
+                const int pre = bpsw_pre(n, c_small);
+                if (pre == 0) break;
+                res.flags |= CCS_F_PRIME | (pre == 2 ? kLucasPending : 0u);
+            } else {
+                if (!test_prime(n, A, ok)) break;
+                res.flags |= CCS_F_PRIME;
+            }
             queue = true;
         } while (0);
         if (!ok) res.flags |= CCS_F_UNSUPPORTED;
@@ -884,6 +898,17 @@ __global__ __launch_bounds__(256) void chain_kernel(ProveArgs A) {
         const U2 n = survivor_n(A, idx);
         ccs_chain_result res = ((ccs_chain_result*)A.out)[idx];
         bool ok = true;
+        if (res.flags & kLucasPending) {   // n passed the base-2 MR: its Lucas test decides
+            res.flags &= ~kLucasPending;
+            bool okn = true;
+            const bool prime = bpsw_lucas(n, okn);
+            if (!okn) res.flags |= CCS_F_UNSUPPORTED;
+            if (!prime) {
+                res.flags &= ~CCS_F_PRIME;
+                ((ccs_chain_result*)A.out)[idx] = res;
+                continue;
+            }
+        }
         // backward: consecutive primes at 1, 2, ... steps back (at most 65)
         u32 back = 0;
         U2 root = n;
