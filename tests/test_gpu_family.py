@@ -237,3 +237,30 @@ def test_pool_group_launch_matches_oracle(ctx, oracle):
     ctx.dfree(d_cnt)
     for sh in shapes:
         sh.close()
+
+
+def test_large_batch_prove_on_base2_pseudoprimes(ctx):
+    """More than 4096 survivors take the large-batch prove (strong MR to base 2 per survivor,
+    then the Lucas tests of the base-2 probable primes gathered one per lane).  Family shape
+    D = 2, E = 0 makes any odd n a survivor (n = 2Q - 1 with Q = (n + 1) / 2), so base-2
+    strong pseudoprimes without small factors (rejected only by the Lucas test), primes and
+    random odd values go through it: every record equals the host confirmation engine."""
+    import sympy
+    from test_confirm import strong_pseudoprimes_base2_without_small_factors
+    rng = random.Random(0xB2)
+    spsp = strong_pseudoprimes_base2_without_small_factors(24, rng)
+    primes = [int(sympy.randprime(1 << 80, 1 << 100)) for _ in range(200)]
+    values = spsp + primes + [rng.randrange(1 << 60, 1 << 100) | 1 for _ in range(4400)]
+    rng.shuffle(values)
+    shape = ccs.FamilyShape(ctx, 2, 0, 2, 1, [3])
+    jobs = [(0, (v + 1) // 2, 1) for v in values]
+    surv = np.zeros(len(values), ccs.JOB_SURVIVOR_DTYPE)
+    surv["job_idx"] = np.arange(len(values), dtype=np.uint32)
+    say(f"[family] large-batch prove: {len(values)} values, {len(spsp)} base-2 strong pseudoprimes")
+    res = shape.prove_jobs(jobs, surv)
+    host, _ = ccs.confirm_values(values)
+    for f in ("flags", "chain_len", "back_steps", "root_lo", "root_hi"):
+        assert (res[f] == host[f]).all(), f
+    prime = {v for v, r in zip(values, res) if int(r["flags"]) & ccs.F_PRIME}
+    assert not prime & set(spsp) and set(primes) <= prime
+    shape.close()
