@@ -647,8 +647,10 @@ static int upload_tables(ccs_shape* s) {
     if (far_bits.size() > (1u << kTierBOffBits) || image.size() > (1u << kTierBOffBits))
         return fail(CCS_ERR_INVALID, "stage-B bitsets exceed 2^18 words: fewer stage-B primes");
     std::vector<DeepPrime> deep(nc ? nc : 1);
-    std::vector<u32> hash((size_t)(nc ? nc : 1) * kMaxHashSlots + (size_t)nc * 4, 0xffffffffu);
-    std::fill(hash.begin() + (size_t)nc * kMaxHashSlots, hash.end(), 0u);   // nc masks
+    // nc hashes (one empty one when nc = 0), then the nc masks
+    const size_t hash_words = (size_t)(nc ? nc : 1) * kMaxHashSlots;
+    std::vector<u32> hash(hash_words + (size_t)nc * 4, 0xffffffffu);
+    std::fill(hash.begin() + hash_words, hash.end(), 0u);
     for (u32 i = 0; i < nc; ++i) {
         const u32 idx = nd + nb + i;
         const PrimeMod pm = make_primemod(s->fp[idx], wfull(s));
