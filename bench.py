@@ -25,12 +25,12 @@ PUBLISHED_GPU_CAND_PER_S = 65e9
 # rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch), each scaled by what the
 # counter reads for a 1 GiB device copy (bench.py --copy-calibration): FETCH_SIZE reads
 # 524308 KiB for 1 GiB fetched (x2.0), WRITE_SIZE 1048576 KiB for 1 GiB written (x1.0).
-SIEVE_TRAFFIC = {"c2": {"fetch_kib": 2646.72, "write_kib": 373.81, "fetch_scale": 2.0, "write_scale": 1.0,
+SIEVE_TRAFFIC = {"c2": {"fetch_kib": 2065.25, "write_kib": 1879.53, "fetch_scale": 2.0, "write_scale": 1.0,
                         "profile": "profiles/sieve_kernel_cc18_31primorial_pmc.md"}}
 # hardware utilization of the sieve kernel from the same counter profile (SQ counters over
 # 256 CUs x kernel cycles): wave64 VALU issue, LDS port bytes against 128 B/clk per CU,
 # LDS bank-conflict cycles
-SIEVE_ISSUE = {"c2": {"valu_issue_frac": 0.56, "lds_port_frac": 0.72, "lds_bank_conflict_frac": 0.064,
+SIEVE_ISSUE = {"c2": {"valu_issue_frac": 0.57, "lds_port_frac": 0.76, "lds_bank_conflict_frac": 0.068,
                       "profile": "profiles/sieve_kernel_cc18_31primorial_pmc.md"}}
 
 

@@ -502,9 +502,9 @@ static u32 tierb_in_lds(const ccs_shape* s, const ccs_tuning* t) {
 }
 
 // Starting point measured on the 31# depth-18 workload: dense primes below 200 (320 at
-// shallow depths) folded in pairs up to period 2^16, eight rows per lane, stage B over the
-// next 512 primes below 4096 (1024 below 16384 at shallow depths), 2^20-candidate units,
-// one block of 16 waves per CU.  Shrunk until the tables fit the device's LDS.
+// shallow depths) folded up to period 2^16, eight rows per lane, stage B sized by the
+// expected survival (below), 2^20-candidate units, one block of 16 waves per CU.  Shrunk
+// until the tables fit the device's LDS.
 static void default_tuning(const ccs_shape* s, ccs_tuning* t) {
     u32 nd = 0;
     while (nd < s->fp.size() && nd < 2u * (u32)kMaxDense && s->fp[nd] < 200) ++nd;
@@ -516,16 +516,19 @@ static void default_tuning(const ccs_shape* s, ccs_tuning* t) {
     for (u32 i = 0; i < nd; ++i) sd *= 1.0 - (double)s->forb[i].size() / s->fp[i];
     if (sd > 2.5e-3)
         while (nd < s->fp.size() && nd < 2u * (u32)kMaxDense && s->fp[nd] < 320) ++nd;
+    // Stage B: the smallest multiple of 64 primes (at least 128) after which at most 4e-6 of
+    // the candidates are expected to survive, up to 1024 primes below 16384; the deep stage
+    // takes the rest (with its masks above 1e-6 survival).  Measured: config 2 (31#, depth
+    // 18) 192 primes +3 % over 512 (128: +1.5 %, 256: +0.4 %, 64: -7 %); config 3 (37#,
+    // depth 19) 128 primes +7 % over 512; the shallow 31# depths 12 and 14 take 1024.
     u32 nb = 0;
-    while (nd + nb < s->fp.size() && nb < (u32)kDefaultTierB && s->fp[nd + nb] < 4096) ++nb;
-    // A shallow depth sends many candidates on to the deep stage, whose test of one prime
-    // costs several times a stage-B probe: when more than 2e-6 of the candidates are
-    // expected to pass this stage B, widen it to 1024 primes (measured on the 31# depth
-    // profiles: depth 12 -13 %, depth 14 -6 %, depth 16 even, depths 18-20 +1-2 %).
     double surv = 1.0;
-    for (u32 i = 0; i < nd + nb; ++i) surv *= 1.0 - (double)s->forb[i].size() / s->fp[i];
-    if (surv > 2e-6)
-        while (nd + nb < s->fp.size() && nb < (u32)kMaxTierB && s->fp[nd + nb] < kTierBMaxQ) ++nb;
+    for (u32 i = 0; i < nd; ++i) surv *= 1.0 - (double)s->forb[i].size() / s->fp[i];
+    while (nd + nb < s->fp.size() && nb < (u32)kMaxTierB && s->fp[nd + nb] < kTierBMaxQ) {
+        surv *= 1.0 - (double)s->forb[nd + nb].size() / s->fp[nd + nb];
+        ++nb;
+        if (nb >= 128 && nb % 64 == 0 && surv <= 4e-6) break;
+    }
     t->n_dense = nd;
     t->n_tierb = nb;
     t->rows_per_group = 8;

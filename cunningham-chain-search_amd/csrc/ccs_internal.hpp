@@ -20,7 +20,6 @@ constexpr int kMaxDense = 64;       // dense primes held one per lane
 constexpr int kMaxHashSlots = 64;   // forbidden-residue hash per deep prime
 constexpr int kTierBGroup = 64;     // stage-B primes tested per wave step (one per lane)
 constexpr int kMaxTierB = 1024;     // stage-B primes: at most 16 groups of 64
-constexpr int kDefaultTierB = 512;  // the default tuning's stage B (the autotuner may widen it)
 constexpr unsigned kTierBMaxQ = 16384;   // stage-B primes are below this (14 bits)
 constexpr unsigned kTierBOffBits = 18;   // bitset word offsets of stage-B primes (< 2^18)
 constexpr unsigned kLdsCtlWords = 8;   // sieve scheduler words after the LDS tables

@@ -157,7 +157,7 @@ typedef struct {
 /* Kernel tiling knobs (chosen by the autotuner, or set explicitly). */
 typedef struct {
     uint32_t n_dense;          /* filter primes handled by the dense pattern stage (<= 128, at most 64 patterns) */
-    uint32_t n_tierb;          /* next primes handled by stage B (<= 512, all < 4096) */
+    uint32_t n_tierb;          /* next primes handled by stage B (<= 1024, all < 16384) */
     uint32_t rows_per_group;   /* 1, 2, 4 or 8 rows of 2048 candidates per lane step */
     uint32_t unit_j;           /* candidates per work unit, multiple of 2048, <= 2^20 */
     uint32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu             */
