@@ -18,6 +18,8 @@
  *                               v13 self-tests 15-19 (mod-6 bucket, CC17 base, q7 roots)
  *   filter predicate .......... brute-force p_i mod q over every position, and the reference
  *                               root sample (data/sample/cc10plus_roots_sample_1000.txt)
+ *   class-form filter ......... orc_filter_class against orc_first_kill_brute over windows of
+ *                               classes mod 6*31# and 6*37# (u and n crossing 2^64)
  *   primality / chains ........ sympy.isprime (independent library), known chains CC5..CC18,
  *                               Carmichael numbers and strong pseudoprimes to bases 2..13
  *   BPSW vs GMP on BPSW pseudoprimes: parity unpinned (none is known to exist).
@@ -318,6 +320,17 @@ ORC_EXPORT int orc_first_kill_brute(void *hp, u64 lo, u64 hi) {
         }
     }
     return s->nfp;
+}
+
+/* orc_first_kill_brute for every n = a + W*(u0 + j), j in [0, count): the direct
+ * predicate over a class window, the model orc_filter_class is checked against */
+ORC_EXPORT void orc_first_kill_brute_class(void *hp, u64 a_lo, u64 a_hi, u64 W_lo, u64 W_hi, u64 u0_lo, u64 u0_hi,
+                                           u64 count, int *out) {
+    u128 a = ((u128)a_hi << 64) | a_lo, W = ((u128)W_hi << 64) | W_lo, u0 = ((u128)u0_hi << 64) | u0_lo;
+    for (u64 j = 0; j < count; j++) {
+        u128 n = a + W * (u0 + j);
+        out[j] = orc_first_kill_brute(hp, (u64)n, (u64)(n >> 64));
+    }
 }
 
 /* CRT/wheel admissibility of n itself (all p_i, i < target, coprime to every

@@ -67,6 +67,7 @@ def lib():
     for name in ("orc_first_kill", "orc_first_kill_brute", "orc_on_lattice"):
         getattr(L, name).argtypes = [C.c_void_p, C.c_uint64, C.c_uint64]
         getattr(L, name).restype = C.c_int
+    L.orc_first_kill_brute_class.argtypes = [C.c_void_p] + [C.c_uint64] * 7 + [i32p]
     L.orc_decompose.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint32),
                                 C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.orc_decompose.restype = C.c_int
@@ -187,6 +188,12 @@ class Shape:
 
     def first_kill_brute(self, n):
         return lib().orc_first_kill_brute(self.h, *split(n))
+
+    def first_kill_brute_class(self, a, W, u0, count):
+        """first_kill_brute of n = a + W*(u0 + j) for j in [0, count) (int32 array)"""
+        out = np.zeros(max(count, 1), np.int32)
+        lib().orc_first_kill_brute_class(self.h, *split(a), *split(W), *split(u0), count, out)
+        return out[:count]
 
     def accept(self, n):
         return self.first_kill(n) == len(self.fp)

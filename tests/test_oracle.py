@@ -358,6 +358,57 @@ def test_survival_rate_matches_analytic(ref_shape):
     assert abs(len(surv) - expected) < 5 * math.sqrt(expected)
 
 
+@pytest.mark.parametrize("wheel,depth", [([23, 29, 31], 10), ([23, 29, 31], 18),
+                                         ([23, 29, 31, 37], 10), ([23, 29, 31, 37], 18)])
+@pytest.mark.parametrize("primes", ["small", "large"])
+def test_filter_class_against_direct_predicate(oracle, wheel, depth, primes):
+    """orc_filter_class (the class-form checker of the batched, group and served tests)
+    against first_kill_brute, which evaluates q | p_i directly (itself pinned above by
+    test_predicate_matches_direct_divisibility): survivors and the alive counts after each
+    cut, for classes mod 6*31# and 6*37# at depths 10 and 18, in windows of 2^18 where u
+    crosses 2^64 and where n crosses 2^64.  "small": the primes from the first above the
+    wheel to 1500 (dense- and stage-B-sized); "large": primes near 2^16 and near 2^18
+    (deep-sized tables), alone so that they do the killing."""
+    import sys
+    from conftest import PKG
+    sys.path.insert(0, PKG)
+    from ccs import workloads as WL
+    if primes == "small":   # enough primes that a few dozen of 2^18 candidates survive
+        fp = [int(q) for q in oracle.primes_between(wheel[-1] + 1, 1500 if depth <= 10 else 400)]
+        cuts = [sum(1 for q in fp if q < 100), sum(1 for q in fp if q < 200), len(fp)]
+    else:
+        fp = [int(q) for q in oracle.primes_between(64000, 65535)][:24]
+        fp += [int(q) for q in oracle.primes_between(261000, (1 << 18) - 1)][:24]
+        cuts = [12, 24, len(fp)]
+    # the class-form predicate needs only the filter primes and the depth: the oracle
+    # object's own lattice is kept small (wheel 23, target 18)
+    s = oracle.Shape(CRT, [23], 18, depth, fp, 97)
+    W = 6 * math.prod(CRT) * math.prod(wheel)
+    nfp = len(fp)
+    rng = WL.XorShift64(0xC1A55 + depth + len(wheel) + len(fp))
+    count = 1 << 18
+    for u0_kind in ("u_crosses_2^64", "n_crosses_2^64"):
+        a, Wc = WL.random_admissible_class(rng, CRT + wheel, depth)
+        assert Wc == W
+        if u0_kind == "u_crosses_2^64":
+            u0 = (1 << 64) - count // 2
+        else:
+            u0 = (1 << 64) // W - count // 2
+            assert (a + W * u0) < (1 << 64) < (a + W * (u0 + count))
+        got, alive = s.filter_class(a, W, u0, count, cap=count, cuts=cuts)
+        kills = s.first_kill_brute_class(a, W, u0, count)
+        # the window's n computed in Python's integers on a sample
+        for j in random.Random(depth).sample(range(count), 256):
+            assert kills[j] == s.first_kill_brute(a + W * (u0 + j))
+        want = np.nonzero(kills == nfp)[0]
+        assert np.array_equal(got.astype(np.int64), want)
+        assert alive[0] == count
+        for c, cut in enumerate(cuts):
+            assert alive[c + 1] == int((kills >= cut).sum()), (u0_kind, cut)
+        # the window exercises every cut: some candidates die between consecutive cuts
+        assert count > alive[1] > alive[2] > alive[3] > 0
+
+
 @pytest.mark.parametrize("name", ["reference", "c1", "c2"])
 def test_bitvector_filter_equals_per_candidate(oracle, name):
     import sys
