@@ -11,6 +11,8 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <fstream>
+#include <map>
 #include <string>
 
 #include <fcntl.h>
@@ -250,35 +252,32 @@ extern "C" int ccs_campaign_save(const ccs_campaign* c, const char* path) {
 
 extern "C" int ccs_campaign_resume(ccs_campaign* c, const char* path) {
     if (!c || !path) return cfail(CCS_ERR_INVALID, "null argument");
-    FILE* fp = std::fopen(path, "r");
-    if (!fp) return cfail(CCS_ERR_INVALID, std::string("cannot open checkpoint file '") + path + "'");
-    char line[512];
-    if (!std::fgets(line, sizeof(line), fp) || std::strncmp(line, "CC18_CUDA_v10", 13) != 0) {
-        std::fclose(fp);
-        return cfail(CCS_ERR_INVALID, "checkpoint file has wrong version marker");
+    // the file: a format line, then key=value lines (unknown keys are ignored)
+    std::ifstream in(path);
+    if (!in) return cfail(CCS_ERR_INVALID, std::string("checkpoint ") + path + ": cannot be read");
+    std::string line;
+    if (!std::getline(in, line) || line.compare(0, 13, "CC18_CUDA_v10") != 0)
+        return cfail(CCS_ERR_INVALID, std::string("checkpoint ") + path + ": not a CC18_CUDA_v10 file");
+    std::map<std::string, std::string> kv;
+    while (std::getline(in, line)) {
+        const size_t eq = line.find('=');
+        if (eq == std::string::npos) continue;
+        std::string val = line.substr(eq + 1);
+        while (!val.empty() && (val.back() == '\r' || val.back() == ' ')) val.pop_back();
+        kv[line.substr(0, eq)] = val;
     }
-    int bits = 0, target = 0, depth = 0;
-    unsigned lanes = 0, lane_id = 0, best = 0;
-    char prefix[256] = "";
-    unsigned long long thi = 0, tlo = 0, tiles = 0, primes = 0, chains = 0;
-    while (std::fgets(line, sizeof(line), fp)) {
-        line[std::strcspn(line, "\n")] = '\0';
-        if (std::sscanf(line, "bits=%d", &bits) == 1) continue;
-        if (std::sscanf(line, "target=%d", &target) == 1) continue;
-        if (std::sscanf(line, "depth=%d", &depth) == 1) continue;
-        if (std::sscanf(line, "prefix=%255s", prefix) == 1) continue;
-        if (std::sscanf(line, "prefix_lanes=%u", &lanes) == 1) continue;
-        if (std::sscanf(line, "prefix_lane_id=%u", &lane_id) == 1) continue;
-        if (std::sscanf(line, "tile_hi=%llx", &thi) == 1) continue;
-        if (std::sscanf(line, "tile_lo=%llx", &tlo) == 1) continue;
-        if (std::sscanf(line, "total_tiles=%llu", &tiles) == 1) continue;
-        if (std::sscanf(line, "total_primes=%llu", &primes) == 1) continue;
-        if (std::sscanf(line, "total_chains=%llu", &chains) == 1) continue;
-        if (std::sscanf(line, "best_chain=%u", &best) == 1) continue;
-    }
-    std::fclose(fp);
+    auto num = [&](const char* key, int base) -> unsigned long long {
+        auto it = kv.find(key);
+        return it == kv.end() ? 0ull : std::strtoull(it->second.c_str(), nullptr, base);
+    };
+    const int bits = (int)num("bits", 10), target = (int)num("target", 10), depth = (int)num("depth", 10);
+    const unsigned lanes = (unsigned)num("prefix_lanes", 10), lane_id = (unsigned)num("prefix_lane_id", 10);
+    const unsigned best = (unsigned)num("best_chain", 10);
+    const unsigned long long thi = num("tile_hi", 16), tlo = num("tile_lo", 16), tiles = num("total_tiles", 10);
+    const unsigned long long primes = num("total_primes", 10), chains = num("total_chains", 10);
+    const std::string prefix = kv.count("prefix") ? kv["prefix"] : std::string();
     auto mismatch = [&](const char* what, const std::string& a, const std::string& b) {
-        return cfail(CCS_ERR_INVALID, std::string("checkpoint ") + what + "=" + a + " does not match current " + b);
+        return cfail(CCS_ERR_INVALID, std::string("checkpoint was written with ") + what + "=" + a + ", this run has " + b);
     };
     if (bits != c->p.bits) return mismatch("bits", std::to_string(bits), std::to_string(c->p.bits));
     if (target != c->p.target) return mismatch("target", std::to_string(target), std::to_string(c->p.target));

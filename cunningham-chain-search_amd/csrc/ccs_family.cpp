@@ -1,15 +1,14 @@
 // Fingerprint families: the host side of the v16 Q-iteration front end.
 //
 // A family is p_0 = Q*D*2^E - 1 for an immune-factor product D (a pool entry).  This file
-// parses pool lists and pool files into entries (v16...cu:556-738, :828-880), computes a
+// reads pool lists and pool files into entries (the input format of v16's --seed-pool and
+// pool files, v16...cu:556-738, :828-880, read here by its own grammar), computes a
 // family's Q window for a bit band (v16...cu:4405-4468) and flags the first occurrence of
 // each p_0 (the dedup of v16...cu:2478-2541).  The family's GPU sieve is an ordinary job
 // set on a family shape (ccs_family_shape_build).
 #include <algorithm>
-#include <cstdio>
-#include <cstdlib>
 #include <cstring>
-#include <functional>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -27,231 +26,258 @@ bool is_prime32(uint32_t q) {
     return true;
 }
 
-// the primes up to 503, the table v16 draws primorials and auto variants from
-const std::vector<uint32_t>& small_primes() {
-    static const std::vector<uint32_t> t = [] {
-        std::vector<uint32_t> v;
-        for (uint32_t q = 2; q <= 503; ++q)
-            if (is_prime32(q)) v.push_back(q);
-        return v;
-    }();
-    return t;
-}
+constexpr uint64_t kMaxPrimorial = 500;   // largest Z of "Z#" and "auto:Z#:K"
+constexpr uint64_t kMaxPower = 16;        // largest exponent of "P^k"
 
-int bad(const std::string& msg) { return ccs::set_error(CCS_ERR_INVALID, msg.c_str()); }
+int reject(const std::string& msg) { return ccs::set_error(CCS_ERR_INVALID, msg.c_str()); }
 
-void set_name(ccs_pool_entry& e, const std::string& name) {
-    std::memset(e.name, 0, sizeof(e.name));
-    std::strncpy(e.name, name.c_str(), sizeof(e.name) - 1);
-}
+// Pool-list grammar (blanks = spaces and tabs, allowed around every token):
+//   list      = item { "," item }               (blank items are skipped)
+//   item      = auto | primorial | product
+//   primorial = Z "#" { "/" P }                 (primes <= Z, each P left out; 2 <= Z <= 500)
+//   product   = factor { "*" factor }           factor = P [ "^" k ]   (1 <= k <= 16)
+//   auto      = "auto:" Z "#:" k1 [ "-" k2 ]    (every way of leaving out k1..k2 primes <= Z
+//                                                that are not in keep[], k2 clamped to their count)
+// An item's D is the product of its primes (with multiplicity) and must fit in 128 bits.
+class Reader {
+public:
+    explicit Reader(const std::string& text) : t_(text) {}
+    void blanks() {
+        while (i_ < t_.size() && (t_[i_] == ' ' || t_[i_] == '\t')) ++i_;
+    }
+    bool at_end() {
+        blanks();
+        return i_ >= t_.size();
+    }
+    bool accept(char c) {
+        blanks();
+        if (i_ < t_.size() && t_[i_] == c) {
+            ++i_;
+            return true;
+        }
+        return false;
+    }
+    bool accept_word(const char* w) {
+        blanks();
+        const size_t n = std::strlen(w);
+        if (t_.compare(i_, n, w) != 0) return false;
+        i_ += n;
+        return true;
+    }
+    // a decimal number (at most 18 digits); false when none starts here
+    bool number(uint64_t& v) {
+        blanks();
+        size_t j = i_;
+        v = 0;
+        while (j < t_.size() && t_[j] >= '0' && t_[j] <= '9' && j - i_ < 18) v = v * 10 + (uint64_t)(t_[j++] - '0');
+        if (j == i_ || (j < t_.size() && t_[j] >= '0' && t_[j] <= '9')) return false;
+        i_ = j;
+        return true;
+    }
+    bool starts_primorial() const {   // digits, blanks, then '#'
+        size_t j = i_;
+        while (j < t_.size() && t_[j] >= '0' && t_[j] <= '9') ++j;
+        while (j < t_.size() && (t_[j] == ' ' || t_[j] == '\t')) ++j;
+        return j < t_.size() && t_[j] == '#';
+    }
+    const std::string& text() const { return t_; }
 
-// D from the entry's primes, with the u128 overflow check
-int finish_entry(ccs_pool_entry& e, const std::string& spec) {
+private:
+    const std::string& t_;
+    size_t i_ = 0;
+};
+
+// An entry under construction: prime factors with multiplicity and their product.
+struct Factors {
+    uint32_t p[64];
+    uint32_t n = 0;
     u128 D = 1;
-    for (uint32_t i = 0; i < e.n_primes; ++i) {
-        const uint32_t q = e.primes[i];
-        if (!is_prime32(q)) return bad(std::to_string(q) + " in '" + spec + "' is not prime");
-        const u128 nd = D * (u128)q;
-        if (nd / q != D) return bad("D overflows u128 in '" + spec + "'");
-        D = nd;
+    int add(uint64_t q, const std::string& item) {
+        if (q > 0xffffffffull || !is_prime32((uint32_t)q))
+            return reject("pool item '" + item + "': factor " + std::to_string(q) + " is not a prime");
+        if (n == 64) return reject("pool item '" + item + "': more than 64 prime factors");
+        if (D > ~(u128)0 / q) return reject("pool item '" + item + "': D does not fit in 128 bits");
+        p[n++] = (uint32_t)q;
+        D *= q;
+        return CCS_OK;
     }
-    e.D.lo = (uint64_t)D;
-    e.D.hi = (uint64_t)(D >> 64);
-    return CCS_OK;
+    void store(const std::string& name, ccs_pool_entry& e) const {
+        std::memset(&e, 0, sizeof(e));
+        std::strncpy(e.name, name.c_str(), sizeof(e.name) - 1);
+        std::memcpy(e.primes, p, n * sizeof(uint32_t));
+        e.n_primes = n;
+        e.D.lo = (uint64_t)D;
+        e.D.hi = (uint64_t)(D >> 64);
+    }
+};
+
+std::vector<uint32_t> primes_upto(uint64_t z) {
+    std::vector<uint32_t> v;
+    for (uint32_t q = 2; q <= z; ++q)
+        if (is_prime32(q)) v.push_back(q);
+    return v;
 }
 
-// one spec: "Z#[/X/Y...]" or "P[^k]*Q[^k]*..."
-int parse_spec(const std::string& spec, ccs_pool_entry& e) {
-    std::memset(&e, 0, sizeof(e));
-    set_name(e, spec);
-    const char* s = spec.c_str();
-    const char* hash = std::strchr(s, '#');
-    if (hash) {
-        const long Z = std::strtol(s, nullptr, 10);
-        if (Z < 2 || Z > 500) return bad("bad primorial '" + spec + "'");
-        std::vector<uint32_t> ps;
-        for (uint32_t q : small_primes()) {
-            if ((long)q > Z) break;
-            if (ps.size() < 64) ps.push_back(q);
-        }
-        const char* p = hash + 1;
-        while (*p) {
-            while (*p == '/' || *p == ' ') ++p;
-            if (!*p) break;
-            char* end;
-            const unsigned long omit = std::strtoul(p, &end, 10);
-            if (end == p) return bad("malformed omit in '" + spec + "'");
-            auto it = std::find(ps.begin(), ps.end(), (uint32_t)omit);
-            if (it == ps.end()) return bad("prime " + std::to_string(omit) + " not in " + std::to_string(Z) + "# ('" + spec + "')");
-            ps.erase(it);
-            p = end;
-        }
-        for (uint32_t q : ps) e.primes[e.n_primes++] = q;
-    } else {
-        const char* p = s;
-        while (*p) {
-            while (*p == ' ' || *p == '*') ++p;
-            if (!*p) break;
-            char* end;
-            const unsigned long pr = std::strtoul(p, &end, 10);
-            if (end == p) return bad("malformed prime in '" + spec + "'");
-            long k = 1;
-            if (*end == '^') {
-                const char* q = end + 1;
-                k = std::strtol(q, &end, 10);
-                if (end == q || k < 1 || k > 16) return bad("bad exponent in '" + spec + "'");
-            }
-            for (long i = 0; i < k; ++i) {
-                if (e.n_primes >= 64) return bad("too many primes in '" + spec + "'");
-                e.primes[e.n_primes++] = (uint32_t)pr;
-            }
-            p = end;
-        }
-    }
-    return finish_entry(e, spec);
-}
-
-struct Sink {
-    ccs_pool_entry* out;
+struct Out {
+    ccs_pool_entry* dst;
     uint32_t cap, n;
-    void push(const ccs_pool_entry& e) {
-        if (n < cap) out[n] = e;
+    void put(const ccs_pool_entry& e) {
+        if (n < cap) dst[n] = e;
         ++n;
     }
 };
 
-// "auto:Z#:K" / "auto:Z#:K1-K2": every way of dropping K of the primes <= Z outside keep[],
-// in lexicographic order of the dropped primes (expand_auto_pool_one + emit_auto_combo)
-int expand_auto(const std::string& spec, const uint32_t* keep, uint32_t n_keep, Sink& sink) {
-    const char* p = spec.c_str() + 5;
-    char* end;
-    const long Z = std::strtol(p, &end, 10);
-    if (Z < 2 || Z > 500 || end == p) return bad("bad auto spec '" + spec + "' (expected auto:Z#:K)");
-    if (*end != '#') return bad("bad auto spec '" + spec + "' (expected # after Z)");
-    ++end;
-    if (*end != ':') return bad("bad auto spec '" + spec + "' (expected : after Z#)");
-    ++end;
-    const long kmin = std::strtol(end, &end, 10);
-    long kmax = kmin;
-    if (*end == '-') {
-        ++end;
-        kmax = std::strtol(end, &end, 10);
+int read_primorial(Reader& r, const std::string& item, Out& out) {
+    uint64_t z = 0;
+    if (!r.number(z) || !r.accept('#') || z < 2 || z > kMaxPrimorial)
+        return reject("pool item '" + item + "': a primorial needs 2 <= Z <= 500 before '#'");
+    std::vector<uint32_t> keep = primes_upto(z);
+    while (r.accept('/')) {
+        uint64_t q = 0;
+        if (!r.number(q)) return reject("pool item '" + item + "': expected a prime after '/'");
+        auto it = std::find(keep.begin(), keep.end(), q);
+        if (it == keep.end())
+            return reject("pool item '" + item + "': " + std::to_string(q) + " is not one of the primes <= " +
+                          std::to_string(z) + " still in the product");
+        keep.erase(it);
     }
-    if (kmin < 0 || kmax < kmin) return bad("bad K range in '" + spec + "'");
-    std::vector<uint32_t> all, drop;
-    for (uint32_t q : small_primes()) {
-        if ((long)q > Z) break;
-        all.push_back(q);
-        bool must = false;
-        for (uint32_t i = 0; i < n_keep; ++i) must |= keep[i] == q;
-        if (!must && drop.size() < 64) drop.push_back(q);
-    }
-    if (kmax > (long)drop.size()) kmax = (long)drop.size();
-    std::vector<uint32_t> chosen;
-    int rc = CCS_OK;
-    // recursive combination walk, smallest dropped primes first
-    std::function<void(size_t, long)> walk = [&](size_t start, long k_left) {
-        if (rc) return;
-        if (k_left == 0) {
-            ccs_pool_entry e;
-            std::memset(&e, 0, sizeof(e));
-            std::string name = std::to_string(Z) + "#";
-            for (uint32_t q : chosen) name += "/" + std::to_string(q);
-            set_name(e, name);
-            for (uint32_t q : all) {
-                if (std::find(chosen.begin(), chosen.end(), q) != chosen.end()) continue;
-                if (e.n_primes < 64) e.primes[e.n_primes++] = q;
-            }
-            rc = finish_entry(e, name);
-            if (!rc) sink.push(e);
-            return;
-        }
-        for (size_t i = start; i + (size_t)k_left <= drop.size(); ++i) {
-            chosen.push_back(drop[i]);
-            walk(i + 1, k_left - 1);
-            chosen.pop_back();
-            if (rc) return;
-        }
-    };
-    for (long k = kmin; k <= kmax && !rc; ++k) walk(0, k);
-    return rc;
+    if (!r.at_end()) return reject("pool item '" + item + "': unexpected text after the primorial");
+    Factors f;
+    for (uint32_t q : keep)
+        if (int rc = f.add(q, item)) return rc;
+    ccs_pool_entry e;
+    f.store(item, e);
+    out.put(e);
+    return CCS_OK;
 }
 
-int parse_list(const std::string& list, const uint32_t* keep, uint32_t n_keep, Sink& sink) {
-    size_t pos = 0;
-    while (pos <= list.size()) {
-        size_t comma = list.find(',', pos);
-        if (comma == std::string::npos) comma = list.size();
-        std::string tok = list.substr(pos, comma - pos);
-        pos = comma + 1;
-        size_t b = 0;
-        while (b < tok.size() && tok[b] == ' ') ++b;
-        tok = tok.substr(b);
-        while (!tok.empty() && (tok.back() == ' ' || tok.back() == '\n')) tok.pop_back();
-        if (tok.empty()) continue;
-        int rc;
-        if (tok.compare(0, 5, "auto:") == 0) {
-            rc = expand_auto(tok, keep, n_keep, sink);
-        } else {
-            ccs_pool_entry e;
-            rc = parse_spec(tok, e);
-            if (!rc) sink.push(e);
-        }
-        if (rc) return rc;
-    }
+int read_product(Reader& r, const std::string& item, Out& out) {
+    Factors f;
+    do {
+        uint64_t q = 0, k = 1;
+        if (!r.number(q)) return reject("pool item '" + item + "': expected a prime factor");
+        if (r.accept('^') && (!r.number(k) || k < 1 || k > kMaxPower))
+            return reject("pool item '" + item + "': an exponent must be 1..16");
+        for (uint64_t m = 0; m < k; ++m)
+            if (int rc = f.add(q, item)) return rc;
+    } while (r.accept('*'));
+    if (!r.at_end()) return reject("pool item '" + item + "': expected '*' between factors");
+    ccs_pool_entry e;
+    f.store(item, e);
+    out.put(e);
     return CCS_OK;
+}
+
+// every subset of `pool` of size k, in lexicographic order of the positions, left out of
+// `all`; the entries are named "Z#/q1/q2..."
+int emit_subsets(const std::vector<uint32_t>& all, const std::vector<uint32_t>& pool, size_t k, uint64_t z,
+                 Out& out) {
+    std::vector<size_t> pick(k);
+    for (size_t i = 0; i < k; ++i) pick[i] = i;
+    for (;;) {
+        std::string name = std::to_string(z) + "#";
+        std::vector<uint32_t> left;
+        for (size_t i : pick) {
+            left.push_back(pool[i]);
+            name += "/" + std::to_string(pool[i]);
+        }
+        Factors f;
+        for (uint32_t q : all)
+            if (std::find(left.begin(), left.end(), q) == left.end())
+                if (int rc = f.add(q, name)) return rc;
+        ccs_pool_entry e;
+        f.store(name, e);
+        out.put(e);
+        // next subset: advance the last position that can still move right
+        size_t i = k;
+        while (i > 0 && pick[i - 1] == pool.size() - k + (i - 1)) --i;
+        if (i == 0) return CCS_OK;
+        ++pick[i - 1];
+        for (size_t m = i; m < k; ++m) pick[m] = pick[m - 1] + 1;
+    }
+}
+
+int read_auto(Reader& r, const std::string& item, const uint32_t* keep, uint32_t n_keep, Out& out) {
+    uint64_t z = 0, k1 = 0, k2 = 0;
+    if (!r.number(z) || z < 2 || z > kMaxPrimorial || !r.accept('#') || !r.accept(':'))
+        return reject("pool item '" + item + "': the form is auto:Z#:k or auto:Z#:k1-k2 with 2 <= Z <= 500");
+    if (!r.number(k1)) return reject("pool item '" + item + "': expected the number of primes to leave out");
+    k2 = k1;
+    if (r.accept('-') && !r.number(k2)) return reject("pool item '" + item + "': expected k2 after '-'");
+    if (k2 < k1) return reject("pool item '" + item + "': k2 is below k1");
+    if (!r.at_end()) return reject("pool item '" + item + "': unexpected text after the auto range");
+    const std::vector<uint32_t> all = primes_upto(z);
+    std::vector<uint32_t> pool;   // the primes that may be left out
+    for (uint32_t q : all)
+        if (std::find(keep, keep + n_keep, q) == keep + n_keep) pool.push_back(q);
+    k2 = std::min<uint64_t>(k2, pool.size());
+    for (uint64_t k = k1; k <= k2; ++k)
+        if (int rc = emit_subsets(all, pool, (size_t)k, z, out)) return rc;
+    return CCS_OK;
+}
+
+int read_list(const std::string& text, const uint32_t* keep, uint32_t n_keep, Out& out) {
+    size_t from = 0;
+    for (;;) {
+        const size_t comma = std::min(text.find(',', from), text.size());
+        size_t a = from, b = comma;
+        while (a < b && (text[a] == ' ' || text[a] == '\t')) ++a;
+        while (b > a && (text[b - 1] == ' ' || text[b - 1] == '\t' || text[b - 1] == '\r' || text[b - 1] == '\n')) --b;
+        if (b > a) {
+            const std::string item = text.substr(a, b - a);
+            Reader r(item);
+            int rc;
+            if (r.accept_word("auto:")) rc = read_auto(r, item, keep, n_keep, out);
+            else if (r.starts_primorial()) rc = read_primorial(r, item, out);
+            else rc = read_product(r, item, out);
+            if (rc) return rc;
+        }
+        if (comma >= text.size()) return CCS_OK;
+        from = comma + 1;
+    }
 }
 
 }  // namespace
 
 extern "C" int ccs_pool_parse(const char* list, const uint32_t* keep, uint32_t n_keep, ccs_pool_entry* out,
                               uint32_t cap, uint32_t* n_out) {
-    if (!list || !n_out || (cap && !out) || (n_keep && !keep)) return bad("null argument");
-    Sink sink{out, cap, 0};
-    const int rc = parse_list(list, keep, n_keep, sink);
-    *n_out = sink.n;
+    if (!list || !n_out || (cap && !out) || (n_keep && !keep)) return reject("null argument");
+    Out o{out, cap, 0};
+    const int rc = read_list(list, keep, n_keep, o);
+    *n_out = o.n;
     if (rc) return rc;
-    return sink.n > cap ? CCS_OVERFLOW : CCS_OK;
+    return o.n > cap ? CCS_OVERFLOW : CCS_OK;
 }
 
+// A pool file holds one list per line; from a line's first '#' on is a comment, so a file
+// lists products only (the primorial form reads as its number).
 extern "C" int ccs_pool_load(const char* path, const uint32_t* keep, uint32_t n_keep, ccs_pool_entry* out,
                              uint32_t cap, uint32_t* n_out) {
-    if (!path || !n_out || (cap && !out) || (n_keep && !keep)) return bad("null argument");
-    FILE* fp = std::fopen(path, "r");
-    if (!fp) return bad(std::string("cannot open pool file '") + path + "'");
-    Sink sink{out, cap, 0};
-    char line[1024];
-    int lineno = 0, rc = CCS_OK;
-    while (!rc && std::fgets(line, sizeof(line), fp)) {
-        ++lineno;
-        char* hash = std::strchr(line, '#');
-        if (hash) *hash = '\0';
-        size_t L = std::strlen(line);
-        while (L > 0 && (line[L - 1] == '\n' || line[L - 1] == '\r' || line[L - 1] == ' ' || line[L - 1] == '\t'))
-            line[--L] = '\0';
-        char* tok = line;
-        while (*tok == ' ' || *tok == '\t') ++tok;
-        if (!*tok) continue;
-        rc = parse_list(tok, keep, n_keep, sink);
-        if (rc) {
-            const std::string msg = std::string(ccs_last_error()) + " (" + path + " line " + std::to_string(lineno) + ")";
-            rc = bad(msg);
+    if (!path || !n_out || (cap && !out) || (n_keep && !keep)) return reject("null argument");
+    std::ifstream in(path);
+    if (!in) return reject(std::string("pool file ") + path + ": cannot be read");
+    Out o{out, cap, 0};
+    std::string line;
+    for (uint64_t no = 1; std::getline(in, line); ++no) {
+        const size_t c = line.find('#');
+        if (c != std::string::npos) line.resize(c);
+        if (read_list(line, keep, n_keep, o) != CCS_OK) {
+            *n_out = o.n;
+            return reject(std::string(path) + ":" + std::to_string(no) + ": " + ccs_last_error());
         }
     }
-    std::fclose(fp);
-    *n_out = sink.n;
-    if (rc) return rc;
-    return sink.n > cap ? CCS_OVERFLOW : CCS_OK;
+    *n_out = o.n;
+    return o.n > cap ? CCS_OVERFLOW : CCS_OK;
 }
 
 extern "C" int ccs_family_window(ccs_u128 Dv, int32_t E, int32_t bits_lo, int32_t bits_hi, ccs_u128* q_min,
                                  ccs_u128* q_max) {
-    if (!q_min || !q_max) return bad("null argument");
-    if (bits_lo < 2 || bits_hi < bits_lo || bits_hi > 127) return bad("bits must satisfy 2 <= bits_lo <= bits_hi <= 127");
-    if (E < 0 || E > 126) return bad("exp_start must be in 0..126");
+    if (!q_min || !q_max) return reject("null argument");
+    if (bits_lo < 2 || bits_hi < bits_lo || bits_hi > 127) return reject("bits must satisfy 2 <= bits_lo <= bits_hi <= 127");
+    if (E < 0 || E > 126) return reject("exp_start must be in 0..126");
     const u128 D = ((u128)Dv.hi << 64) | Dv.lo;
-    if (D == 0) return bad("D must be >= 1");
-    if (D >> (127 - E)) return bad("D*2^E must be below 2^127");
+    if (D == 0) return reject("D must be >= 1");
+    if (D >> (127 - E)) return reject("D*2^E must be below 2^127");
     const u128 DE = D << E;
     const u128 lower = (u128)1 << (bits_lo - 1), upper_m1 = ((u128)1 << bits_hi) - 1;
     const u128 qmin = (lower + DE - 1) / DE, qmax = upper_m1 / DE;
@@ -263,7 +289,7 @@ extern "C" int ccs_family_window(ccs_u128 Dv, int32_t E, int32_t bits_lo, int32_
 }
 
 extern "C" int ccs_dedup_u128(const ccs_u128* values, uint32_t count, uint8_t* first) {
-    if (count && (!values || !first)) return bad("null argument");
+    if (count && (!values || !first)) return reject("null argument");
     std::vector<uint32_t> idx(count);
     for (uint32_t i = 0; i < count; ++i) idx[i] = i;
     auto key = [&](uint32_t i) { return ((u128)values[i].hi << 64) | values[i].lo; };
