@@ -73,6 +73,19 @@ class Context:
     def wait(self, stream, ev):
         L.check(self.lib.ccs_stream_wait_event(self.h, stream, ev), "ccs_stream_wait_event")
 
+    def status(self, stream=None):
+        """stage-B overflow status of the stream's launches since the last call
+        (ccs_stream_status): a dict, with "overflow" True when some launch overflowed"""
+        st = L.LaunchStatus()
+        rc = self.lib.ccs_stream_status(self.h, stream, C.byref(st))
+        if rc < 0:
+            L.check(rc, "ccs_stream_status")
+        return dict(st.as_dict(), overflow=rc == L.CCS_OVERFLOW)
+
+    def limit_stage_b(self, entries):
+        """test knob: launches use at most `entries` of their stage-B buffer (0 = all)"""
+        L.check(self.lib.ccs_debug_limit_stage_b(self.h, entries), "ccs_debug_limit_stage_b")
+
     def profile_start(self):
         L.check(self.lib.ccs_profile_start(self.h), "ccs_profile_start")
 
@@ -406,6 +419,14 @@ class JobSet:
         L.check(self.lib.ccs_jobset_stats(self.ctx.h, self.h, d_count, C.byref(st)), "jobset_stats")
         return st.as_dict()
 
+    def status(self, stream=None):
+        """stage-B overflow status of the set's launches since the last call (ccs_jobset_status)"""
+        st = L.LaunchStatus()
+        rc = self.lib.ccs_jobset_status(self.ctx.h, self.h, stream, C.byref(st))
+        if rc < 0:
+            L.check(rc, "ccs_jobset_status")
+        return dict(st.as_dict(), overflow=rc == L.CCS_OVERFLOW)
+
     def graph(self, d_out, cap, d_count):
         g = C.c_void_p()
         L.check(self.lib.ccs_graph_create(self.ctx.h, self.h, d_out, cap, d_count, C.byref(g)), "graph_create")
@@ -455,19 +476,23 @@ class GroupSet(JobSet):
 
 class Server:
     """Continuous-batching request loop (ccs_server): submit jobs of any depth / width /
-    length, poll completions with their timestamps, fetch per-job results."""
+    length, poll completions with their timestamps, fetch per-job results.  Each admitted
+    job's sieve phases stay resident on the device (recompute=True turns that off: the
+    comparison mode)."""
 
-    def __init__(self, ctx, shapes_by_depth, mem_budget_bytes=256 << 20, max_active=1024, surv_per_job=16,
+    def __init__(self, ctx, shapes_by_depth, mem_budget_bytes=256 << 20, max_active=1024,
                  max_slice_survivors=1 << 16, slice_candidates=1 << 35, min_chunk=1 << 18,
-                 max_chunk=1 << 32, bits=0, test="bpsw", bases=(2,)):
+                 max_chunk=1 << 32, bits=0, test="bpsw", bases=(2,), recompute=False):
         self.ctx, self.lib = ctx, ctx.lib
         self.depths = sorted(shapes_by_depth)
         self.shapes = [shapes_by_depth[d] for d in self.depths]
         arr = (C.c_void_p * len(self.shapes))(*[sh.h.value for sh in self.shapes])
         dep = (C.c_int32 * len(self.depths))(*self.depths)
-        p = L.ServerParams(mem_budget_bytes, max_active, surv_per_job, max_slice_survivors, 0,
+        flags = L.SERVER_RECOMPUTE_PHASES if recompute else 0
+        p = L.ServerParams(mem_budget_bytes, max_active, flags, max_slice_survivors, 0,
                            slice_candidates, min_chunk, max_chunk, _prove_params(bits, test, bases))
         self.h = C.c_void_p()
+        self.incomplete = False
         L.check(self.lib.ccs_server_create(ctx.h, arr, dep, len(self.shapes), C.byref(p), C.byref(self.h)),
                 "ccs_server_create")
 
@@ -478,22 +503,32 @@ class Server:
         return jid.value
 
     def poll(self, cap=4096):
+        """finished jobs since the last poll (each with its flags; JOB_INCOMPLETE marks a job
+        whose survivors were cut by a full buffer)"""
         buf = (L.JobDone * cap)()
         n = C.c_uint32(0)
-        L.check(self.lib.ccs_server_poll(self.h, buf, cap, C.byref(n)), "ccs_server_poll")
+        rc = self.lib.ccs_server_poll(self.h, buf, cap, C.byref(n))
+        if rc < 0:
+            L.check(rc, "ccs_server_poll")
+        self.incomplete |= rc == L.CCS_OVERFLOW
         return [buf[i].as_dict() for i in range(n.value)]
 
     def results(self, job_id):
+        """(u_off uint64 array, chain results) of a finished job"""
         n = C.c_uint32(0)
         L.check(self.lib.ccs_server_job_results(self.h, job_id, None, None, 0, C.byref(n)), "results")
-        surv = np.zeros(n.value, JOB_SURVIVOR_DTYPE)
+        u_off = np.zeros(n.value, np.uint64)
         res = np.zeros(n.value, CHAIN_RESULT_DTYPE)
-        L.check(self.lib.ccs_server_job_results(self.h, job_id, surv.ctypes.data, res.ctypes.data, n.value,
+        L.check(self.lib.ccs_server_job_results(self.h, job_id, u_off.ctypes.data, res.ctypes.data, n.value,
                                                 C.byref(n)), "results")
-        return surv, res
+        return u_off, res
 
     def drain(self):
-        L.check(self.lib.ccs_server_drain(self.h), "ccs_server_drain")
+        """wait for every submitted job; True when every job so far finished complete"""
+        rc = self.lib.ccs_server_drain(self.h)
+        if rc < 0:
+            L.check(rc, "ccs_server_drain")
+        return rc == L.CCS_OK
 
     def stats(self):
         c = L.ServerCounters()

@@ -38,7 +38,8 @@ EXPORTS = [
     "ccs_campaign_free", "ccs_campaign_next", "ccs_campaign_record", "ccs_campaign_get_state",
     "ccs_campaign_save", "ccs_campaign_resume", "ccs_family_shape_build", "ccs_family_window",
     "ccs_pool_parse", "ccs_pool_load", "ccs_dedup_u128", "ccs_family_top_gate_launch",
-    "ccs_groupset_create", "ccs_groupset_update",
+    "ccs_groupset_create", "ccs_groupset_update", "ccs_stream_status", "ccs_jobset_status",
+    "ccs_debug_limit_stage_b",
 ]
 
 
@@ -108,6 +109,14 @@ class FilterStats(C.Structure):
                 for f, t in self._fields_}
 
 
+class LaunchStatus(C.Structure):
+    _fields_ = [("overflowed_launches", C.c_uint32), ("worst_stage_b", C.c_uint32), ("dropped", C.c_uint64),
+                ("stage_b_cap", C.c_uint32), ("_pad", C.c_uint32)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_ if f != "_pad"}
+
+
 class ProveParams(C.Structure):
     _fields_ = [("bits", C.c_int32), ("test_kind", C.c_int32), ("n_bases", C.c_uint32),
                 ("bases", C.c_uint32 * 16)]
@@ -169,9 +178,13 @@ class KernelTimes(C.Structure):
 
 
 class ServerParams(C.Structure):
-    _fields_ = [("mem_budget_bytes", C.c_uint64), ("max_active", C.c_uint32), ("surv_per_job", C.c_uint32),
+    _fields_ = [("mem_budget_bytes", C.c_uint64), ("max_active", C.c_uint32), ("flags", C.c_uint32),
                 ("max_slice_survivors", C.c_uint32), ("pad", C.c_uint32), ("slice_candidates", C.c_uint64),
                 ("min_chunk", C.c_uint64), ("max_chunk", C.c_uint64), ("prove", ProveParams)]
+
+
+SERVER_RECOMPUTE_PHASES = 1
+JOB_INCOMPLETE = 1
 
 
 class Request(C.Structure):
@@ -182,18 +195,20 @@ class Request(C.Structure):
 class JobDone(C.Structure):
     _fields_ = [("job_id", C.c_uint64), ("tag", C.c_uint64), ("candidates", C.c_uint64),
                 ("n_survivors", C.c_uint32), ("n_primes", C.c_uint32), ("best_len", C.c_uint32),
-                ("pad", C.c_uint32), ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+                ("flags", C.c_uint32), ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
 
     def as_dict(self):
-        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class ServerCounters(C.Structure):
     _fields_ = [("candidates", C.c_uint64), ("slices", C.c_uint64), ("launches", C.c_uint64),
-                ("resident_bytes", C.c_uint64), ("active", C.c_uint32), ("pending", C.c_uint32)]
+                ("resident_bytes", C.c_uint64), ("active", C.c_uint32), ("pending", C.c_uint32),
+                ("job_bytes", C.c_uint64), ("fixed_bytes", C.c_uint64), ("slots", C.c_uint32),
+                ("incomplete", C.c_uint32), ("busy_s", C.c_double)]
 
     def as_dict(self):
-        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+        return {f: (float(getattr(self, f)) if f == "busy_s" else int(getattr(self, f))) for f, _ in self._fields_}
 
 
 class Job(C.Structure):
@@ -254,6 +269,8 @@ def load():
     L.ccs_filter.argtypes = [vp, vp, U128, C.c_uint32, vp, C.c_uint32, C.POINTER(C.c_uint32),
                              C.POINTER(FilterStats)]
     L.ccs_last_stats.argtypes = [vp, C.POINTER(FilterStats)]
+    L.ccs_stream_status.argtypes = [vp, vp, C.POINTER(LaunchStatus)]
+    L.ccs_debug_limit_stage_b.argtypes = [vp, C.c_uint32]
     L.ccs_prove_launch.argtypes = [vp, vp, U128, vp, vp, C.c_uint32, C.POINTER(ProveParams), vp, vp]
     L.ccs_prove.argtypes = [vp, vp, U128, vp, C.c_uint32, C.POINTER(ProveParams), vp]
     L.ccs_confirm_values.argtypes = [vp, C.c_uint32, C.POINTER(ProveParams), C.c_uint32, vp,
@@ -284,6 +301,7 @@ def load():
     L.ccs_jobset_launch.argtypes = [vp, vp, vp, C.c_uint32, vp, vp]
     L.ccs_jobset_launch_each.argtypes = [vp, vp, vp, C.c_uint32, vp, vp]
     L.ccs_jobset_stats.argtypes = [vp, vp, vp, C.POINTER(FilterStats)]
+    L.ccs_jobset_status.argtypes = [vp, vp, vp, C.POINTER(LaunchStatus)]
     L.ccs_graph_create.argtypes = [vp, vp, vp, C.c_uint32, vp, C.POINTER(vp)]
     L.ccs_graph_launch.argtypes = [vp, vp, vp]
     L.ccs_graph_free.argtypes = [vp]

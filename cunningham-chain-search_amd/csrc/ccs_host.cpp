@@ -58,7 +58,7 @@ struct StreamScratch {
     hipStream_t st = nullptr;
     MidRec* mid = nullptr;
     u32 mid_cap = 0;
-    u32* counters = nullptr;            // [0] mid_count [1] out_count [2..3] unit counter (u64)
+    u32* counters = nullptr;            // launch counters + sticky status (kLaunchWords / kStatusWord)
     unsigned long long* stats = nullptr;   // [0] dense alive [1] stage-B tested
     void* chain = nullptr;              // prove chain queue: [0] count, [16..] indices
     u64 chain_bytes = 0;
@@ -86,6 +86,7 @@ struct ccs_ctx {
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<int, size_t>> marks;   // (kind, index of the begin event)
+    u32 stage_b_limit = 0;              // test knob: stage-B buffer entries used (0 = all)
 };
 
 // begin/end events of one kernel launch while profiling (kind: 0 sieve, 1 deep, 2 prove)
@@ -133,7 +134,8 @@ static StreamScratch* stream_scratch(ccs_ctx* c, hipStream_t st) {
         if (x.st == st) return &x;
     StreamScratch x;
     x.st = st;
-    if (hipMalloc((void**)&x.counters, 64) != hipSuccess || hipMalloc((void**)&x.stats, 64) != hipSuccess) {
+    if (hipMalloc((void**)&x.counters, 64) != hipSuccess || hipMalloc((void**)&x.stats, 64) != hipSuccess ||
+        hipMemset(x.counters, 0, 64) != hipSuccess) {
         free_scratch(x);
         fail(CCS_ERR_NOMEM, "launch counters");
         return nullptr;
@@ -974,6 +976,12 @@ struct Scratch {
 
 static Scratch to_scratch(const StreamScratch* x) { return Scratch{x->mid, x->mid_cap, x->counters, x->stats}; }
 
+// the stage-B capacity a launch may use: all of the buffer, or less under the test knob
+static Scratch limited(const ccs_ctx* c, Scratch x) {
+    if (c->stage_b_limit && c->stage_b_limit < x.mid_cap) x.mid_cap = c->stage_b_limit;
+    return x;
+}
+
 static ShapeTab shape_tab(const ccs_shape* s) {
     ShapeTab t;
     t.lds_image = s->d_lds_image;
@@ -1009,6 +1017,8 @@ static void fill_sieve_common(const ccs_shape* s, const Scratch& x, SieveArgs& a
     a.lds_ctl = (s->lds_words + 1) & ~1u;
     a.one = 1;
     a.unit_j = s->tune.unit_j;
+    a.phases = nullptr;
+    a.ph_stride = 0;
     a.unit_counter = (u64*)(x.counters + 2);
     a.mid = x.mid;
     a.mid_cap = x.mid_cap;
@@ -1022,9 +1032,10 @@ static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, 
                               u64 n_units, u32* d_count, hipStream_t st, bool zero_out = true,
                               bool zero_stats = true, u32 nb_max = 0) {
     // fill kernels rather than memset nodes, so the sequence can be graph-captured
-    if (launch_zero3(x.counters, 16, zero_stats ? (u32*)x.stats : nullptr, 16, zero_out ? d_count : nullptr, 1,
-                     st) != hipSuccess)
+    if (launch_zero3(x.counters, kLaunchWords, zero_stats ? (u32*)x.stats : nullptr, 16,
+                     zero_out ? d_count : nullptr, 1, st) != hipSuccess)
         return fail(CCS_ERR_HIP, "zero");
+    if (n_units >> 32) return fail(CCS_ERR_INVALID, "a launch takes fewer than 2^32 work units");
     sa.n_units = n_units;
     // a group launch takes its stage-B register width from its widest shape (nb_max)
     const u32 nbm = nb_max > s->nb ? nb_max : s->nb;
@@ -1047,6 +1058,7 @@ static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, 
     da.mid_cap = x.mid_cap;
     da.out_count = d_count;
     da.stats = x.stats;
+    da.status = x.counters + kStatusWord;
     prof_begin(c, 1, st);
     e = launch_deep(da, x.mid_cap, st);
     if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("deep launch: ") + hipGetErrorString(e));
@@ -1065,12 +1077,13 @@ static int filter_launch_impl(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint3
     const u64 jmax = ((u64)tile_count + off + 5) / 6;
     const u64 cpc = (jmax + s->tune.unit_j - 1) / s->tune.unit_j;
     const u64 n_units = cpc * s->classes.size();
+    if (n_units >> 32) return fail(CCS_ERR_INVALID, "launch has 2^32 work units or more: fewer tiles per launch");
     const u64 cand = ccs_count_candidates(s, ts, tile_count);
     StreamScratch* ss = stream_scratch(c, st);
     if (!ss) return CCS_ERR_NOMEM;
     int rc = ensure_mid(ss, (u64)(cand * s->est_survival_b * 2.0) + (1u << 20));
     if (rc) return rc;
-    const Scratch x = to_scratch(ss);
+    const Scratch x = limited(c, to_scratch(ss));
     SieveArgs sa;
     fill_sieve_common(s, x, sa);
     sa.mode = 0;
@@ -1154,10 +1167,40 @@ extern "C" int ccs_filter(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint32_t 
     const u32 keep = std::min(n, cap);
     if (keep && h_out) HIPCHK(hipMemcpy(h_out, c->out, (size_t)keep * sizeof(ccs_survivor), hipMemcpyDeviceToHost));
     *h_count = n;
-    rc = collect_stats(c, n, to_scratch(stream_scratch(c, c->stream)));
+    rc = collect_stats(c, n, limited(c, to_scratch(stream_scratch(c, c->stream))));
     if (stats) *stats = c->last;
     if (rc) return rc;
     return n > cap ? CCS_OVERFLOW : CCS_OK;
+}
+
+// read and clear the sticky overflow status of a scratch, once `st` has finished
+static int read_status(ccs_ctx* c, const Scratch& x, hipStream_t st, ccs_launch_status* out) {
+    HIPCHK(hipStreamSynchronize(st));
+    u32 w[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpy(w, x.counters + kStatusWord, sizeof(w), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemset(x.counters + kStatusWord, 0, sizeof(w)));
+    std::memset(out, 0, sizeof(*out));
+    out->overflowed_launches = w[0];
+    out->worst_stage_b = w[1];
+    out->dropped = (u64)w[2] | ((u64)w[3] << 32);
+    out->stage_b_cap = limited(c, x).mid_cap;
+    return w[0] ? CCS_OVERFLOW : CCS_OK;
+}
+
+extern "C" int ccs_stream_status(ccs_ctx* c, void* stream, ccs_launch_status* out) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!out) return fail(CCS_ERR_INVALID, "null argument");
+    const hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    StreamScratch* ss = stream_scratch(c, st);
+    if (!ss) return CCS_ERR_NOMEM;
+    return read_status(c, to_scratch(ss), st, out);
+}
+
+extern "C" int ccs_debug_limit_stage_b(ccs_ctx* c, uint32_t entries) {
+    if (!c) return fail(CCS_ERR_INVALID, "null context");
+    c->stage_b_limit = entries;
+    return CCS_OK;
 }
 
 extern "C" int ccs_last_stats(ccs_ctx* c, ccs_filter_stats* out) {
@@ -1350,6 +1393,8 @@ struct ccs_jobset {
     u64* staging_units = nullptr;
     std::vector<u32> job_shape;
     u32 lds_ctl = 0, nb_max = 0;
+    const u32* d_phases = nullptr;  // resident phase arena of the jobs (served form), or null
+    u32 ph_stride = 0;
 };
 
 extern "C" void ccs_jobset_free(ccs_jobset* js) {
@@ -1403,6 +1448,9 @@ static int job_record(const ccs_shape* s, const ccs_job& j, JobRec& r) {
     r.W_hi = s->W_hi;
     r.shape = 0;
     r.unit_j = 0;
+    r.ph_off = 0;
+    r.ph_slot = kNoSlot;
+    r.pad = 0;
     return CCS_OK;
 }
 
@@ -1463,6 +1511,7 @@ extern "C" int ccs_jobset_create(ccs_ctx* c, const ccs_shape* s, const ccs_job* 
     if (e == hipSuccess) e = hipMalloc((void**)&js->d_prefix, ((size_t)n_jobs + 1) * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&js->x.mid, want * sizeof(MidRec));
     if (e == hipSuccess) e = hipMalloc((void**)&js->x.counters, 64);
+    if (e == hipSuccess) e = hipMemset(js->x.counters, 0, 64);
     if (e == hipSuccess) e = hipMalloc((void**)&js->x.stats, 64);
     if (e == hipSuccess) e = hipMemcpy(js->d_jobs, jr.data(), jr.size() * sizeof(JobRec), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(js->d_prefix, js->prefix.data(), js->prefix.size() * 8, hipMemcpyHostToDevice);
@@ -1585,6 +1634,7 @@ extern "C" int ccs_groupset_create(ccs_ctx* c, const ccs_shape* const* shapes, u
     if (e == hipSuccess) e = hipMalloc((void**)&js->d_shape_units, ((size_t)n_shapes + 1) * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&js->x.mid, want * sizeof(MidRec));
     if (e == hipSuccess) e = hipMalloc((void**)&js->x.counters, 64);
+    if (e == hipSuccess) e = hipMemset(js->x.counters, 0, 64);
     if (e == hipSuccess) e = hipMalloc((void**)&js->x.stats, 64);
     if (e == hipSuccess) e = hipMemcpy(js->d_jobs, jr.data(), jr.size() * sizeof(JobRec), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(js->d_prefix, js->prefix.data(), js->prefix.size() * 8, hipMemcpyHostToDevice);
@@ -1601,11 +1651,12 @@ extern "C" int ccs_groupset_create(ccs_ctx* c, const ccs_shape* const* shapes, u
 }
 
 // new jobs (at most as many as the set was created with) for the same shapes; asynchronous
-// on `stream` like ccs_jobset_update
-extern "C" int ccs_groupset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jobs, const uint32_t* job_shape,
-                                   uint32_t n_jobs, void* stream) {
-    int rc = ctx_check(c);
-    if (rc) return rc;
+// on `st`.  With slots, job i reads its phases from slot slots[i] of the arena d_phases,
+// where they are the phases at u0_i - offs[i].
+static int groupset_update_impl(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jobs, const uint32_t* job_shape,
+                                const uint32_t* slots, const uint64_t* offs, uint32_t n_jobs, const u32* d_phases,
+                                u32 stride, hipStream_t st) {
+    int rc;
     if (!js || !h_jobs || !n_jobs) return fail(CCS_ERR_INVALID, "null argument or no jobs");
     if (js->group.empty()) return fail(CCS_ERR_INVALID, "not a group set (use ccs_jobset_update)");
     if (n_jobs > js->cap_jobs) return fail(CCS_ERR_INVALID, "more jobs than the set was created with");
@@ -1634,7 +1685,13 @@ extern "C" int ccs_groupset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_
     for (u32 i = 0; i < n_jobs; ++i) {
         (void)job_record(js->group[job_shape[i]], h_jobs[i], js->staging[i]);
         js->staging[i].shape = job_shape[i];
+        if (slots) {
+            js->staging[i].ph_slot = slots[i];
+            js->staging[i].ph_off = offs[i];
+        }
     }
+    js->d_phases = slots ? d_phases : nullptr;
+    js->ph_stride = slots ? stride : 0;
     group_units_per_job(c, js, h_jobs, n_jobs, js->staging);
     u64 units = 0, cand = 0;
     for (u32 i = 0; i < n_jobs; ++i) {
@@ -1651,13 +1708,33 @@ extern "C" int ccs_groupset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_
     std::vector<u64> su;
     group_units(js, su);
     std::memcpy(js->staging_units, su.data(), su.size() * 8);
-    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
     HIPCHK(hipMemcpyAsync(js->d_jobs, js->staging, (size_t)n_jobs * sizeof(JobRec), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(js->d_prefix, js->staging_prefix, ((size_t)n_jobs + 1) * 8, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(js->d_shape_units, js->staging_units, su.size() * 8, hipMemcpyHostToDevice, st));
     HIPCHK(hipEventRecord(js->staged, st));
     return CCS_OK;
 }
+
+extern "C" int ccs_groupset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jobs, const uint32_t* job_shape,
+                                   uint32_t n_jobs, void* stream) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    return groupset_update_impl(c, js, h_jobs, job_shape, nullptr, nullptr, n_jobs, nullptr, 0,
+                                stream ? (hipStream_t)stream : c->stream);
+}
+
+namespace ccs {
+u32 shape_phase_words(const ccs_shape* s) { return s->nd + s->nb; }
+const PrimeMod* shape_pmod(const ccs_shape* s) { return s->d_pmod; }
+int groupset_update_resident(ccs_ctx* c, ccs_jobset* js, const ccs_job* jobs, const uint32_t* job_shape,
+                             const uint32_t* slots, const uint64_t* offs, uint32_t n_jobs, const u32* d_phases,
+                             u32 stride, hipStream_t st) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!slots || !offs || !d_phases) return fail(CCS_ERR_INVALID, "null argument");
+    return groupset_update_impl(c, js, jobs, job_shape, slots, offs, n_jobs, d_phases, stride, st);
+}
+}  // namespace ccs
 
 // replace the jobs of a set (n_jobs <= the count it was created with; same shape) with an
 // asynchronous upload on `stream`; h_jobs must stay valid until the stream reaches it
@@ -1686,6 +1763,8 @@ extern "C" int ccs_jobset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jo
         return fail(CCS_ERR_INVALID, "jobs exceed the stage-B buffer of the set");
     js->prefix.resize((size_t)n_jobs + 1);
     js->unit_j = set_unit(c, s, h_jobs, n_jobs);
+    js->d_phases = nullptr;
+    js->ph_stride = 0;
     js->counts.resize(n_jobs);
     js->hjobs.resize(n_jobs);
     js->each_valid = false;
@@ -1707,10 +1786,12 @@ extern "C" int ccs_jobset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_jo
     return CCS_OK;
 }
 
-static void jobset_args(const ccs_jobset* js, SieveArgs& sa, DeepArgs& da, void* d_out, u32 cap) {
+static void jobset_args(const ccs_ctx* c, const ccs_jobset* js, SieveArgs& sa, DeepArgs& da, void* d_out, u32 cap) {
     const ccs_shape* s = js->shape;
-    fill_sieve_common(s, js->x, sa);
+    fill_sieve_common(s, limited(c, js->x), sa);
     sa.unit_j = js->unit_j;
+    sa.phases = js->group.empty() ? nullptr : js->d_phases;   // (resident jobs are group-set jobs)
+    sa.ph_stride = js->ph_stride;
     sa.mode = 1;
     sa.jobsrc.jobs = js->d_jobs;
     sa.jobsrc.unit_prefix = js->d_prefix;
@@ -1751,9 +1832,9 @@ extern "C" int ccs_jobset_launch(ccs_ctx* c, ccs_jobset* js, ccs_job_survivor* d
     if (!js || !d_out || !d_count) return fail(CCS_ERR_INVALID, "null argument");
     SieveArgs sa;
     DeepArgs da;
-    jobset_args(js, sa, da, d_out, cap);
+    jobset_args(c, js, sa, da, d_out, cap);
     jobset_stats_begin(c, js);
-    return run_filter_kernels(c, js->shape, js->x, sa, da, js->n_units, d_count,
+    return run_filter_kernels(c, js->shape, limited(c, js->x), sa, da, js->n_units, d_count,
                               stream ? (hipStream_t)stream : c->stream, true, true, js->nb_max);
 }
 
@@ -1788,7 +1869,7 @@ extern "C" int ccs_jobset_launch_each(ccs_ctx* c, ccs_jobset* js, ccs_job_surviv
         const u64 units = (js->counts[i] + js->each_unit[i] - 1) / js->each_unit[i];
         SieveArgs sa;
         DeepArgs da;
-        jobset_args(js, sa, da, d_out, cap);   // the deep kernel keeps the whole job table
+        jobset_args(c, js, sa, da, d_out, cap);   // the deep kernel keeps the whole job table
         sa.unit_j = js->each_unit[i];
         sa.jobsrc.jobs = js->d_jobs + i;
         sa.jobsrc.unit_prefix = js->d_each + 2 * (size_t)i;
@@ -1800,10 +1881,17 @@ extern "C" int ccs_jobset_launch_each(ccs_ctx* c, ccs_jobset* js, ccs_job_surviv
             sa.n_group = 1;
             sa.tab = shape_tab(sh);
         }
-        rc = run_filter_kernels(c, js->shape, js->x, sa, da, units, d_count, st, false, false, js->nb_max);
+        rc = run_filter_kernels(c, js->shape, limited(c, js->x), sa, da, units, d_count, st, false, false, js->nb_max);
         if (rc) return rc;
     }
     return CCS_OK;
+}
+
+extern "C" int ccs_jobset_status(ccs_ctx* c, ccs_jobset* js, void* stream, ccs_launch_status* out) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!js || !out) return fail(CCS_ERR_INVALID, "null argument");
+    return read_status(c, js->x, stream ? (hipStream_t)stream : c->stream, out);
 }
 
 extern "C" int ccs_jobset_stats(ccs_ctx* c, ccs_jobset* js, uint32_t* d_count, ccs_filter_stats* out) {
@@ -1814,7 +1902,7 @@ extern "C" int ccs_jobset_stats(ccs_ctx* c, ccs_jobset* js, uint32_t* d_count, c
     u32 n = 0;
     if (d_count) HIPCHK(hipMemcpy(&n, d_count, 4, hipMemcpyDeviceToHost));
     jobset_stats_begin(c, js);
-    rc = collect_stats(c, n, js->x);
+    rc = collect_stats(c, n, limited(c, js->x));
     *out = c->last;
     return rc;
 }
@@ -1854,8 +1942,9 @@ extern "C" int ccs_graph_create(ccs_ctx* c, ccs_jobset* js, ccs_job_survivor* d_
     }
     SieveArgs sa;
     DeepArgs da;
-    jobset_args(js, sa, da, d_out, cap);
-    rc = run_filter_kernels(c, js->shape, js->x, sa, da, js->n_units, d_count, c->stream, true, true, js->nb_max);
+    jobset_args(c, js, sa, da, d_out, cap);
+    rc = run_filter_kernels(c, js->shape, limited(c, js->x), sa, da, js->n_units, d_count, c->stream, true, true,
+                            js->nb_max);
     e = hipStreamEndCapture(c->stream, &g->graph);
     c->profiling = prof;
     c->timed = timed;
@@ -1918,7 +2007,7 @@ extern "C" int ccs_batch(ccs_ctx* c, const ccs_shape* s, const ccs_job* jobs, ui
         hipMemcpy(h_out, c->out, (size_t)keep * sizeof(ccs_job_survivor), hipMemcpyDeviceToHost) != hipSuccess)
         rc = fail(CCS_ERR_HIP, "copy");
     *h_count = n;
-    if (!rc) rc = collect_stats(c, n, js->x);
+    if (!rc) rc = collect_stats(c, n, limited(c, js->x));
     if (stats) *stats = c->last;
     ccs_jobset_free(js);
     if (rc) return rc;

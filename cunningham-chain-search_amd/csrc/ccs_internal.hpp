@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/ccs.h"
+
 namespace ccs {
 
 typedef uint64_t u64;
@@ -23,6 +25,13 @@ constexpr int kMaxTierB = 1024;     // stage-B primes: at most 16 groups of 64
 constexpr unsigned kTierBMaxQ = 16384;   // stage-B primes are below this (14 bits)
 constexpr unsigned kTierBOffBits = 18;   // bitset word offsets of stage-B primes (< 2^18)
 constexpr unsigned kLdsCtlWords = 8;   // sieve scheduler words after the LDS tables
+// Launch counters of a scratch (16 dwords): words [0, kLaunchWords) are cleared before every
+// filter launch ([0] stage-B count, [2..3] unit cursor); words from kStatusWord on are the
+// sticky overflow status, cleared only when the caller reads it: [+0] launches whose stage-B
+// survivors exceeded the buffer, [+1] the largest such true count, [+2..+3] stage-B survivors
+// dropped (u64).
+constexpr unsigned kLaunchWords = 8;
+constexpr unsigned kStatusWord = 8;
 // prove launches over at most this many survivors run as one kernel with a wave per
 // survivor (every chain position at once); larger ones as a gate kernel (a lane per
 // survivor) and a chain kernel (a wave per prime survivor)
@@ -91,6 +100,23 @@ struct JobRec {     // class-form job: n = a + W*(u0 + j), j in [0, count); a an
     u64 W, W_hi;      // the modulus of the job's shape
     u32 shape;        // the job's shape within a group launch (0 otherwise)
     u32 unit_j;       // candidates per work unit of this job (a launch's unit_j overrides)
+    // resident residue state (served jobs): the phases of the job's sieve primes at some u_r
+    // are kept in slot ph_slot of the launch's phase arena, and u0 = u_r + ph_off; kNoSlot =
+    // none (phases are computed from u0)
+    u64 ph_off;
+    u32 ph_slot;
+    u32 pad;
+};
+constexpr u32 kNoSlot = 0xffffffffu;
+
+// Resident phases of one job: (u_r + a*W^-1) mod q for every dense slot and stage-B prime of
+// its shape, in the order of the shape's PrimeMod table (the dense slots, then stage B).
+struct PhaseInit {
+    const PrimeMod* pmod;     // the job's shape's residue constants
+    u32 n;                    // dense slots + stage-B primes of the shape
+    u32 slot;                 // slot in the phase arena
+    u64 a, a_hi;
+    u64 u_lo, u_hi;           // u_r
 };
 
 struct MidRec {     // stage-B survivor: owner (class or job index) and j
@@ -140,6 +166,8 @@ struct SieveArgs {
     const u64* shape_units;   // [n_group + 1] first unit of each shape's jobs (group launch)
     u32 unit_j;               // candidates per work unit (job launches: 0 = each job's own)
     u32 one;                  // 1 (see the dense loop)
+    const u32* phases;        // phase arena of resident jobs (JobRec::ph_slot), or null
+    u32 ph_stride;            // words per slot
     // work
     u64 n_units;
     u64 unit_base;            // added to the cursor value: first unit of this launch
@@ -180,6 +208,7 @@ struct DeepArgs {
     u32 out_cap;
     u32* out_count;
     unsigned long long* stats; // [2] += stage-B survivors of this launch (may be null)
+    u32* status;               // sticky overflow status (kStatusWord layout, may be null)
 };
 
 // Prove: GMP confirmation engine semantics (v13.cu:1794-1924) per survivor.
@@ -228,5 +257,17 @@ hipError_t launch_zero(u32* p, size_t n_words, hipStream_t s);
 // zeroes up to three ranges of at most 256 words each (null pointers are skipped)
 hipError_t launch_zero3(u32* a, u32 na, u32* b, u32 nb, u32* c, u32 nc, hipStream_t s);
 hipError_t launch_top_gate(const TopGateArgs& a, hipStream_t s);
+hipError_t launch_phase_init(const PhaseInit* d_jobs, u32 n_jobs, u32* phases, u32 stride, hipStream_t s);
+
+// ---- served form hooks (ccs_host.cpp, for ccs_server.cpp) ----
+// words of resident phase state per job of a shape (dense slots + stage-B primes)
+u32 shape_phase_words(const ccs_shape* s);
+// the residue constants of a shape (device, dense slots then stage B)
+const PrimeMod* shape_pmod(const ccs_shape* s);
+// a group-set update whose jobs carry resident phases: job i reads slot slots[i] of the arena,
+// whose phases are at u0_i - offs[i]
+int groupset_update_resident(ccs_ctx* c, ccs_jobset* js, const ccs_job* jobs, const uint32_t* job_shape,
+                             const uint32_t* slots, const uint64_t* offs, uint32_t n_jobs, const u32* d_phases,
+                             u32 stride, hipStream_t st);
 
 }  // namespace ccs

@@ -70,7 +70,12 @@ struct Unit {
     u32 n;       // candidates in the unit
     u32 owner;   // class or job index
     u32 j0;      // j of the first candidate
+    u32 ph_slot; // resident phases of the job (kNoSlot: none)
+    u64 ph_off;  // u_start minus the u of the resident phases
 };
+
+// (r + x) mod q for a resident phase r < q and an offset x < 2^40 (exact below 2^52)
+__device__ __forceinline__ u32 advance_phase(u32 r, u64 x, const PrimeMod& pm) { return mod52(x + r, pm.q, pm.rcp); }
 
 __device__ __forceinline__ void add128(u64& lo, u64& hi, u64 v) {
     u64 l = lo + v;
@@ -116,6 +121,8 @@ __device__ __forceinline__ bool resolve(const SieveArgs& A, u64 unit, u32 lane, 
         U.n = (u32)(end - beg);
         U.owner = (u32)c;
         U.j0 = (u32)beg;
+        U.ph_slot = kNoSlot;
+        U.ph_off = 0;
         return true;
     }
     const JobSrc& J = A.jobsrc;
@@ -134,6 +141,8 @@ __device__ __forceinline__ bool resolve(const SieveArgs& A, u64 unit, u32 lane, 
     U.n = (u32)n;
     U.owner = job + J.job_base;
     U.j0 = (u32)beg;
+    U.ph_slot = A.phases ? jr.ph_slot : kNoSlot;   // (read by the resident variant only)
+    U.ph_off = jr.ph_off + beg;
     return true;
 }
 
@@ -202,7 +211,10 @@ __device__ __forceinline__ bool stage_b_pass(const u32* __restrict__ lds, const 
     return !ballot(stage_b_bit<NBV>(lds, R, 0, j)) && stage_b_rest<NBV>(lds, far, nb_lds, R, j);
 }
 
-template <int RG, int NBV, bool GROUP>
+// GROUP: the launch sieves the jobs of several shapes (tables loaded per shape);
+// RESIDENT: its jobs may carry resident phases (a group-launch variant of its own, so the
+// launches without them keep their register allocation)
+template <int RG, int NBV, bool GROUP, bool RESIDENT>
 __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
     extern __shared__ u32 lds[];
     const u32 tid = threadIdx.x;
@@ -292,7 +304,7 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
     // so every wave passes the same barriers, and all leave in the same round.
     const u32 wpb = blockDim.x >> 6, wib = tid >> 6;
     const u32 n_blocks = gridDim.x;
-    const u32 n_units = (u32)A.n_units;   // < 2^32 (checked at launch)
+    const u32 n_units = (u32)A.n_units;   // < 2^32: the launchers reject larger launches
     u32 n_pool = n_units / 16;
     if (n_pool < n_blocks * wpb) n_pool = n_blocks * wpb;
     if (n_pool > n_units) n_pool = n_units;
@@ -399,6 +411,8 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
             U.n = bcast32(U.n);
             U.owner = bcast32(U.owner);
             U.j0 = bcast32(U.j0);
+            U.ph_slot = bcast32(U.ph_slot);
+            U.ph_off = bcast64(U.ph_off);
 
             // phases: dense prime p held by lane p as an LDS bit address, stage-B prime
             // (64*i + lane) held in R.s[i].  A unit that continues the previous one (same
@@ -411,6 +425,16 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                     const u32 x = R.s[i] + prev_n;
                     const int qt = (int)((float)x * __uint_as_float(R.rcp[i]));
                     R.s[i] = (u32)(__mul24(qt, R.nq(i)) + (int)x);
+                }
+            } else if (RESIDENT && U.ph_slot != kNoSlot) {
+                // a served job's phases are resident: advanced from its slot by the unit's offset
+                const u32* ph = A.phases + (size_t)U.ph_slot * A.ph_stride;
+                sd = 0;
+                if (lane < t_nd()) sd = doff * 32u + advance_phase(ph[lane], U.ph_off, t_pmod()[lane]);
+    #pragma unroll
+                for (int i = 0; i < NBV; ++i) {
+                    const u32 p = (u32)i * 64u + lane;
+                    R.s[i] = p < t_nb() ? advance_phase(ph[t_nd() + p], U.ph_off, t_pmod()[t_nd() + p]) : 0u;
                 }
             } else {
                 sd = 0;
@@ -591,12 +615,24 @@ __device__ __forceinline__ u32 deep_mod(const Limbs& L, const DeepPrime& d) {
 
 // one wave per stage-B survivor; the 64 lanes test 64 deep primes at a time, so the rare
 // survivors are not serialised over thousands of dependent per-prime steps
+// a stage-B buffer that overflowed: the sieve dropped the records past its capacity, so the
+// launch's survivors are incomplete; the sticky status tells the caller
+__device__ __forceinline__ void note_stage_b(const DeepArgs& A, u32 all) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (A.stats) atomicAdd(&A.stats[2], (unsigned long long)all);
+    if (A.status && all > A.mid_cap) {
+        atomicAdd(&A.status[0], 1u);
+        atomicMax(&A.status[1], all);
+        atomicAdd((unsigned long long*)(A.status + 2), (unsigned long long)(all - A.mid_cap));
+    }
+}
+
 __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
     const u32 lane = lane_id();
     const u32 all = *A.mid_count;
     u32 total = all;
     if (total > A.mid_cap) total = A.mid_cap;
-    if (A.stats && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.stats[2], (unsigned long long)all);
+    note_stage_b(A, all);
     const u32 waves = gridDim.x * (blockDim.x >> 6);
     const u32 wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     u32 q_n = 0, q_owner = 0, q_j = 0;
@@ -704,7 +740,7 @@ __global__ __launch_bounds__(256) void emit_kernel(DeepArgs A) {
     const u32 all = *A.mid_count;
     u32 total = all;
     if (total > A.mid_cap) total = A.mid_cap;
-    if (A.stats && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&A.stats[2], (unsigned long long)all);
+    note_stage_b(A, all);
     const u32 stride = gridDim.x * blockDim.x;
     for (u32 e0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); e0 < total; e0 += stride) {
         const u32 e = e0 + lane;
@@ -1020,6 +1056,16 @@ __global__ __launch_bounds__(256) void top_gate_kernel(TopGateArgs A) {
     }
 }
 
+// resident phases of newly admitted served jobs: one block per job, a thread per prime
+__global__ __launch_bounds__(256) void phase_init_kernel(const PhaseInit* jobs, u32 n_jobs, u32* phases,
+                                                         u32 stride) {
+    for (u32 j = blockIdx.x; j < n_jobs; j += gridDim.x) {
+        const PhaseInit J = jobs[j];
+        u32* out = phases + (size_t)J.slot * stride;
+        for (u32 p = threadIdx.x; p < J.n; p += blockDim.x) out[p] = phase(J.pmod[p], J.a, J.a_hi, J.u_lo, J.u_hi);
+    }
+}
+
 __global__ void zero_kernel(u32* p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0;
@@ -1041,10 +1087,12 @@ __global__ void zero3_kernel(u32* a, u32 na, u32* b, u32 nb, u32* c, u32 nc) {
 template <int RG, int NBV>
 static hipError_t launch_sieve_t(const SieveArgs& a, int grid, int block, size_t lds, hipStream_t s) {
     (void)hipGetLastError();   // launch errors only: drop any earlier non-launch error
-    if (a.group)
-        hipLaunchKernelGGL((sieve_kernel<RG, NBV, true>), dim3(grid), dim3(block), lds, s, a);
+    if (a.group && a.phases)
+        hipLaunchKernelGGL((sieve_kernel<RG, NBV, true, true>), dim3(grid), dim3(block), lds, s, a);
+    else if (a.group)
+        hipLaunchKernelGGL((sieve_kernel<RG, NBV, true, false>), dim3(grid), dim3(block), lds, s, a);
     else
-        hipLaunchKernelGGL((sieve_kernel<RG, NBV, false>), dim3(grid), dim3(block), lds, s, a);
+        hipLaunchKernelGGL((sieve_kernel<RG, NBV, false, false>), dim3(grid), dim3(block), lds, s, a);
     return hipGetLastError();
 }
 
@@ -1112,6 +1160,14 @@ hipError_t launch_top_gate(const TopGateArgs& a, hipStream_t s) {
     if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, a.out_count, (size_t)1);
     hipLaunchKernelGGL(top_gate_kernel, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase_init(const PhaseInit* d_jobs, u32 n_jobs, u32* phases, u32 stride, hipStream_t s) {
+    (void)hipGetLastError();
+    if (!n_jobs) return hipSuccess;
+    const u32 blocks = n_jobs < 1024u ? n_jobs : 1024u;
+    hipLaunchKernelGGL(phase_init_kernel, dim3(blocks), dim3(256), 0, s, d_jobs, n_jobs, phases, stride);
     return hipGetLastError();
 }
 

@@ -409,6 +409,23 @@ int main(int argc, char** argv) {
     int best = (int)st.chains.best_chain;
     uint64_t mismatches = 0;
 
+    // stage-B overflow of the filter launches since the last check (read at report time and
+    // at the end: the read waits for the filter stream, so not per batch); the reference
+    // warns and clamps on survivor overflow (v13.cu:3846-3852)
+    auto check_stage_b = [&]() -> int {
+        ccs_launch_status ls;
+        const int rc = ccs_stream_status(ctx, s_filter, &ls);
+        if (rc < 0) {
+            std::fprintf(stderr, "ccs_stream_status: %s\n", ccs_last_error());
+            return 1;
+        }
+        if (rc == CCS_OVERFLOW)
+            std::fprintf(stderr, "WARNING: stage-B buffer overflow in %u batch(es) (worst %u entries, buffer %u): "
+                         "%llu candidates dropped before the deep stage\n", ls.overflowed_launches, ls.worst_stage_b,
+                         ls.stage_b_cap, (unsigned long long)ls.dropped);
+        return 0;
+    };
+
     // collect the oldest batch: counts, results, lines, stats
     auto collect = [&]() -> int {
         const InFlight f = flight.front();
@@ -513,6 +530,7 @@ int main(int argc, char** argv) {
         if (o.cpu_prove && flight.size() == 2 && collect()) return 1;   // host proves while the GPU filters
         const double t = now_s();
         if (t - last_report >= o.report) {
+            if (check_stage_b()) return 1;
             CHECK(ccs_campaign_get_state(camp, &st));
             const double el = t - t0;
             std::printf("[%.1fs] tiles=%llu  cand=%.2fB  surv=%llu (%.6f%%)  primes=%llu  best=CC%u  rate=%.2fB/s",
@@ -539,6 +557,7 @@ int main(int argc, char** argv) {
     }
     while (!flight.empty())
         if (collect()) return 1;
+    if (check_stage_b()) return 1;
     (void)stop_msg;
 
     CHECK(ccs_campaign_get_state(camp, &st));

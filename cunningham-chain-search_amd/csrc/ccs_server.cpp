@@ -1,16 +1,19 @@
 // Served form: a continuous-batching request loop over the batched kernels.
 //
-// Requests (class-form search jobs of any depth, width and length) are admitted while
-// their resident device footprint fits the memory budget; every slice takes the next
-// chunk of every admitted job, grouped by depth profile, runs one persistent batched
-// filter launch + one prove launch per group, and hands results back per job.  Requests
-// that arrive while slices are on the GPU join the next slice.  Two buffer lanes
-// alternate so the GPU runs slice k+1 while the host distributes slice k; within a lane
-// the proves run on a stream of their own, behind an event per group, so they overlap the
-// next groups' filter launches.
+// Requests (class-form search jobs of any depth, width and length) are admitted while a
+// slot of the server's device arena is free: the arena holds each admitted job's residue
+// state (the phases of its sieve primes at the request's u0, written once by a kernel at
+// admission) and is sized by the memory budget.  Every slice takes the next chunk of every
+// admitted job, runs one persistent group launch over all depth profiles (the jobs' sieve
+// phases advanced from their resident state) + one prove launch, and hands results back per
+// job.  Requests that arrive while slices are on the GPU join the next slice.  Two buffer
+// lanes alternate so the GPU runs slice k+1 while the host distributes slice k; within a
+// lane the proves run on a stream of their own, behind an event per group, so they overlap
+// the next groups' filter launches.
 //
 // The reference has no serving mode; its closest pattern is the v16 dispatch ring with
-// an asynchronous prove pool (v16...cu:2938-3279, :5594-5965).
+// an asynchronous prove pool (v16...cu:2938-3279, :5594-5965), whose drained per-variant
+// cursors (v16...cu:4947-4955, :3104-3115) are the analogue of the resident job state here.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,10 +28,14 @@
 #include <vector>
 
 #include "../../include/ccs.h"
+#include "ccs_internal.hpp"
 
 namespace {
 
-constexpr uint32_t kEagerCopy = 1024;   // survivors copied back with every slice
+using ccs::kNoSlot;
+
+constexpr uint32_t kEagerCopy = 1024;        // survivors copied back with every slice
+constexpr uint64_t kMaxRequest = 1ull << 48;  // resident phases advance exactly below this offset
 
 double now_s() {
     using namespace std::chrono;
@@ -39,12 +46,13 @@ struct Job {
     uint64_t id = 0;
     ccs_request req{};
     int group = -1;
+    uint32_t slot = kNoSlot;        // arena slot while admitted
     uint64_t cursor = 0;            // next u offset to hand out
     uint32_t inflight = 0;          // slices launched, not yet collected
     double t_submit = 0, t_admit = 0, t_done = 0;
-    std::vector<ccs_job_survivor> surv;   // u_off relative to the request's u0
+    std::vector<uint64_t> u_off;    // survivors: u offset within the request
     std::vector<ccs_chain_result> res;
-    bool done = false;
+    bool done = false, incomplete = false;
 };
 
 struct Entry {          // one job's chunk inside a slice
@@ -52,7 +60,7 @@ struct Entry {          // one job's chunk inside a slice
     uint64_t base;      // u offset of the chunk within the job
 };
 
-struct Lane {           // buffers of one in-flight slice for one depth group
+struct Lane {           // buffers of one in-flight slice for one launch group
     ccs_jobset* js = nullptr;
     ccs_job_survivor* d_surv = nullptr;
     ccs_chain_result* d_res = nullptr;
@@ -61,7 +69,9 @@ struct Lane {           // buffers of one in-flight slice for one depth group
     ccs_chain_result* h_res = nullptr;    // pinned
     uint32_t* h_count = nullptr;          // pinned
     std::vector<ccs_job> jobs;
-    std::vector<uint32_t> job_shape;      // group lanes: the depth profile of each job
+    std::vector<uint32_t> job_shape;      // the depth profile of each job
+    std::vector<uint32_t> slots;          // the arena slot of each job
+    std::vector<uint64_t> offs;           // each chunk's u offset from the resident phases
     std::vector<Entry> entries;
     hipEvent_t filtered = nullptr;        // the group's filter launch is done (prove may start)
     bool busy = false;
@@ -74,10 +84,20 @@ struct ccs_server {
     ccs_server_params p{};
     std::vector<const ccs_shape*> shapes;   // index = depth group
     std::vector<int32_t> depths;
-    hipStream_t streams[2] = {nullptr, nullptr};    // per lane: filter launches
+    hipStream_t streams[2] = {nullptr, nullptr};    // per lane: phase inits and filter launches
     hipStream_t pstreams[2] = {nullptr, nullptr};   // per lane: prove launches and copies
-    std::vector<Lane> lanes[2];             // [lane][group]; one group set per lane when grouped
+    std::vector<Lane> lanes[2];             // [lane][launch group]; one group when grouped
     bool grouped = false;                   // several depth profiles: one launch per slice
+    bool resident = true;                   // phases from the arena (else from u0 per unit)
+    // the arena: per slot `stride` words of phases
+    uint32_t stride = 0, n_slots = 0;
+    uint32_t* d_phases = nullptr;
+    uint64_t job_bytes = 0, fixed_bytes = 0;
+    std::vector<uint32_t> free_slots;
+    std::vector<ccs::PhaseInit> inits;      // admitted since the last slice launch
+    ccs::PhaseInit* h_init[2] = {nullptr, nullptr};   // pinned staging per lane
+    ccs::PhaseInit* d_init[2] = {nullptr, nullptr};
+    hipEvent_t ev_init[2] = {nullptr, nullptr};        // the lane's last phase-init launch
     std::mutex mu;
     std::condition_variable cv;
     std::deque<uint64_t> pending;
@@ -87,17 +107,13 @@ struct ccs_server {
     uint64_t next_id = 1;
     uint64_t resident_bytes = 0;
     uint64_t stat_candidates = 0, stat_slices = 0, stat_launches = 0;
-    bool stop = false, drain = false;
+    uint32_t n_incomplete = 0;
+    double busy_s = 0;
+    bool stop = false;
     int error = 0;
     std::string error_msg;
     std::thread th;
 };
-
-static uint64_t job_footprint(const ccs_server_params& p) {
-    // device-resident state of an admitted job: its job record in both lanes' slice tables
-    // plus its share of the survivor and result slots
-    return 2 * (sizeof(ccs_job) + (uint64_t)p.surv_per_job * (sizeof(ccs_job_survivor) + sizeof(ccs_chain_result)));
-}
 
 static int server_fail(ccs_server* s, int rc, const char* what) {
     std::lock_guard<std::mutex> g(s->mu);
@@ -115,12 +131,24 @@ static int collect_lane(ccs_server* s, int li) {
     if (!any) return CCS_OK;
     if (hipStreamSynchronize(s->streams[li]) != hipSuccess || hipStreamSynchronize(s->pstreams[li]) != hipSuccess)
         return CCS_ERR_HIP;
+    // survivors lost to a full stage-B buffer or a full slice buffer: the slice's jobs are
+    // finished incomplete (and flagged so)
+    std::vector<char> lost(s->lanes[li].size(), 0);
+    for (size_t gi = 0; gi < s->lanes[li].size(); ++gi) {
+        Lane& L = s->lanes[li][gi];
+        if (!L.busy) continue;
+        ccs_launch_status ls;
+        const int rc = ccs_jobset_status(s->ctx, L.js, s->streams[li], &ls);
+        if (rc < 0) return rc;
+        lost[gi] = rc == CCS_OVERFLOW || *L.h_count > s->p.max_slice_survivors;
+    }
     const double t = now_s();
     std::lock_guard<std::mutex> g(s->mu);
-    for (auto& L : s->lanes[li]) {
+    for (size_t gi = 0; gi < s->lanes[li].size(); ++gi) {
+        Lane& L = s->lanes[li][gi];
         if (!L.busy) continue;
         L.busy = false;
-        uint32_t n = std::min(*L.h_count, s->p.max_slice_survivors);
+        const uint32_t n = std::min(*L.h_count, s->p.max_slice_survivors);
         if (n > kEagerCopy) {   // the rest of a large slice's survivors
             if (hipMemcpy(L.h_surv + kEagerCopy, L.d_surv + kEagerCopy, (size_t)(n - kEagerCopy) * sizeof(ccs_job_survivor),
                           hipMemcpyDeviceToHost) != hipSuccess ||
@@ -132,20 +160,14 @@ static int collect_lane(ccs_server* s, int li) {
             const ccs_job_survivor sv = L.h_surv[i];
             const Entry& e = L.entries[sv.job_idx];
             Job& j = s->jobs[e.job];
-            ccs_job_survivor out;
-            out.job_idx = 0;                                   // per-job list: the job itself
-            out.u_off = (uint32_t)(e.base + sv.u_off);         // u offset within the request
-            j.surv.push_back(out);
+            j.u_off.push_back(e.base + sv.u_off);   // 64-bit: requests can exceed 2^32
             ccs_chain_result r = L.h_res[i];
-            r.survivor_idx = (uint32_t)(j.surv.size() - 1);
+            r.survivor_idx = (uint32_t)(j.u_off.size() - 1);
             j.res.push_back(r);
-        }
-        if (*L.h_count > s->p.max_slice_survivors) {
-            s->error = CCS_OVERFLOW;
-            s->error_msg = "slice survivor buffer overflow";
         }
         for (const Entry& e : L.entries) {
             Job& j = s->jobs[e.job];
+            if (lost[gi]) j.incomplete = true;
             if (--j.inflight == 0 && j.cursor >= j.req.count && !j.done) {
                 j.done = true;
                 j.t_done = t;
@@ -153,17 +175,21 @@ static int collect_lane(ccs_server* s, int li) {
                 d.job_id = j.id;
                 d.tag = j.req.tag;
                 d.candidates = j.req.count;
-                d.n_survivors = (uint32_t)j.surv.size();
+                d.n_survivors = (uint32_t)j.u_off.size();
                 uint32_t primes = 0, best = 0;
                 for (auto& r : j.res)
                     if (r.flags & CCS_F_PRIME) { ++primes; best = std::max(best, r.chain_len); }
                 d.n_primes = primes;
                 d.best_len = best;
+                d.flags = j.incomplete ? CCS_JOB_INCOMPLETE : 0u;
+                s->n_incomplete += j.incomplete ? 1u : 0u;
                 d.t_submit = j.t_submit;
                 d.t_admit = j.t_admit;
                 d.t_done = j.t_done;
                 s->finished.push_back(d);
-                s->resident_bytes -= job_footprint(s->p);
+                s->resident_bytes -= s->job_bytes;
+                s->free_slots.push_back(j.slot);
+                j.slot = kNoSlot;
                 s->active.erase(std::find(s->active.begin(), s->active.end(), j.id));
             }
         }
@@ -172,22 +198,55 @@ static int collect_lane(ccs_server* s, int li) {
     return CCS_OK;
 }
 
+// the phase-init kernel for the jobs admitted since the last slice, on lane li's filter stream
+// (the other lane's next slice waits for it through ev_init[li])
+static int launch_inits(ccs_server* s, int li, std::vector<ccs::PhaseInit>& inits) {
+    if (inits.empty()) return CCS_OK;
+    const uint32_t n = (uint32_t)inits.size();   // at most n_slots: one per free slot
+    std::memcpy(s->h_init[li], inits.data(), (size_t)n * sizeof(ccs::PhaseInit));
+    if (hipMemcpyAsync(s->d_init[li], s->h_init[li], (size_t)n * sizeof(ccs::PhaseInit), hipMemcpyHostToDevice,
+                       s->streams[li]) != hipSuccess ||
+        ccs::launch_phase_init(s->d_init[li], n, s->d_phases, s->stride, s->streams[li]) != hipSuccess ||
+        hipEventRecord(s->ev_init[li], s->streams[li]) != hipSuccess)
+        return ccs::set_error(CCS_ERR_HIP, "phase init launch");
+    inits.clear();
+    return CCS_OK;
+}
+
 static void server_loop(ccs_server* s) {
     int li = 0;
     for (;;) {
         // admission and slice planning under the lock
-        std::vector<std::vector<Entry>> plan(s->shapes.size());
-        std::vector<std::vector<ccs_job>> recs(s->shapes.size());
+        const size_t n_groups = s->lanes[0].size();
+        std::vector<std::vector<Entry>> plan(n_groups);
+        std::vector<std::vector<ccs_job>> recs(n_groups);
+        std::vector<std::vector<uint32_t>> shp(n_groups), slots(n_groups);
+        std::vector<std::vector<uint64_t>> offs(n_groups);
+        std::vector<ccs::PhaseInit> inits;
+        double t_busy = 0;
         {
             std::unique_lock<std::mutex> g(s->mu);
             for (;;) {
-                while (!s->pending.empty() && s->active.size() < s->p.max_active &&
-                       s->resident_bytes + job_footprint(s->p) <= s->p.mem_budget_bytes) {
+                while (!s->pending.empty() && s->active.size() < s->p.max_active && !s->free_slots.empty()) {
                     Job& j = s->jobs[s->pending.front()];
                     s->pending.pop_front();
                     j.t_admit = now_s();
-                    s->resident_bytes += job_footprint(s->p);
+                    j.slot = s->free_slots.back();
+                    s->free_slots.pop_back();
+                    s->resident_bytes += s->job_bytes;
                     s->active.push_back(j.id);
+                    if (s->resident) {
+                        const ccs_shape* sh = s->shapes[j.group];
+                        ccs::PhaseInit pi;
+                        pi.pmod = ccs::shape_pmod(sh);
+                        pi.n = ccs::shape_phase_words(sh);
+                        pi.slot = j.slot;
+                        pi.a = j.req.a;
+                        pi.a_hi = 0;
+                        pi.u_lo = j.req.u0.lo;
+                        pi.u_hi = j.req.u0.hi;
+                        s->inits.push_back(pi);
+                    }
                 }
                 bool work = false;
                 for (uint64_t id : s->active)
@@ -196,60 +255,73 @@ static void server_loop(ccs_server* s) {
                 for (auto& lv : s->lanes)
                     for (auto& L : lv) inflight |= L.busy;
                 if (work || inflight || s->stop) break;
-                if (s->drain && s->pending.empty()) { s->drain = false; s->cv.notify_all(); }
+                s->cv.notify_all();   // idle: a drain may be waiting
                 s->cv.wait(g);
             }
             if (s->stop) break;
+            t_busy = now_s();
+            inits.swap(s->inits);
             // per-job quota: an even share of the slice target, at least one unit
             uint64_t n_work = 0;
             for (uint64_t id : s->active) n_work += s->jobs[id].cursor < s->jobs[id].req.count;
             uint64_t quota = n_work ? s->p.slice_candidates / n_work : 0;
             quota = std::max<uint64_t>(quota, s->p.min_chunk);
             quota = std::min<uint64_t>(quota, s->p.max_chunk);
-            size_t planned = 0;
             for (uint64_t id : s->active) {
                 Job& j = s->jobs[id];
                 if (j.cursor >= j.req.count) continue;
-                if (planned >= s->p.max_active) continue;   // (active jobs never exceed it)
                 const uint64_t n = std::min<uint64_t>(quota, j.req.count - j.cursor);
-                uint64_t u0lo = j.req.u0.lo + j.cursor;
-                uint64_t u0hi = j.req.u0.hi + (u0lo < j.req.u0.lo);
+                const uint64_t u0lo = j.req.u0.lo + j.cursor;
                 ccs_job r{};
                 r.a = j.req.a;
                 r.u0.lo = u0lo;
-                r.u0.hi = u0hi;
+                r.u0.hi = j.req.u0.hi + (u0lo < j.req.u0.lo);
                 r.count = n;
-                recs[j.group].push_back(r);
-                plan[j.group].push_back(Entry{id, j.cursor});
-                ++planned;
+                const size_t gi = s->grouped ? 0 : (size_t)j.group;
+                recs[gi].push_back(r);
+                shp[gi].push_back(s->grouped ? (uint32_t)j.group : 0u);
+                slots[gi].push_back(j.slot);
+                offs[gi].push_back(j.cursor);   // the resident phases are those at u0
+                plan[gi].push_back(Entry{id, j.cursor});
                 j.cursor += n;
                 j.inflight++;
                 s->stat_candidates += n;
             }
         }
-        // launch the slice on lane li: one group launch over every depth profile, or one
-        // launch per profile
-        bool launched = false;
-        if (s->grouped) {   // flatten the plan in profile order (group sets want jobs grouped)
-            for (size_t gi = 1; gi < s->shapes.size(); ++gi) {
-                plan[0].insert(plan[0].end(), plan[gi].begin(), plan[gi].end());
-                recs[0].insert(recs[0].end(), recs[gi].begin(), recs[gi].end());
+        // group sets want their jobs grouped by shape: order each launch group's jobs by profile
+        for (size_t gi = 0; gi < n_groups; ++gi) {
+            std::vector<size_t> order(recs[gi].size());
+            for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+            std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return shp[gi][a] < shp[gi][b]; });
+            Lane& L = s->lanes[li][gi];
+            L.jobs.clear();
+            L.job_shape.clear();
+            L.slots.clear();
+            L.offs.clear();
+            L.entries.clear();
+            for (size_t i : order) {
+                L.jobs.push_back(recs[gi][i]);
+                L.job_shape.push_back(shp[gi][i]);
+                L.slots.push_back(slots[gi][i]);
+                L.offs.push_back(offs[gi][i]);
+                L.entries.push_back(plan[gi][i]);
             }
         }
-        for (size_t gi = 0; gi < s->lanes[li].size(); ++gi) {
-            if (plan[gi].empty()) continue;
+        int rc = launch_inits(s, li, inits);
+        // slices of the other lane may hold jobs whose phases the other lane's stream wrote
+        if (!rc && s->resident && hipStreamWaitEvent(s->streams[li], s->ev_init[li ^ 1], 0) != hipSuccess)
+            rc = ccs::set_error(CCS_ERR_HIP, "stream wait");
+        if (rc) { server_fail(s, rc, "phase init"); return; }
+        bool launched = false;
+        for (size_t gi = 0; gi < n_groups; ++gi) {
             Lane& L = s->lanes[li][gi];
-            L.jobs = recs[gi];
-            L.entries = plan[gi];
-            int rc;
-            if (s->grouped) {
-                L.job_shape.clear();
-                for (const Entry& e : L.entries) L.job_shape.push_back((uint32_t)s->jobs[e.job].group);
-                rc = ccs_groupset_update(s->ctx, L.js, L.jobs.data(), L.job_shape.data(), (uint32_t)L.jobs.size(),
-                                         s->streams[li]);
-            } else {
-                rc = ccs_jobset_update(s->ctx, L.js, L.jobs.data(), (uint32_t)L.jobs.size(), s->streams[li]);
-            }
+            if (L.jobs.empty()) continue;
+            const uint32_t nj = (uint32_t)L.jobs.size();
+            if (s->resident)
+                rc = ccs::groupset_update_resident(s->ctx, L.js, L.jobs.data(), L.job_shape.data(), L.slots.data(),
+                                                   L.offs.data(), nj, s->d_phases, s->stride, s->streams[li]);
+            else
+                rc = ccs_groupset_update(s->ctx, L.js, L.jobs.data(), L.job_shape.data(), nj, s->streams[li]);
             if (!rc) rc = ccs_jobset_launch(s->ctx, L.js, L.d_surv, s->p.max_slice_survivors, L.d_count, s->streams[li]);
             // the prove of this group runs on the lane's prove stream, so the next group's
             // filter launch does not wait behind the prove's latency-bound tail
@@ -278,32 +350,75 @@ static void server_loop(ccs_server* s) {
         }
         if (launched) s->stat_slices++;
         // while lane li runs, collect the other lane
-        if (collect_lane(s, li ^ 1)) { server_fail(s, CCS_ERR_HIP, "collect"); return; }
+        if ((rc = collect_lane(s, li ^ 1))) { server_fail(s, rc, "collect"); return; }
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->busy_s += now_s() - t_busy;
+        }
         li ^= 1;
     }
     collect_lane(s, 0);
     collect_lane(s, 1);
 }
 
+static void destroy(ccs_server* s) {
+    for (int li = 0; li < 2; ++li) {
+        if (s->streams[li]) (void)hipStreamSynchronize(s->streams[li]);
+        if (s->pstreams[li]) (void)hipStreamSynchronize(s->pstreams[li]);
+        for (auto& L : s->lanes[li]) {
+            if (L.filtered) (void)hipEventDestroy(L.filtered);
+            if (L.js) ccs_jobset_free(L.js);
+            if (L.d_surv) ccs_dfree(s->ctx, L.d_surv);
+            if (L.d_res) ccs_dfree(s->ctx, L.d_res);
+            if (L.d_count) ccs_dfree(s->ctx, L.d_count);
+            if (L.h_surv) (void)hipHostFree(L.h_surv);
+            if (L.h_res) (void)hipHostFree(L.h_res);
+            if (L.h_count) (void)hipHostFree(L.h_count);
+        }
+        if (s->h_init[li]) (void)hipHostFree(s->h_init[li]);
+        if (s->d_init[li]) ccs_dfree(s->ctx, s->d_init[li]);
+        if (s->ev_init[li]) (void)hipEventDestroy(s->ev_init[li]);
+        if (s->streams[li]) (void)hipStreamDestroy(s->streams[li]);
+        if (s->pstreams[li]) (void)hipStreamDestroy(s->pstreams[li]);
+    }
+    if (s->d_phases) ccs_dfree(s->ctx, s->d_phases);
+    delete s;
+}
+
 extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, const int32_t* depths,
                                  uint32_t n_shapes, const ccs_server_params* params, ccs_server** out) {
-    if (!ctx || !shapes || !depths || !n_shapes || !params || !out) return CCS_ERR_INVALID;
+    if (!ctx || !shapes || !depths || !n_shapes || !params || !out)
+        return ccs::set_error(CCS_ERR_INVALID, "null argument");
     *out = nullptr;
     ccs_server* s = new ccs_server();
     s->ctx = ctx;
     s->p = *params;
     if (!s->p.max_active) s->p.max_active = 1024;
-    if (!s->p.surv_per_job) s->p.surv_per_job = 16;
     if (!s->p.max_slice_survivors) s->p.max_slice_survivors = 1u << 16;
     if (!s->p.slice_candidates) s->p.slice_candidates = 1ull << 35;
     if (!s->p.min_chunk) s->p.min_chunk = 1u << 18;
     if (!s->p.max_chunk) s->p.max_chunk = 1ull << 32;
     if (!s->p.mem_budget_bytes) s->p.mem_budget_bytes = 256ull << 20;
     s->p.max_chunk = std::min<uint64_t>(s->p.max_chunk, 1ull << 32);
+    s->resident = !(s->p.flags & CCS_SERVER_RECOMPUTE_PHASES);
     for (uint32_t i = 0; i < n_shapes; ++i) {
+        if (!shapes[i]) { delete s; return ccs::set_error(CCS_ERR_INVALID, "null shape"); }
         s->shapes.push_back(shapes[i]);
         s->depths.push_back(depths[i]);
+        s->stride = std::max(s->stride, ccs::shape_phase_words(shapes[i]));
     }
+    s->stride = (s->stride + 63) & ~63u;   // slots on 256-byte boundaries
+    // one admitted job's device state: its phases in the arena, and its rows (job record and
+    // unit prefix) in both lanes' slice tables
+    s->job_bytes = (s->resident ? (uint64_t)s->stride * 4 : 0) + 2 * (sizeof(ccs::JobRec) + 8);
+    const uint64_t fit = s->p.mem_budget_bytes / s->job_bytes;
+    s->n_slots = (uint32_t)std::min<uint64_t>(s->p.max_active, fit);
+    if (!s->n_slots) {
+        delete s;
+        return ccs::set_error(CCS_ERR_INVALID, "the memory budget holds no job's resident state");
+    }
+    s->p.max_active = std::min(s->p.max_active, s->n_slots);
+    for (uint32_t k = s->n_slots; k-- > 0;) s->free_slots.push_back(k);
     int rc = CCS_OK;
     // several depth profiles with one row layout: one group launch per slice
     s->grouped = n_shapes > 1;
@@ -315,6 +430,8 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
             if (!rc && t.rows_per_group != t0.rows_per_group) s->grouped = false;
         }
     }
+    if (s->resident && !rc)
+        rc = ccs_dmalloc(ctx, (void**)&s->d_phases, (uint64_t)s->n_slots * s->stride * 4);
     const uint64_t slice = std::max<uint64_t>(s->p.slice_candidates, s->p.min_chunk);
     const uint64_t per = std::min<uint64_t>(std::max<uint64_t>(s->p.slice_candidates / s->p.max_active, s->p.min_chunk),
                                             s->p.max_chunk);
@@ -329,32 +446,34 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
     };
     for (int li = 0; li < 2 && !rc; ++li) {
         if (hipStreamCreateWithFlags(&s->streams[li], hipStreamNonBlocking) != hipSuccess ||
-            hipStreamCreateWithFlags(&s->pstreams[li], hipStreamNonBlocking) != hipSuccess)
-            rc = CCS_ERR_HIP;
+            hipStreamCreateWithFlags(&s->pstreams[li], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&s->ev_init[li], hipEventDisableTiming) != hipSuccess)
+            rc = ccs::set_error(CCS_ERR_HIP, "server streams");
+        if (!rc && s->resident) {
+            if (hipHostMalloc((void**)&s->h_init[li], (size_t)s->n_slots * sizeof(ccs::PhaseInit)) != hipSuccess)
+                rc = ccs::set_error(CCS_ERR_NOMEM, "phase init staging");
+            if (!rc) rc = ccs_dmalloc(ctx, (void**)&s->d_init[li], (uint64_t)s->n_slots * sizeof(ccs::PhaseInit));
+        }
         s->lanes[li].resize(s->grouped ? 1 : n_shapes);
         for (uint32_t gi = 0; gi < s->lanes[li].size() && !rc; ++gi) {
             Lane& L = s->lanes[li][gi];
-            if (s->grouped) {
-                std::vector<ccs_job> dummy;
-                std::vector<uint32_t> dshape;
-                for (uint32_t k = 0; k < n_shapes; ++k)
-                    for (uint64_t left = slice; left > 0;) {
-                        const uint64_t c = std::min<uint64_t>(left, s->p.max_chunk);
-                        dummy.push_back(dummy_job(c));
-                        dshape.push_back(k);
-                        left -= c;
-                    }
-                while (dummy.size() < s->p.max_active) {
-                    dummy.push_back(dummy_job(per));
-                    dshape.push_back(n_shapes - 1);
+            // a group set over every profile (grouped), or over the one profile of this group
+            const uint32_t first = s->grouped ? 0 : gi, count = s->grouped ? n_shapes : 1;
+            std::vector<ccs_job> dummy;
+            std::vector<uint32_t> dshape;
+            for (uint32_t k = 0; k < count; ++k)
+                for (uint64_t left = slice; left > 0;) {
+                    const uint64_t c = std::min<uint64_t>(left, s->p.max_chunk);
+                    dummy.push_back(dummy_job(c));
+                    dshape.push_back(k);
+                    left -= c;
                 }
-                rc = ccs_groupset_create(ctx, shapes, n_shapes, dummy.data(), dshape.data(), (uint32_t)dummy.size(),
-                                         &L.js);
-            } else {
-                std::vector<ccs_job> dummy(s->p.max_active, dummy_job(per));
-                dummy[0].count = std::min<uint64_t>(slice, s->p.max_chunk);
-                rc = ccs_jobset_create(ctx, shapes[gi], dummy.data(), (uint32_t)dummy.size(), &L.js);
+            while (dummy.size() < s->p.max_active) {
+                dummy.push_back(dummy_job(per));
+                dshape.push_back(count - 1);
             }
+            rc = ccs_groupset_create(ctx, shapes + first, count, dummy.data(), dshape.data(), (uint32_t)dummy.size(),
+                                     &L.js);
             if (!rc) rc = ccs_dmalloc(ctx, (void**)&L.d_surv, (uint64_t)s->p.max_slice_survivors * sizeof(ccs_job_survivor));
             if (!rc) rc = ccs_dmalloc(ctx, (void**)&L.d_res, (uint64_t)s->p.max_slice_survivors * sizeof(ccs_chain_result));
             if (!rc) rc = ccs_dmalloc(ctx, (void**)&L.d_count, 64);
@@ -362,10 +481,11 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
             if (!rc && hipHostMalloc((void**)&L.h_res, (size_t)s->p.max_slice_survivors * sizeof(ccs_chain_result)) != hipSuccess) rc = CCS_ERR_NOMEM;
             if (!rc && hipHostMalloc((void**)&L.h_count, 64) != hipSuccess) rc = CCS_ERR_NOMEM;
             if (!rc && hipEventCreateWithFlags(&L.filtered, hipEventDisableTiming) != hipSuccess) rc = CCS_ERR_HIP;
+            s->fixed_bytes += (uint64_t)s->p.max_slice_survivors * (sizeof(ccs_job_survivor) + sizeof(ccs_chain_result));
         }
     }
     if (rc) {
-        ccs_server_shutdown(s);
+        destroy(s);
         return rc;
     }
     s->th = std::thread(server_loop, s);
@@ -374,13 +494,17 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
 }
 
 extern "C" int ccs_server_submit(ccs_server* s, const ccs_request* req, uint64_t* job_id) {
-    if (!s || !req) return CCS_ERR_INVALID;
+    if (!s || !req) return ccs::set_error(CCS_ERR_INVALID, "null argument");
     int group = -1;
     for (size_t i = 0; i < s->depths.size(); ++i)
         if (s->depths[i] == req->depth) group = (int)i;
-    if (group < 0) return CCS_ERR_INVALID;
+    if (group < 0) return ccs::set_error(CCS_ERR_INVALID, "no shape of the request's depth");
+    if (req->count >= kMaxRequest) return ccs::set_error(CCS_ERR_INVALID, "request count must be below 2^48");
+    ccs_shape_info info;
+    if (ccs_shape_query(s->shapes[group], &info) != CCS_OK) return CCS_ERR_INVALID;
+    if (info.W_hi || req->a >= info.W) return ccs::set_error(CCS_ERR_INVALID, "request residue a must be < W");
     std::lock_guard<std::mutex> g(s->mu);
-    if (s->stop) return s->error ? s->error : CCS_ERR_BUSY;
+    if (s->stop) return s->error < 0 ? s->error : ccs::set_error(CCS_ERR_BUSY, "server is shut down");
     Job j;
     j.id = s->next_id++;
     j.req = *req;
@@ -403,34 +527,41 @@ extern "C" int ccs_server_submit(ccs_server* s, const ccs_request* req, uint64_t
     return CCS_OK;
 }
 
+// CCS_OVERFLOW when a returned job finished incomplete, < 0 when the server failed
 extern "C" int ccs_server_poll(ccs_server* s, ccs_job_done* out, uint32_t cap, uint32_t* n) {
-    if (!s || !n) return CCS_ERR_INVALID;
+    if (!s || !n || (cap && !out)) return ccs::set_error(CCS_ERR_INVALID, "null argument");
     std::lock_guard<std::mutex> g(s->mu);
     const uint32_t k = std::min<uint32_t>(cap, (uint32_t)s->finished.size());
-    for (uint32_t i = 0; i < k; ++i) out[i] = s->finished[i];
+    bool incomplete = false;
+    for (uint32_t i = 0; i < k; ++i) {
+        out[i] = s->finished[i];
+        incomplete |= (out[i].flags & CCS_JOB_INCOMPLETE) != 0;
+    }
     s->finished.erase(s->finished.begin(), s->finished.begin() + k);
     *n = k;
-    return s->error < 0 ? s->error : CCS_OK;
+    if (s->error < 0) return ccs::set_error(s->error, s->error_msg.c_str());
+    return incomplete ? CCS_OVERFLOW : CCS_OK;
 }
 
-extern "C" int ccs_server_job_results(ccs_server* s, uint64_t job_id, ccs_job_survivor* surv,
-                                      ccs_chain_result* res, uint32_t cap, uint32_t* n) {
-    if (!s || !n) return CCS_ERR_INVALID;
+extern "C" int ccs_server_job_results(ccs_server* s, uint64_t job_id, uint64_t* u_off, ccs_chain_result* res,
+                                      uint32_t cap, uint32_t* n) {
+    if (!s || !n) return ccs::set_error(CCS_ERR_INVALID, "null argument");
     std::lock_guard<std::mutex> g(s->mu);
     auto it = s->jobs.find(job_id);
-    if (it == s->jobs.end() || !it->second.done) return CCS_ERR_INVALID;
+    if (it == s->jobs.end() || !it->second.done) return ccs::set_error(CCS_ERR_INVALID, "no finished job of that id");
     const Job& j = it->second;
-    const uint32_t k = std::min<uint32_t>(cap, (uint32_t)j.surv.size());
+    const uint32_t k = std::min<uint32_t>(cap, (uint32_t)j.u_off.size());
     for (uint32_t i = 0; i < k; ++i) {
-        if (surv) surv[i] = j.surv[i];
+        if (u_off) u_off[i] = j.u_off[i];
         if (res) res[i] = j.res[i];
     }
-    *n = (uint32_t)j.surv.size();
-    return k < j.surv.size() ? CCS_OVERFLOW : CCS_OK;
+    *n = (uint32_t)j.u_off.size();
+    return k < j.u_off.size() ? CCS_OVERFLOW : CCS_OK;
 }
 
+// CCS_OVERFLOW when some job so far finished incomplete, < 0 when the server failed
 extern "C" int ccs_server_drain(ccs_server* s) {
-    if (!s) return CCS_ERR_INVALID;
+    if (!s) return ccs::set_error(CCS_ERR_INVALID, "null argument");
     std::unique_lock<std::mutex> g(s->mu);
     s->cv.wait(g, [&] {
         if (s->stop) return true;
@@ -438,46 +569,38 @@ extern "C" int ccs_server_drain(ccs_server* s) {
         for (uint64_t id : s->active) if (!s->jobs[id].done) return false;
         return true;
     });
-    return s->error < 0 ? s->error : CCS_OK;
+    if (s->error < 0) return ccs::set_error(s->error, s->error_msg.c_str());
+    return s->n_incomplete ? CCS_OVERFLOW : CCS_OK;
 }
 
 extern "C" int ccs_server_stats(ccs_server* s, ccs_server_counters* out) {
-    if (!s || !out) return CCS_ERR_INVALID;
+    if (!s || !out) return ccs::set_error(CCS_ERR_INVALID, "null argument");
     std::lock_guard<std::mutex> g(s->mu);
+    std::memset(out, 0, sizeof(*out));
     out->candidates = s->stat_candidates;
     out->slices = s->stat_slices;
     out->launches = s->stat_launches;
     out->active = (uint32_t)s->active.size();
     out->pending = (uint32_t)s->pending.size();
     out->resident_bytes = s->resident_bytes;
+    out->job_bytes = s->job_bytes;
+    out->fixed_bytes = s->fixed_bytes;
+    out->slots = s->n_slots;
+    out->incomplete = s->n_incomplete;
+    out->busy_s = s->busy_s;
     return CCS_OK;
 }
 
 extern "C" int ccs_server_shutdown(ccs_server* s) {
-    if (!s) return CCS_ERR_INVALID;
+    if (!s) return ccs::set_error(CCS_ERR_INVALID, "null argument");
     {
         std::lock_guard<std::mutex> g(s->mu);
         s->stop = true;
         s->cv.notify_all();
     }
     if (s->th.joinable()) s->th.join();
-    for (int li = 0; li < 2; ++li) {
-        if (s->streams[li]) (void)hipStreamSynchronize(s->streams[li]);
-        if (s->pstreams[li]) (void)hipStreamSynchronize(s->pstreams[li]);
-        for (auto& L : s->lanes[li]) {
-            if (L.filtered) (void)hipEventDestroy(L.filtered);
-            if (L.js) ccs_jobset_free(L.js);
-            if (L.d_surv) ccs_dfree(s->ctx, L.d_surv);
-            if (L.d_res) ccs_dfree(s->ctx, L.d_res);
-            if (L.d_count) ccs_dfree(s->ctx, L.d_count);
-            if (L.h_surv) (void)hipHostFree(L.h_surv);
-            if (L.h_res) (void)hipHostFree(L.h_res);
-            if (L.h_count) (void)hipHostFree(L.h_count);
-        }
-        if (s->streams[li]) (void)hipStreamDestroy(s->streams[li]);
-        if (s->pstreams[li]) (void)hipStreamDestroy(s->pstreams[li]);
-    }
     const int err = s->error;
-    delete s;
-    return err < 0 ? err : CCS_OK;
+    const std::string msg = s->error_msg;
+    destroy(s);
+    return err < 0 ? ccs::set_error(err, msg.c_str()) : CCS_OK;
 }

@@ -326,7 +326,8 @@ int ccs_campaign_resume(ccs_campaign* c, const char* path);
 /* ---- plain filter: K1 over tiles [tile_start, tile_start + tile_count), all bases
  *      (replaces prepare_batch v13.cu:1232-1259 + cc18_filter_kernel :1551-1714 and its
  *      launch :3769-3796).  Survivors go to d_out in unspecified order; *d_count (device)
- *      receives the true count.  Asynchronous on `stream`. ---- */
+ *      receives the true count of the launch's survivors, unless its stage-B buffer
+ *      overflowed (ccs_stream_status).  Asynchronous on `stream`. ---- */
 int ccs_filter_launch(ccs_ctx* ctx, const ccs_shape* shape, ccs_u128 tile_start,
                       uint32_t tile_count, ccs_survivor* d_out, uint32_t cap, uint32_t* d_count,
                       void* stream);
@@ -334,6 +335,26 @@ int ccs_filter_launch(ccs_ctx* ctx, const ccs_shape* shape, ccs_u128 tile_start,
 int ccs_filter(ccs_ctx* ctx, const ccs_shape* shape, ccs_u128 tile_start, uint32_t tile_count,
                ccs_survivor* h_out, uint32_t cap, uint32_t* h_count, ccs_filter_stats* stats);
 int ccs_last_stats(ccs_ctx* ctx, ccs_filter_stats* out);
+
+/* Stage-B overflow of asynchronous launches (the reference warns and clamps on survivor
+ * overflow, v13.cu:3846-3852; v16 aborts, v16...cu:5729-5743).  The candidates that pass the
+ * dense stage and stage B wait in a device buffer sized from the shape's expected survival
+ * (twice the expectation + 2^20 entries) before the deep stage; a launch with more of them
+ * drops the excess, and its *d_count then undercounts.  Every such launch is recorded in a
+ * sticky status of the stream's (or job set's) launch scratch.  ccs_stream_status /
+ * ccs_jobset_status wait for the stream, return the status of the launches since the last
+ * call and clear it: CCS_OK when none overflowed, CCS_OVERFLOW when some did. */
+typedef struct {
+    uint32_t overflowed_launches;   /* launches whose stage-B survivors exceeded the buffer  */
+    uint32_t worst_stage_b;         /* the largest true stage-B count among them             */
+    uint64_t dropped;               /* stage-B survivors lost over those launches            */
+    uint32_t stage_b_cap;           /* entries of the buffer                                 */
+    uint32_t _pad;
+} ccs_launch_status;
+int ccs_stream_status(ccs_ctx* ctx, void* stream, ccs_launch_status* out);
+/* Test knob (no reference counterpart): launches on the context use at most `entries` of
+ * their stage-B buffer (0 = all of it), so tests can force the overflow path. */
+int ccs_debug_limit_stage_b(ccs_ctx* ctx, uint32_t entries);
 
 /* ---- prove (replaces cc18_prove_kernel v13.cu:1462-1538 and the GMP prover pool
  *      prover_worker :1794-1924): one ccs_chain_result per survivor.  The launch reads
@@ -410,6 +431,9 @@ int ccs_jobset_launch_each(ccs_ctx* ctx, ccs_jobset* js, ccs_job_survivor* d_out
                            uint32_t* d_count, void* stream);
 /* counters of the last launch on the set (synchronises) */
 int ccs_jobset_stats(ccs_ctx* ctx, ccs_jobset* js, uint32_t* d_count, ccs_filter_stats* out);
+/* stage-B overflow status of the set's launches since the last call, once `stream` is done
+ * (see ccs_stream_status) */
+int ccs_jobset_status(ccs_ctx* ctx, ccs_jobset* js, void* stream, ccs_launch_status* out);
 int ccs_graph_create(ccs_ctx* ctx, ccs_jobset* js, ccs_job_survivor* d_out, uint32_t cap,
                      uint32_t* d_count, ccs_graph** out);
 int ccs_graph_launch(ccs_ctx* ctx, ccs_graph* graph, void* stream);
@@ -436,13 +460,18 @@ int ccs_prove_jobs(ccs_ctx* ctx, const ccs_shape* shape, const ccs_job* h_jobs, 
 /* ---- served form: continuous-batching request loop (no counterpart in the reference;
  *      closest is the v16 dispatch ring + prove pool, v16...cu:2938-3279, :5594-5965).
  *      The context is owned by the server's thread until ccs_server_shutdown; submit,
- *      poll, results and drain may be called from any one client thread. ---- */
+ *      poll, results and drain may be called from any one client thread.  The shapes must
+ *      not be retuned while a server uses them. ---- */
 typedef struct ccs_server ccs_server;
 
+#define CCS_SERVER_RECOMPUTE_PHASES 1u  /* no resident residue state: every work unit computes
+                                           its sieve phases from its u (the comparison mode) */
+
 typedef struct {
-    uint64_t mem_budget_bytes;      /* device bytes for admitted jobs' resident state      */
+    uint64_t mem_budget_bytes;      /* device bytes for admitted jobs' resident state: a job  */
+                                    /*   is admitted while a slot of it is free (counters.job_bytes) */
     uint32_t max_active;            /* admitted jobs at most (and slice table size)         */
-    uint32_t surv_per_job;          /* survivor + result slots reserved per admitted job    */
+    uint32_t flags;                 /* CCS_SERVER_*                                         */
     uint32_t max_slice_survivors;   /* survivor buffer per slice and depth group            */
     uint32_t pad;
     uint64_t slice_candidates;      /* target candidates per slice over all admitted jobs   */
@@ -456,27 +485,40 @@ typedef struct {
     int32_t bits;                   /* informational: the width the range was drawn from    */
     uint64_t a;                     /* residue class mod W of the depth's shape              */
     ccs_u128 u0;                    /* n = a + W*(u0 + j), j in [0, count)                   */
-    uint64_t count;                 /* any length; sliced by the server                      */
+    uint64_t count;                 /* below 2^48; sliced by the server                      */
 } ccs_request;
 
+#define CCS_JOB_INCOMPLETE 1u       /* survivors of the job were lost to a full stage-B or */
+                                    /*   slice buffer: its results are a subset             */
 typedef struct {
     uint64_t job_id, tag, candidates;
-    uint32_t n_survivors, n_primes, best_len, pad;
+    uint32_t n_survivors, n_primes, best_len, flags;   /* flags: CCS_JOB_*                  */
     double t_submit, t_admit, t_done;   /* steady-clock seconds */
 } ccs_job_done;
 
 typedef struct {
-    uint64_t candidates, slices, launches, resident_bytes;
+    uint64_t candidates, slices, launches;
+    uint64_t resident_bytes;        /* device bytes of the admitted jobs' resident state     */
     uint32_t active, pending;
+    uint64_t job_bytes;             /* resident device state of one admitted job: its sieve  */
+                                    /*   phases and its rows in both lanes' slice tables     */
+    uint64_t fixed_bytes;           /* the server's slice survivor and result buffers        */
+    uint32_t slots;                 /* jobs the memory budget admits at once                 */
+    uint32_t incomplete;            /* jobs finished with CCS_JOB_INCOMPLETE                 */
+    double busy_s;                  /* seconds with admitted work or slices in flight        */
 } ccs_server_counters;
 
 int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, const int32_t* depths,
                       uint32_t n_shapes, const ccs_server_params* params, ccs_server** out);
 int ccs_server_submit(ccs_server* server, const ccs_request* req, uint64_t* job_id);
+/* finished jobs since the last poll; CCS_OVERFLOW when one of them is CCS_JOB_INCOMPLETE */
 int ccs_server_poll(ccs_server* server, ccs_job_done* h_out, uint32_t cap, uint32_t* n);
-/* survivors (u_off within the request; job_idx 0) and their prove results of a finished job */
-int ccs_server_job_results(ccs_server* server, uint64_t job_id, ccs_job_survivor* h_surv,
+/* survivors of a finished job as u offsets within the request (n = a + W*(u0 + u_off)),
+ * with their prove results (survivor_idx indexes h_u_off) */
+int ccs_server_job_results(ccs_server* server, uint64_t job_id, uint64_t* h_u_off,
                            ccs_chain_result* h_res, uint32_t cap, uint32_t* n);
+/* waits until every submitted job has finished; CCS_OVERFLOW when some job so far finished
+ * CCS_JOB_INCOMPLETE */
 int ccs_server_drain(ccs_server* server);
 int ccs_server_stats(ccs_server* server, ccs_server_counters* out);
 int ccs_server_shutdown(ccs_server* server);

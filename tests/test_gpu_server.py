@@ -41,11 +41,12 @@ def run_server(ctx, shapes, reqs, stagger=0.0, **kw):
             done[d["job_id"]] = d
     out = {}
     for jid, r in zip(ids, reqs):
-        surv, res = srv.results(jid)
-        order = np.argsort(surv["u_off"], kind="stable")
-        out[r["id"]] = (surv["u_off"][order].tolist(), res[order])
+        u_off, res = srv.results(jid)
+        order = np.argsort(u_off, kind="stable")
+        out[r["id"]] = ([int(x) for x in u_off[order]], res[order])
         assert done[jid]["t_done"] >= done[jid]["t_admit"] >= done[jid]["t_submit"]
-        assert done[jid]["n_survivors"] == len(surv)
+        assert done[jid]["n_survivors"] == len(u_off)
+        assert done[jid]["flags"] == 0
     st = srv.stats()
     srv.shutdown()
     return out, st
@@ -82,19 +83,82 @@ def test_alone_equals_mixed_and_oracle(ctx, oracle, shapes):
 
 def test_admission_by_memory_budget(ctx, shapes):
     reqs = [r for r in WL.served_requests(n_jobs=40, seed=0x5F, lo_exp=14, hi_exp=18) if r["depth"] in shapes][:12]
+    probe = ccs.Server(ctx, shapes, slice_candidates=1 << 20, min_chunk=1 << 14)
+    st = probe.stats()
+    probe.shutdown()
+    # one job's resident state: the phases of its widest profile's sieve primes (at least the
+    # stage-B primes of depth 12) plus its rows in both lanes' slice tables
+    assert st["job_bytes"] >= 4 * int(shapes[12].info.n_tierb) + 2 * 80
     # a budget for two resident jobs: the rest wait for admission, all complete
-    srv = ccs.Server(ctx, shapes, mem_budget_bytes=2 * 2 * (32 + 16 * 40), slice_candidates=1 << 20,
+    srv = ccs.Server(ctx, shapes, mem_budget_bytes=2 * st["job_bytes"] + 100, slice_candidates=1 << 20,
                      min_chunk=1 << 14)
+    assert srv.stats()["slots"] == 2
     for r in reqs:
         srv.submit(r["depth"], r["a"], r["u0"], r["count"], tag=r["id"])
-    peak = 0
+    peak = peak_bytes = 0
     t_end = time.time() + 120
     seen = 0
     while seen < len(reqs) and time.time() < t_end:
         st = srv.stats()
         peak = max(peak, st["active"])
+        peak_bytes = max(peak_bytes, st["resident_bytes"])
         seen += len(srv.poll())
         time.sleep(0.001)
     srv.shutdown()
     assert seen == len(reqs)
-    assert peak <= 2
+    assert peak <= 2 and peak_bytes <= 2 * st["job_bytes"]
+
+
+def test_resident_state_equals_recomputed(ctx, shapes):
+    """the same jobs with resident phases and with phases computed per unit give the same
+    survivors and prove results (several slices per job, so the resident phases are
+    advanced across slices)"""
+    reqs = [r for r in WL.served_requests(n_jobs=60, seed=0x6A, lo_exp=20, hi_exp=24) if r["depth"] in shapes][:12]
+    say(f"[server] resident vs recomputed, {len(reqs)} jobs")
+    res_r, st_r = run_server(ctx, shapes, reqs, slice_candidates=1 << 24, min_chunk=1 << 16)
+    res_c, _ = run_server(ctx, shapes, reqs, slice_candidates=1 << 24, min_chunk=1 << 16, recompute=True)
+    assert st_r["slices"] > 4
+    for r in reqs:
+        assert res_r[r["id"]][0] == res_c[r["id"]][0]
+        for f in ("chain_len", "back_steps", "flags", "root_lo", "root_hi"):
+            assert np.array_equal(res_r[r["id"]][1][f], res_c[r["id"]][1][f]), f
+
+
+def test_survivor_offsets_beyond_2_32(ctx, oracle, shapes):
+    """a request longer than 2^32 candidates: survivors past offset 2^32 are reported at
+    their full 64-bit offset, equal to the oracle's class-form filter there"""
+    r = [x for x in WL.served_requests(n_jobs=40, seed=0x6B) if x["depth"] == 12][0]
+    count = (1 << 32) + (1 << 22)
+    say(f"[server] one depth-12 request of {count} candidates")
+    out, st = run_server(ctx, shapes, [dict(r, count=count)], slice_candidates=1 << 31)
+    u_off = out[r["id"]][0]
+    p = WL.shape_params("c5d12")
+    o = oracle.Shape(p["crt"], p["wheel"], p["target"], p["depth"], p["filter_primes"], p["pos0_max_q"])
+    lo = [u for u in u_off if u < (1 << 22)]
+    hi = [u - (1 << 32) for u in u_off if u >= (1 << 32)]
+    want_lo, _ = o.filter_class(r["a"], r["W"], r["u0"], 1 << 22)
+    want_hi, _ = o.filter_class(r["a"], r["W"], r["u0"] + (1 << 32), 1 << 22)
+    assert len(want_hi) > 0
+    assert lo == [int(x) for x in want_lo]
+    assert hi == [int(x) for x in want_hi]
+    assert st["candidates"] == count
+
+
+def test_overflow_marks_jobs_incomplete(ctx, shapes):
+    """a stage-B buffer too small for a slice: the slice's jobs finish flagged incomplete and
+    drain reports it"""
+    reqs = [r for r in WL.served_requests(n_jobs=60, seed=0x6C) if r["depth"] == 12][:3]
+    ctx.limit_stage_b(16)
+    try:
+        srv = ccs.Server(ctx, shapes, slice_candidates=1 << 24, min_chunk=1 << 16)
+        for r in reqs:
+            srv.submit(12, r["a"], r["u0"], 1 << 23, tag=r["id"])
+        complete = srv.drain()
+        done = srv.poll()
+        st = srv.stats()
+        srv.shutdown()
+    finally:
+        ctx.limit_stage_b(0)
+    assert not complete
+    assert len(done) == 3 and all(d["flags"] & ccs._lib.JOB_INCOMPLETE for d in done)
+    assert st["incomplete"] == 3
