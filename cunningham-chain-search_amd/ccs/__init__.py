@@ -254,6 +254,13 @@ class Shape:
         L.check(self.lib.ccs_prove_launch(self.ctx.h, self.h, L.U128.of(tile_start), d_surv, d_count, cap,
                                           C.byref(pp), d_out, stream), "ccs_prove_launch")
 
+    def chain_roots_launch(self, d_surv, d_res, d_count, cap, d_out, out_cap, d_root_count, d_prime_count=None,
+                           stream=None):
+        """The v13 K2 output (ChainResult records of the chain roots) from prove results
+        (ccs_chain_roots_launch)."""
+        L.check(self.lib.ccs_chain_roots_launch(self.ctx.h, d_surv, d_res, d_count, cap, d_out, out_cap,
+                                                d_root_count, d_prime_count, stream), "chain_roots_launch")
+
     def prove(self, tile_start, surv, bits=0, test="bpsw", bases=(2,)):
         surv = np.ascontiguousarray(surv, SURVIVOR_DTYPE)
         out = np.zeros(len(surv), CHAIN_RESULT_DTYPE)

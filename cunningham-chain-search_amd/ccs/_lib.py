@@ -39,7 +39,7 @@ EXPORTS = [
     "ccs_campaign_save", "ccs_campaign_resume", "ccs_family_shape_build", "ccs_family_window",
     "ccs_pool_parse", "ccs_pool_load", "ccs_dedup_u128", "ccs_family_top_gate_launch",
     "ccs_groupset_create", "ccs_groupset_update", "ccs_stream_status", "ccs_jobset_status",
-    "ccs_debug_limit_stage_b",
+    "ccs_debug_limit_stage_b", "ccs_chain_roots_launch",
 ]
 
 
@@ -220,6 +220,7 @@ JOB_SURVIVOR_DTYPE = np.dtype([("job_idx", "<u4"), ("u_off", "<u4")])
 CHAIN_RESULT_DTYPE = np.dtype([("survivor_idx", "<u4"), ("chain_len", "<u4"), ("back_steps", "<u4"),
                                ("flags", "<u4"), ("root_lo", "<u8"), ("root_hi", "<u8")])
 JOB_DTYPE = np.dtype([("a", "<u8"), ("u0_lo", "<u8"), ("u0_hi", "<u8"), ("count", "<u8")])
+CHAIN_ROOT_DTYPE = np.dtype([("base_idx", "<u4"), ("k_offset", "<u4"), ("tile_idx", "<u4"), ("chain_len", "<u4")])
 
 _lib = None
 
@@ -270,6 +271,7 @@ def load():
                              C.POINTER(FilterStats)]
     L.ccs_last_stats.argtypes = [vp, C.POINTER(FilterStats)]
     L.ccs_stream_status.argtypes = [vp, vp, C.POINTER(LaunchStatus)]
+    L.ccs_chain_roots_launch.argtypes = [vp, vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp, vp, vp]
     L.ccs_debug_limit_stage_b.argtypes = [vp, C.c_uint32]
     L.ccs_prove_launch.argtypes = [vp, vp, U128, vp, vp, C.c_uint32, C.POINTER(ProveParams), vp, vp]
     L.ccs_prove.argtypes = [vp, vp, U128, vp, C.c_uint32, C.POINTER(ProveParams), vp]

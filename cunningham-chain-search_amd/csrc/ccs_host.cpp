@@ -1274,6 +1274,26 @@ extern "C" int ccs_prove_launch(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, con
     return CCS_OK;
 }
 
+extern "C" int ccs_chain_roots_launch(ccs_ctx* c, const ccs_survivor* d_surv, const ccs_chain_result* d_res,
+                                      const uint32_t* d_count, uint32_t cap, ccs_chain_root* d_out, uint32_t out_cap,
+                                      uint32_t* d_root_count, uint32_t* d_prime_count, void* stream) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!d_surv || !d_res || !d_count || !d_out || !d_root_count) return fail(CCS_ERR_INVALID, "null argument");
+    RootsArgs a;
+    a.surv = d_surv;
+    a.res = d_res;
+    a.count = d_count;
+    a.cap = cap;
+    a.out = d_out;
+    a.out_cap = out_cap;
+    a.out_count = d_root_count;
+    a.primes = d_prime_count;
+    const hipError_t e = launch_chain_roots(a, stream ? (hipStream_t)stream : c->stream);
+    if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("chain roots launch: ") + hipGetErrorString(e));
+    return CCS_OK;
+}
+
 static int prove_host_common(ccs_ctx* c, ProveArgs& a, const void* h_surv, size_t rec_bytes, uint32_t n,
                              ccs_chain_result* h_out) {
     if (n == 0) return CCS_OK;

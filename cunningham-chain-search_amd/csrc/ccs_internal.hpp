@@ -245,6 +245,18 @@ struct TopGateArgs {
     u32* out_count;
 };
 
+// the v13 K2 output: chain roots of a plain launch's prove results
+struct RootsArgs {
+    const ccs_survivor* surv;
+    const ccs_chain_result* res;
+    const u32* count;
+    u32 cap;
+    ccs_chain_root* out;
+    u32 out_cap;
+    u32* out_count;
+    u32* primes;              // prime survivors (may be null)
+};
+
 // sets the text returned by ccs_last_error() and returns code (ccs_host.cpp)
 int set_error(int code, const char* msg);
 
@@ -257,6 +269,7 @@ hipError_t launch_zero(u32* p, size_t n_words, hipStream_t s);
 // zeroes up to three ranges of at most 256 words each (null pointers are skipped)
 hipError_t launch_zero3(u32* a, u32 na, u32* b, u32 nb, u32* c, u32 nc, hipStream_t s);
 hipError_t launch_top_gate(const TopGateArgs& a, hipStream_t s);
+hipError_t launch_chain_roots(const RootsArgs& a, hipStream_t s);
 hipError_t launch_phase_init(const PhaseInit* d_jobs, u32 n_jobs, u32* phases, u32 stride, hipStream_t s);
 
 // ---- served form hooks (ccs_host.cpp, for ccs_server.cpp) ----

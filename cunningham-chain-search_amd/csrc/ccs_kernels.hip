@@ -1056,6 +1056,43 @@ __global__ __launch_bounds__(256) void top_gate_kernel(TopGateArgs A) {
     }
 }
 
+// The v13 K2 output (cc18_prove_kernel, v13.cu:1462-1538): one ChainResult per prime
+// survivor that is a chain root ((n-1)/2 not prime: back_steps 0), with the forward length
+// from it; compacted a wave at a time (one atomic per wave); prime survivors counted.
+__global__ __launch_bounds__(256) void chain_roots_kernel(RootsArgs A) {
+    u32 n = *A.count;
+    if (n > A.cap) n = A.cap;
+    const u32 lane = lane_id();
+    const u32 stride = gridDim.x * blockDim.x;
+    for (u32 e0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); e0 < n; e0 += stride) {
+        const u32 i = e0 + lane;
+        bool prime = false, root = false;
+        ccs_chain_result r;
+        if (i < n) {
+            r = A.res[i];
+            prime = (r.flags & CCS_F_PRIME) != 0;
+            root = prime && r.back_steps == 0;
+        }
+        const u64 pm = ballot(prime), rm = ballot(root);
+        u32 base = 0;
+        if (lane == 0) {
+            if (pm && A.primes) atomicAdd(A.primes, (u32)__builtin_popcountll(pm));
+            if (rm) base = atomicAdd(A.out_count, (u32)__builtin_popcountll(rm));
+        }
+        base = bcast32(base);
+        if (!root) continue;
+        const u32 slot = base + lanes_below(rm);
+        if (slot >= A.out_cap) continue;
+        const ccs_survivor sv = A.surv[i];
+        ccs_chain_root o;
+        o.base_idx = sv.base_idx;
+        o.k_offset = sv.k_offset;
+        o.tile_idx = sv.tile_idx;
+        o.chain_len = r.chain_len;
+        A.out[slot] = o;
+    }
+}
+
 // resident phases of newly admitted served jobs: one block per job, a thread per prime
 __global__ __launch_bounds__(256) void phase_init_kernel(const PhaseInit* jobs, u32 n_jobs, u32* phases,
                                                          u32 stride) {
@@ -1168,6 +1205,17 @@ hipError_t launch_phase_init(const PhaseInit* d_jobs, u32 n_jobs, u32* phases, u
     if (!n_jobs) return hipSuccess;
     const u32 blocks = n_jobs < 1024u ? n_jobs : 1024u;
     hipLaunchKernelGGL(phase_init_kernel, dim3(blocks), dim3(256), 0, s, d_jobs, n_jobs, phases, stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_chain_roots(const RootsArgs& a, hipStream_t s) {
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(zero3_kernel, dim3(1), dim3(256), 0, s, a.out_count, 1u, a.primes, a.primes ? 1u : 0u,
+                       (u32*)nullptr, 0u);
+    u32 blocks = (a.cap + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(chain_roots_kernel, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

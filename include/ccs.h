@@ -369,6 +369,23 @@ int ccs_prove(ccs_ctx* ctx, const ccs_shape* shape, ccs_u128 tile_start,
               const ccs_survivor* h_surv, uint32_t n, const ccs_prove_params* params,
               ccs_chain_result* h_out);
 
+/* The reference's chain result record (v13 ChainResult, v13.cu:218-223): a chain root among
+ * a launch's survivors, named by its Survivor fields, and the chain length from it. */
+typedef struct {
+    uint32_t base_idx, k_offset, tile_idx, chain_len;
+} ccs_chain_root;
+
+/* The v13 K2 output (cc18_prove_kernel, v13.cu:1462-1538) from prove results: one
+ * ccs_chain_root per prime survivor whose (n-1)/2 is not prime (back_steps 0), chain_len =
+ * its length forward, in unspecified order.  Reads min(*d_count, cap) survivors and their
+ * results (same index); *d_root_count = the true number of roots (at most out_cap written),
+ * *d_prime_count (optional) = the prime survivors (v13 gpu_primes_count).  Both counts are
+ * set by the launch.  A non-root survivor, which the GMP engine walks back to its true root,
+ * is left out here, as K2 leaves it out. */
+int ccs_chain_roots_launch(ccs_ctx* ctx, const ccs_survivor* d_surv, const ccs_chain_result* d_res,
+                           const uint32_t* d_count, uint32_t cap, ccs_chain_root* d_out, uint32_t out_cap,
+                           uint32_t* d_root_count, uint32_t* d_prime_count, void* stream);
+
 /* Format accepted chains like the v13 results file (v13.cu:2030-2033):
  * "[TARGET ][NON-ROOT ]CC<len> 0x<HEX> <bits>\n" for len >= log_threshold or >= target.
  * Returns the number of bytes needed (excluding NUL). */

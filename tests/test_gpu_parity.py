@@ -531,3 +531,51 @@ def test_host_confirm_equals_gpu_prove(ctx, ref):
         h, st = ref.confirm_host(t0, surv, bits=bits)
         assert np.array_equal(g, h)
         assert st == ccs.chain_stats(g) and st["primes"] > 0
+
+
+def test_chain_roots_are_the_v13_k2_output(ctx, oracle, ref, oref):
+    """ccs_chain_roots_launch gives the v13 K2 records (v13.cu:1462-1538): one
+    {base_idx, k_offset, tile_idx, chain_len} per prime survivor whose (n-1)/2 is not prime,
+    and the count of prime survivors, from the device survivors and prove results"""
+    t0, _ = WL.reference_range(seed=0x9B, bits=89)
+    cap, pcap = 1 << 16, 4096
+    d_s, d_c, d_r = ctx.dalloc(cap * 16), ctx.dalloc(64), ctx.dalloc(pcap * 32)
+    d_o, d_n = ctx.dalloc(pcap * 16), ctx.dalloc(64)
+    try:
+        ref.filter_launch(t0, 6000, d_s, cap, d_c)
+        ref.prove_launch(t0, d_s, d_c, pcap, d_r, bits=89)
+        ref.chain_roots_launch(d_s, d_r, d_c, pcap, d_o, pcap, d_n, ctypes_ptr(d_n, 4))
+        ctx.synchronize()
+        cnt = np.zeros(2, np.uint32)
+        ctx.d2h(cnt, d_n)
+        n = np.zeros(1, np.uint32)
+        ctx.d2h(n, d_c)
+        surv = np.zeros(n[0], ccs.SURVIVOR_DTYPE)
+        ctx.d2h(surv, d_s)
+        roots = np.zeros(cnt[0], ccs._lib.CHAIN_ROOT_DTYPE)
+        ctx.d2h(roots, d_o)
+        # a launch whose output holds fewer roots still reports their true count
+        ref.chain_roots_launch(d_s, d_r, d_c, pcap, d_o, 2, d_n, None)
+        ctx.synchronize()
+        small = np.zeros(1, np.uint32)
+        ctx.d2h(small, d_n)
+    finally:
+        for ptr in (d_s, d_c, d_r, d_o, d_n):
+            ctx.dfree(ptr)
+    o_surv = np.stack([surv["base_idx"], surv["k_offset"], surv["tile_idx"], surv["_pad"]], 1)
+    want_res = oref.prove(t0, o_surv, bits=89)
+    prime = (want_res[:, 0] & oracle.F_PRIME) != 0
+    root = prime & (want_res[:, 2] == 0)
+    want = sorted((int(s[0]), int(s[1]), int(s[2]), int(r[1])) for s, r, ok in zip(o_surv, want_res, root) if ok)
+    got = sorted(zip(roots["base_idx"].tolist(), roots["k_offset"].tolist(), roots["tile_idx"].tolist(),
+                     roots["chain_len"].tolist()))
+    say(f"[roots] {len(surv)} survivors, {int(prime.sum())} prime, {len(want)} roots")
+    assert len(want) > 10 and got == want
+    assert cnt[1] == int(prime.sum())
+    assert small[0] == len(want)
+
+
+def ctypes_ptr(p, offset):
+    """device pointer p + offset bytes"""
+    import ctypes
+    return ctypes.c_void_p(p.value + offset)
