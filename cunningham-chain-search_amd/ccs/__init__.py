@@ -242,6 +242,12 @@ class Shape:
             raise CcsError(f"survivor overflow: {n.value} > cap {cap}")
         return out[: n.value].copy(), st.as_dict()
 
+    def filter_launch_noalloc(self, tile_start, tile_count, d_out, cap, d_count, stream=None):
+        """ccs_filter_launch with no device allocation (ccs_filter_launch_noalloc): the
+        library's fixed per-device scratch; *d_count = 0xFFFFFFFF on a stage-B overflow"""
+        L.check(self.lib.ccs_filter_launch_noalloc(self.ctx.h, self.h, L.U128.of(tile_start), tile_count, d_out,
+                                                   cap, d_count, stream), "ccs_filter_launch_noalloc")
+
     def filter_launch(self, tile_start, tile_count, d_out, cap, d_count, stream=None):
         """Asynchronous plain filter into device buffers (ccs_filter_launch)."""
         L.check(self.lib.ccs_filter_launch(self.ctx.h, self.h, L.U128.of(tile_start), tile_count,

@@ -39,7 +39,7 @@ EXPORTS = [
     "ccs_campaign_save", "ccs_campaign_resume", "ccs_family_shape_build", "ccs_family_window",
     "ccs_pool_parse", "ccs_pool_load", "ccs_dedup_u128", "ccs_family_top_gate_launch",
     "ccs_groupset_create", "ccs_groupset_update", "ccs_stream_status", "ccs_jobset_status",
-    "ccs_debug_limit_stage_b", "ccs_chain_roots_launch",
+    "ccs_debug_limit_stage_b", "ccs_chain_roots_launch", "ccs_filter_launch_noalloc",
 ]
 
 
@@ -272,6 +272,7 @@ def load():
     L.ccs_last_stats.argtypes = [vp, C.POINTER(FilterStats)]
     L.ccs_stream_status.argtypes = [vp, vp, C.POINTER(LaunchStatus)]
     L.ccs_chain_roots_launch.argtypes = [vp, vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp, vp, vp]
+    L.ccs_filter_launch_noalloc.argtypes = [vp, vp, U128, C.c_uint32, vp, C.c_uint32, vp, vp]
     L.ccs_debug_limit_stage_b.argtypes = [vp, C.c_uint32]
     L.ccs_prove_launch.argtypes = [vp, vp, U128, vp, vp, C.c_uint32, C.POINTER(ProveParams), vp, vp]
     L.ccs_prove.argtypes = [vp, vp, U128, vp, C.c_uint32, C.POINTER(ProveParams), vp]

@@ -87,6 +87,7 @@ struct ccs_ctx {
     size_t ev_used = 0;
     std::vector<std::pair<int, size_t>> marks;   // (kind, index of the begin event)
     u32 stage_b_limit = 0;              // test knob: stage-B buffer entries used (0 = all)
+    unsigned char* noalloc = nullptr;   // this device's copy of the no-allocation scratch
 };
 
 // begin/end events of one kernel launch while profiling (kind: 0 sieve, 1 deep, 2 prove)
@@ -188,6 +189,24 @@ static PrimeMod make_primemod(u32 q, u128 W) {
     return pm;
 }
 
+// ---- the no-allocation launch (ccs_filter_launch_noalloc): its stage-B buffer, counters
+// and statistics live in this file-scope device array; each device holds its own copy until
+// the process ends.  Host code uses the address hipGetSymbolAddress gives for the current
+// device (kept in the context), never the array's name.
+constexpr u32 kNoAllocMid = 1u << 20;        // stage-B entries (8 MiB)
+constexpr size_t kNoAllocHead = 256;         // counters (64 B) and statistics (64 B) first
+static __device__ __align__(256) unsigned char scratch[kNoAllocHead + (size_t)kNoAllocMid * sizeof(MidRec)];
+
+// once per context, with its device current (ccs_ctx_create)
+static int noalloc_setup(ccs_ctx* c) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(CCS_ERR_HIP, std::string("no-allocation scratch address: ") + hipGetErrorString(e));
+    c->noalloc = (unsigned char*)p;
+    return CCS_OK;
+}
+
 extern "C" int ccs_device_count(int* out) {
     if (!out) return fail(CCS_ERR_INVALID, "null");
     int n = 0;
@@ -223,6 +242,11 @@ extern "C" int ccs_ctx_create(int device, ccs_ctx** out) {
         const std::string msg = g_err;
         ccs_ctx_destroy(c);
         return fail(CCS_ERR_NOMEM, msg);
+    }
+    if (noalloc_setup(c) != CCS_OK) {
+        const std::string msg = g_err;
+        ccs_ctx_destroy(c);
+        return fail(CCS_ERR_HIP, msg);
     }
     // prove constants: small primes 2..61, trial primes 67..409
     static const u32 small[18] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61};
@@ -976,6 +1000,15 @@ struct Scratch {
 
 static Scratch to_scratch(const StreamScratch* x) { return Scratch{x->mid, x->mid_cap, x->counters, x->stats}; }
 
+static Scratch noalloc_scratch(const ccs_ctx* c) {
+    Scratch x;
+    x.counters = (u32*)c->noalloc;
+    x.stats = (unsigned long long*)(c->noalloc + 64);
+    x.mid = (MidRec*)(c->noalloc + kNoAllocHead);
+    x.mid_cap = kNoAllocMid;
+    return x;
+}
+
 // the stage-B capacity a launch may use: all of the buffer, or less under the test knob
 static Scratch limited(const ccs_ctx* c, Scratch x) {
     if (c->stage_b_limit && c->stage_b_limit < x.mid_cap) x.mid_cap = c->stage_b_limit;
@@ -1028,14 +1061,18 @@ static void fill_sieve_common(const ccs_shape* s, const Scratch& x, SieveArgs& a
 
 // zero the launch counters, then sieve + deep.  zero_out also clears the caller's count
 // (the per-job loop appends several launches into one output).
+// (fixed: the launch runs in the no-allocation scratch: its status words are cleared with
+// the counters, and no timing events are recorded)
 static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, SieveArgs& sa, DeepArgs& da,
                               u64 n_units, u32* d_count, hipStream_t st, bool zero_out = true,
-                              bool zero_stats = true, u32 nb_max = 0) {
+                              bool zero_stats = true, u32 nb_max = 0, bool fixed = false) {
+    if (n_units >> 32) return fail(CCS_ERR_INVALID, "a launch takes fewer than 2^32 work units");
     // fill kernels rather than memset nodes, so the sequence can be graph-captured
-    if (launch_zero3(x.counters, kLaunchWords, zero_stats ? (u32*)x.stats : nullptr, 16,
+    if (launch_zero3(x.counters, fixed ? 16u : kLaunchWords, zero_stats ? (u32*)x.stats : nullptr, 16,
                      zero_out ? d_count : nullptr, 1, st) != hipSuccess)
         return fail(CCS_ERR_HIP, "zero");
-    if (n_units >> 32) return fail(CCS_ERR_INVALID, "a launch takes fewer than 2^32 work units");
+    const bool timed = c->timed && !fixed, prof = c->profiling;
+    if (fixed) c->profiling = false;
     sa.n_units = n_units;
     // a group launch takes its stage-B register width from its widest shape (nb_max)
     const u32 nbm = nb_max > s->nb ? nb_max : s->nb;
@@ -1046,13 +1083,16 @@ static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, 
     u64 want_grid = (u64)c->n_cu * s->tune.blocks_per_cu;
     const u64 blocks_needed = (n_units + wpb - 1) / wpb;
     if (want_grid > blocks_needed) want_grid = blocks_needed ? blocks_needed : 1;
-    c->last_timed = c->timed;
-    if (c->timed) HIPCHK(hipEventRecord(c->ev[0], st));
+    c->last_timed = timed;
+    if (timed) HIPCHK(hipEventRecord(c->ev[0], st));
     prof_begin(c, 0, st);
     hipError_t e = launch_sieve(sa, (int)s->tune.rows_per_group, nbv, (int)want_grid, block, lds, st);
-    if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("sieve launch: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        c->profiling = prof;
+        return fail(CCS_ERR_HIP, std::string("sieve launch: ") + hipGetErrorString(e));
+    }
     prof_end(c, st);
-    if (c->timed) HIPCHK(hipEventRecord(c->ev[1], st));
+    if (timed) HIPCHK(hipEventRecord(c->ev[1], st));
     da.mid = x.mid;
     da.mid_count = x.counters + 0;
     da.mid_cap = x.mid_cap;
@@ -1061,14 +1101,17 @@ static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, 
     da.status = x.counters + kStatusWord;
     prof_begin(c, 1, st);
     e = launch_deep(da, x.mid_cap, st);
+    c->profiling = prof;
     if (e != hipSuccess) return fail(CCS_ERR_HIP, std::string("deep launch: ") + hipGetErrorString(e));
     prof_end(c, st);
-    if (c->timed) HIPCHK(hipEventRecord(c->ev[2], st));
+    if (timed) HIPCHK(hipEventRecord(c->ev[2], st));
     return CCS_OK;
 }
 
+// fixed: the no-allocation scratch to run in (null: the stream's own launch scratch)
 static int filter_launch_impl(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint32_t tile_count,
-                              ccs_survivor* d_out, uint32_t cap, uint32_t* d_count, hipStream_t st) {
+                              ccs_survivor* d_out, uint32_t cap, uint32_t* d_count, hipStream_t st,
+                              const Scratch* fixed = nullptr) {
     if (!s || !d_out || !d_count) return fail(CCS_ERR_INVALID, "null argument");
     if (s->classes.empty()) return fail(CCS_ERR_INVALID, "shape built without classes");
     const u128 T0 = mk128(ts);
@@ -1079,11 +1122,21 @@ static int filter_launch_impl(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint3
     const u64 n_units = cpc * s->classes.size();
     if (n_units >> 32) return fail(CCS_ERR_INVALID, "launch has 2^32 work units or more: fewer tiles per launch");
     const u64 cand = ccs_count_candidates(s, ts, tile_count);
-    StreamScratch* ss = stream_scratch(c, st);
-    if (!ss) return CCS_ERR_NOMEM;
-    int rc = ensure_mid(ss, (u64)(cand * s->est_survival_b * 2.0) + (1u << 20));
-    if (rc) return rc;
-    const Scratch x = limited(c, to_scratch(ss));
+    Scratch x;
+    if (fixed) {
+        const double want = cand * s->est_survival_b * 2.0 + (1u << 16);
+        if (want > fixed->mid_cap)
+            return fail(CCS_ERR_INVALID, "tile range too large for the no-allocation scratch: " +
+                                             std::to_string((u64)want) + " stage-B entries wanted, " +
+                                             std::to_string(fixed->mid_cap) + " held");
+        x = *fixed;
+    } else {
+        StreamScratch* ss = stream_scratch(c, st);
+        if (!ss) return CCS_ERR_NOMEM;
+        int rc = ensure_mid(ss, (u64)(cand * s->est_survival_b * 2.0) + (1u << 20));
+        if (rc) return rc;
+        x = limited(c, to_scratch(ss));
+    }
     SieveArgs sa;
     fill_sieve_common(s, x, sa);
     sa.mode = 0;
@@ -1110,7 +1163,11 @@ static int filter_launch_impl(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint3
     c->last.candidates = cand;
     c->last.dense_end = s->nd_primes;
     c->last.tierb_end = s->nd_primes + s->nb;
-    return run_filter_kernels(c, s, x, sa, da, n_units, d_count, st);
+    const int rc = run_filter_kernels(c, s, x, sa, da, n_units, d_count, st, true, true, 0, fixed != nullptr);
+    if (rc || !fixed) return rc;
+    if (launch_mark_overflow(x.counters + kStatusWord, d_count, st) != hipSuccess)
+        return fail(CCS_ERR_HIP, "overflow mark launch");
+    return CCS_OK;
 }
 
 extern "C" int ccs_filter_launch(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint32_t tile_count,
@@ -1120,6 +1177,17 @@ extern "C" int ccs_filter_launch(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, ui
     c->timed = false;
     return filter_launch_impl(c, s, ts, tile_count, d_out, cap, d_count,
                               stream ? (hipStream_t)stream : c->stream);
+}
+
+extern "C" int ccs_filter_launch_noalloc(ccs_ctx* c, const ccs_shape* s, ccs_u128 ts, uint32_t tile_count,
+                                         ccs_survivor* d_out, uint32_t cap, uint32_t* d_count, void* stream) {
+    int rc = ctx_check(c);
+    if (rc) return rc;
+    if (!c->noalloc) return fail(CCS_ERR_HIP, "no-allocation scratch unavailable");
+    c->timed = false;
+    const Scratch x = noalloc_scratch(c);
+    return filter_launch_impl(c, s, ts, tile_count, d_out, cap, d_count,
+                              stream ? (hipStream_t)stream : c->stream, &x);
 }
 
 static int ensure_buf(void** p, u64* have, u64 want) {

@@ -270,6 +270,7 @@ hipError_t launch_zero(u32* p, size_t n_words, hipStream_t s);
 hipError_t launch_zero3(u32* a, u32 na, u32* b, u32 nb, u32* c, u32 nc, hipStream_t s);
 hipError_t launch_top_gate(const TopGateArgs& a, hipStream_t s);
 hipError_t launch_chain_roots(const RootsArgs& a, hipStream_t s);
+hipError_t launch_mark_overflow(const u32* status, u32* count, hipStream_t s);
 hipError_t launch_phase_init(const PhaseInit* d_jobs, u32 n_jobs, u32* phases, u32 stride, hipStream_t s);
 
 // ---- served form hooks (ccs_host.cpp, for ccs_server.cpp) ----

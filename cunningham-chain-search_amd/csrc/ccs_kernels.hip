@@ -1103,6 +1103,11 @@ __global__ __launch_bounds__(256) void phase_init_kernel(const PhaseInit* jobs, 
     }
 }
 
+// a no-allocation launch whose stage-B buffer overflowed reports it through its count
+__global__ void mark_overflow_kernel(const u32* status, u32* count) {
+    if (threadIdx.x == 0 && status[0]) *count = 0xffffffffu;
+}
+
 __global__ void zero_kernel(u32* p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0;
@@ -1216,6 +1221,12 @@ hipError_t launch_chain_roots(const RootsArgs& a, hipStream_t s) {
     if (blocks < 1) blocks = 1;
     if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(chain_roots_kernel, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mark_overflow(const u32* status, u32* count, hipStream_t s) {
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(mark_overflow_kernel, dim3(1), dim3(64), 0, s, status, count);
     return hipGetLastError();
 }
 

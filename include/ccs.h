@@ -331,6 +331,18 @@ int ccs_campaign_resume(ccs_campaign* c, const char* path);
 int ccs_filter_launch(ccs_ctx* ctx, const ccs_shape* shape, ccs_u128 tile_start,
                       uint32_t tile_count, ccs_survivor* d_out, uint32_t cap, uint32_t* d_count,
                       void* stream);
+/* The same launch with no device allocation (no counterpart in the reference; for callers
+ * that cannot allocate: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its stage-B buffer (2^20 entries), counters and statistics are one
+ * fixed array of this library in each device's memory (8 MiB + 256 B, held until the process
+ * ends); the caller passes d_out and d_count as for ccs_filter_launch.  Tile ranges whose
+ * expected stage-B survivors (twice the expectation + 2^16) exceed the buffer are rejected
+ * with CCS_ERR_INVALID; a launch whose stage-B survivors still overflow it sets *d_count to
+ * 0xFFFFFFFF.  Every call on one device shares that device's array: calls that overlap on one
+ * device (other streams, other contexts, other threads) race. */
+int ccs_filter_launch_noalloc(ccs_ctx* ctx, const ccs_shape* shape, ccs_u128 tile_start,
+                              uint32_t tile_count, ccs_survivor* d_out, uint32_t cap, uint32_t* d_count,
+                              void* stream);
 /* Synchronous host form with statistics. */
 int ccs_filter(ccs_ctx* ctx, const ccs_shape* shape, ccs_u128 tile_start, uint32_t tile_count,
                ccs_survivor* h_out, uint32_t cap, uint32_t* h_count, ccs_filter_stats* stats);
