@@ -140,9 +140,9 @@ def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_c
     for d, sh in shapes.items():
         sample = [(r["a"], r["u0"], min(r["count"], 1 << 27)) for r in tune_pool if r["depth"] == d][:192]
         tuned[d], _ = sh.autotune_jobs(sample, max_trials=tune_trials)
-        if tuned[d]["rows_per_group"] != 8:   # one row layout lets the server launch all depths at once
+        if tuned[d]["rows_per_group"] != 2:   # one row layout lets the server launch all depths at once
             try:
-                sh.set_tuning(rows_per_group=8)
+                sh.set_tuning(rows_per_group=2)
                 tuned[d] = sh.tuning
             except ccs.CcsError:
                 pass

@@ -1,5 +1,5 @@
-"""Build libccs.so (HIP kernels for gfx950 + the C ABI) in-tree with hipcc, and the
-ccs_search campaign CLI on top of it."""
+"""Build libccs.so (HIP kernels for gfx950 + the C ABI) in-tree with hipcc, the
+ccs_search campaign CLI on top of it, and the LDS throughput probe (lds_probe)."""
 import os
 import subprocess
 import sys
@@ -8,6 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libccs.so")
 CLI = os.path.join(HERE, "ccs_search")
+PROBE = os.path.join(HERE, "lds_probe")
 SOURCES = ["ccs_kernels.hip", "ccs_host.cpp", "ccs_server.cpp", "ccs_confirm.cpp", "ccs_campaign.cpp",
            "ccs_family.cpp"]
 HEADERS = ["ccs_internal.hpp", "ccs_arith.hpp", os.path.join("..", "..", "include", "ccs.h")]
@@ -51,6 +52,14 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(cli_tmp, CLI)
+    # the LDS throughput probe bench.py takes the dense stage's ceiling from
+    probe_tmp = PROBE + ".tmp"
+    cmd = [HIPCC, "-O3", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           os.path.join(CSRC, "lds_probe.hip"), "-o", probe_tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(probe_tmp, PROBE)
     return LIB
 
 

@@ -69,6 +69,7 @@ class ShapeInfo(C.Structure):
         ("n_dense", C.c_uint32), ("n_tierb", C.c_uint32), ("n_deep", C.c_uint32),
         ("rows_per_group", C.c_uint32), ("unit_j", C.c_uint32),
         ("W_hi", C.c_uint64), ("family", C.c_int32), ("exp_start", C.c_int32), ("D", U128),
+        ("n_slots", C.c_uint32), ("n_tierb_lds", C.c_uint32),
     ]
 
 

@@ -474,7 +474,10 @@ static void free_device_tables(ccs_shape* s) {
     s->d_hash = nullptr;
 }
 
-static u32 dense_ext_words(u32 q, u32 rg) { return ((q - 1) >> 5) + 64u * rg + 1u; }
+// words of a dense pattern of period q in LDS: the phase's word ((q-1) >> 5 at most, rounded
+// down to a multiple of 4) + 8 words per lane-window + 256 words per further row, padded to
+// a multiple of 4 so that every pattern starts 16-byte aligned
+static u32 dense_ext_words(u32 q, u32 rg) { return (((q - 1) >> 5) + 256u * rg + 4u + 3u) & ~3u; }
 
 // A dense slot: up to three consecutive dense primes sharing one pattern, whose period is
 // their product (at most pair_max <= 2^18, the bound of the slot's residue constants).
@@ -527,37 +530,39 @@ static u32 tierb_in_lds(const ccs_shape* s, const ccs_tuning* t) {
     return n;
 }
 
-// Starting point measured on the 31# depth-18 workload: dense primes below 200 (320 at
-// shallow depths) folded up to period 2^16, eight rows per lane, stage B sized by the
-// expected survival (below), 2^20-candidate units, one block of 16 waves per CU.  Shrunk
-// until the tables fit the device's LDS.
+// Starting point measured on the 31# depth-18 workload (config 2) and the 37# depth-19 one
+// (config 3) by sweeps of the sieve with 128-bit dense reads: dense primes below 240 (320 at
+// shallow depths) folded up to period 2^16, four rows of 8192 candidates per step, stage B
+// sized by the expected survival (below), 2^20-candidate units, one block of 16 waves per
+// CU.  Shrunk until the tables fit the device's LDS (four rows take 4 KiB of pattern
+// extension per slot, so dense-heavy shapes fall back to two rows).
 static void default_tuning(const ccs_shape* s, ccs_tuning* t) {
     u32 nd = 0;
-    while (nd < s->fp.size() && nd < 2u * (u32)kMaxDense && s->fp[nd] < 200) ++nd;
+    while (nd < s->fp.size() && nd < 2u * (u32)kMaxDense && s->fp[nd] < 240) ++nd;
     // At a shallow depth many candidates survive the dense primes, and stage B's cost per
     // survivor dwarfs a dense pattern's per candidate: when more than 0.25 % survive the
-    // primes below 200, the dense stage takes the primes below 320 as well (measured on
-    // the 31# profiles: depths 12 and 14 -24 %, depth 16 better with 200).
+    // dense primes, the dense stage takes the primes below 320 as well (measured on the 31#
+    // profiles: depths 12 and 14 -24 %).
     double sd = 1.0;
     for (u32 i = 0; i < nd; ++i) sd *= 1.0 - (double)s->forb[i].size() / s->fp[i];
     if (sd > 2.5e-3)
         while (nd < s->fp.size() && nd < 2u * (u32)kMaxDense && s->fp[nd] < 320) ++nd;
-    // Stage B: the smallest multiple of 64 primes (at least 128) after which at most 4e-6 of
+    // Stage B: the smallest multiple of 64 primes (at least 128) after which at most 1e-6 of
     // the candidates are expected to survive, up to 1024 primes below 16384; the deep stage
-    // takes the rest (with its masks above 1e-6 survival).  Measured: config 2 (31#, depth
-    // 18) 192 primes +3 % over 512 (128: +1.5 %, 256: +0.4 %, 64: -7 %); config 3 (37#,
-    // depth 19) 128 primes +7 % over 512; the shallow 31# depths 12 and 14 take 1024.
+    // takes the rest (with its masks above 1e-6 survival).  Measured per 2^36 candidates:
+    // config 2 320 primes 4.16 ms per step (256: 4.23, 192: 4.27, 128: 4.36); config 3 192
+    // primes 20.75 ms per 2^38 (320: 21.28); the shallow 31# depths 12 and 14 take 1024.
     u32 nb = 0;
     double surv = 1.0;
     for (u32 i = 0; i < nd; ++i) surv *= 1.0 - (double)s->forb[i].size() / s->fp[i];
     while (nd + nb < s->fp.size() && nb < (u32)kMaxTierB && s->fp[nd + nb] < kTierBMaxQ) {
         surv *= 1.0 - (double)s->forb[nd + nb].size() / s->fp[nd + nb];
         ++nb;
-        if (nb >= 128 && nb % 64 == 0 && surv <= 4e-6) break;
+        if (nb >= 128 && nb % 64 == 0 && surv <= 1e-6) break;
     }
     t->n_dense = nd;
     t->n_tierb = nb;
-    t->rows_per_group = 8;
+    t->rows_per_group = 4;
     t->unit_j = 1u << 20;
     t->blocks_per_cu = 1;
     t->pair_max = 1u << 16;
@@ -565,7 +570,7 @@ static void default_tuning(const ccs_shape* s, ccs_tuning* t) {
     for (;;) {
         ccs_tuning tt = *t;
         if (validate_tuning(s, &tt) == CCS_OK) return;
-        if (t->rows_per_group > 2) t->rows_per_group >>= 1;
+        if (t->rows_per_group > 1) t->rows_per_group >>= 1;
         else if (t->pair_max > 4096) t->pair_max >>= 2;
         else if (t->n_tierb > 16) t->n_tierb -= 16;
         else if (t->n_dense > 8) --t->n_dense;
@@ -574,8 +579,8 @@ static void default_tuning(const ccs_shape* s, ccs_tuning* t) {
 }
 
 static int validate_tuning(const ccs_shape* s, ccs_tuning* t) {
-    if (t->rows_per_group != 1 && t->rows_per_group != 2 && t->rows_per_group != 4 && t->rows_per_group != 8)
-        return fail(CCS_ERR_INVALID, "rows_per_group must be 1, 2, 4 or 8");
+    if (t->rows_per_group != 1 && t->rows_per_group != 2 && t->rows_per_group != 4)
+        return fail(CCS_ERR_INVALID, "rows_per_group must be 1, 2 or 4");
     if (t->n_dense > 2u * kMaxDense) return fail(CCS_ERR_INVALID, "n_dense > 128");
     if (t->pair_max > (1u << 18)) return fail(CCS_ERR_INVALID, "pair_max > 2^18");
     const auto slots = dense_slots(s, t);
@@ -584,7 +589,7 @@ static int validate_tuning(const ccs_shape* s, ccs_tuning* t) {
     if (t->n_dense + t->n_tierb > s->fp.size())
         return fail(CCS_ERR_INVALID, "n_dense + n_tierb exceeds the filter primes");
     if (t->unit_j == 0 || t->unit_j % kRowBits || t->unit_j > (1u << 20))
-        return fail(CCS_ERR_INVALID, "unit_j must be a multiple of 2048, <= 2^20");
+        return fail(CCS_ERR_INVALID, "unit_j must be a multiple of 8192, <= 2^20");
     for (u32 i = 0; i < t->n_dense + t->n_tierb; ++i)
         if (s->fp[i] >= kTierBMaxQ) return fail(CCS_ERR_INVALID, "dense/stage-B primes must be < 16384");
     if (t->blocks_per_cu < 1 || t->blocks_per_cu > 16) return fail(CCS_ERR_INVALID, "blocks_per_cu");
@@ -933,6 +938,8 @@ extern "C" int ccs_shape_query(const ccs_shape* s, ccs_shape_info* o) {
     o->n_deep = s->nc;
     o->rows_per_group = s->tune.rows_per_group;
     o->unit_j = s->tune.unit_j;
+    o->n_slots = s->nd;
+    o->n_tierb_lds = s->nb_lds;
     return CCS_OK;
 }
 
@@ -1813,6 +1820,7 @@ extern "C" int ccs_groupset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_
 
 namespace ccs {
 u32 shape_phase_words(const ccs_shape* s) { return s->nd + s->nb; }
+const u32* jobset_status_words(const ccs_jobset* js) { return js->x.counters + kStatusWord; }
 const PrimeMod* shape_pmod(const ccs_shape* s) { return s->d_pmod; }
 int groupset_update_resident(ccs_ctx* c, ccs_jobset* js, const ccs_job* jobs, const uint32_t* job_shape,
                              const uint32_t* slots, const uint64_t* offs, uint32_t n_jobs, const u32* d_phases,
@@ -2367,7 +2375,7 @@ static int autotune_impl(ccs_ctx* c, ccs_shape* s, const std::function<int(float
             t.pair_max = pm;
             if ((rc = try_tuning(t))) return rc;
         }
-        for (u32 rg : {1u, 2u, 4u, 8u}) {
+        for (u32 rg : {1u, 2u, 4u}) {
             ccs_tuning t = best;
             if (rg == t.rows_per_group) continue;
             t.rows_per_group = rg;

@@ -17,7 +17,8 @@ typedef uint8_t u8;
 // For the reference lattice n = (tile*WP + k)*M + base (v13.cu:1479-1492) a class is
 // (base, k, rho) with tile = 6u + rho, W = 6*WP*M and a = rho*WP*M + k*M + base.
 
-constexpr int kRowBits = 2048;      // one row = 64 lanes x 32 candidates
+constexpr int kRowBits = 8192;      // one row = 64 lanes x 128 candidates
+constexpr int kRowWords = 4;        // kill words (32 candidates each) per lane and row
 constexpr int kMaxDense = 64;       // dense primes held one per lane
 constexpr int kMaxHashSlots = 64;   // forbidden-residue hash per deep prime
 constexpr int kTierBGroup = 64;     // stage-B primes tested per wave step (one per lane)
@@ -49,8 +50,9 @@ struct PrimeMod {
 };
 
 // Dense stage: kill pattern over u, periodic with period q, stored as an extended bit
-// string in LDS so that a row (64 lanes x 32 bits) starting at any phase s reads the two
-// dwords (s>>5)+lane and (s>>5)+lane+1 with no wrap-around.
+// string in LDS (16-byte aligned) so that a row (64 lanes x 128 bits) starting at any phase
+// s reads the aligned blocks of dwords from ((s>>5) & ~3) + 4*lane, 8 dwords per lane, with
+// no wrap-around.
 // A dense slot is one prime, or two primes folded into one pattern of period q = q1*q2
 // (bit x killed iff x mod q1 or x mod q2 is killed).  The slot's phase is taken modulo q
 // directly (W is prime to q), with the slot's own PrimeMod record: pmod[slot].
@@ -280,6 +282,9 @@ u32 shape_phase_words(const ccs_shape* s);
 const PrimeMod* shape_pmod(const ccs_shape* s);
 // a group-set update whose jobs carry resident phases: job i reads slot slots[i] of the arena,
 // whose phases are at u0_i - offs[i]
+// the sticky overflow status words of a job set's launch scratch (kStatusWord layout); word
+// 0 counts the launches whose stage-B survivors overflowed, and only grows
+const u32* jobset_status_words(const ccs_jobset* js);
 int groupset_update_resident(ccs_ctx* c, ccs_jobset* js, const ccs_job* jobs, const uint32_t* job_shape,
                              const uint32_t* slots, const uint64_t* offs, uint32_t n_jobs, const u32* d_phases,
                              u32 stride, hipStream_t st);

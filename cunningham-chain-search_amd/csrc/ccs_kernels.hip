@@ -211,12 +211,104 @@ __device__ __forceinline__ bool stage_b_pass(const u32* __restrict__ lds, const 
     return !ballot(stage_b_bit<NBV>(lds, R, 0, j)) && stage_b_rest<NBV>(lds, far, nb_lds, R, j);
 }
 
+typedef u32 u32x4 __attribute__((ext_vector_type(4)));   // one 16-byte LDS block
+
+// Marks the four words of a 16-byte LDS block as used together at this point (an empty asm
+// that reads and writes them: no instruction).  Without it the compiler may split a block
+// read whose words are not all used into 32-bit reads, which run at the narrow reads' LDS
+// rate.
+__device__ __forceinline__ void keep_block(u32x4& v) { __asm__ volatile("" : "+v"(v)); }
+
+// The 128 pattern bits of a lane's window, for RG rows: bits [sh mod 32, +128) of the words
+// C, C+1, ..., C+4 of the eight words lo.xyzw, hi.xyzw read at the window's aligned 16-byte
+// block (v_alignbit takes the low five bits of sh); C is the pattern's word offset within the
+// block, the same for every lane and fixed at compile time (the dense loop runs the patterns
+// of each offset in a loop of its own).
+template <int C, int RG>
+__device__ __forceinline__ void shift_window(const u32x4 (&lo)[RG], const u32x4 (&hi)[RG], u32 sh,
+                                             u32 (&x)[RG][kRowWords]) {
+#pragma unroll
+    for (int r = 0; r < RG; ++r) {
+        const u32 w[8] = {lo[r].x, lo[r].y, lo[r].z, lo[r].w, hi[r].x, hi[r].y, hi[r].z, hi[r].w};
+#pragma unroll
+        for (int k = 0; k < kRowWords; ++k) x[r][k] = __builtin_amdgcn_alignbit(w[C + k + 1], w[C + k], sh);
+    }
+}
+
+// One dense pattern's kill bits for RG rows, OR-ed into acc: the lane reads the two aligned
+// 16-byte blocks at its window (ds_read_b128: the 64 lanes read 1 KiB apiece, conflict-free).
+template <int C, int RG>
+__device__ __forceinline__ void dense_read(const u32* lds, u32 lane, u32 ba, u32x4 (&lo)[RG], u32x4 (&hi)[RG]) {
+    const u32x4* src = (const u32x4*)(lds + ((ba >> 5) & ~3u)) + lane;
+#pragma unroll
+    for (int r = 0; r < RG; ++r) {
+        lo[r] = src[64 * r];
+        hi[r] = src[64 * r + 1];
+    }
+}
+
+// The dense stage of one row group over the patterns whose word offset is C (mask: their
+// slot indices), two patterns per step so that their four block reads per row are in flight
+// together.  sd: lane p holds pattern p's bit address in LDS.
+template <int C, int RG>
+__device__ __forceinline__ void dense_class(const u32* lds, u32 lane, u32 sd, u32 mask, u32 first,
+                                            u32 (&acc)[RG][kRowWords]) {
+    while (mask & (mask - 1u)) {   // two patterns or more left
+        const u32 p0 = (u32)__builtin_ctz(mask);
+        mask &= mask - 1u;
+        const u32 p1 = (u32)__builtin_ctz(mask);
+        mask &= mask - 1u;
+        const u32 b0 = __builtin_amdgcn_readlane(sd, first + p0), b1 = __builtin_amdgcn_readlane(sd, first + p1);
+        u32x4 lo0[RG], hi0[RG], lo1[RG], hi1[RG];
+        dense_read<C, RG>(lds, lane, b0, lo0, hi0);
+        dense_read<C, RG>(lds, lane, b1, lo1, hi1);
+#pragma unroll
+        for (int r = 0; r < RG; ++r) {
+            keep_block(lo0[r]);
+            keep_block(hi0[r]);
+            keep_block(lo1[r]);
+            keep_block(hi1[r]);
+        }
+        u32 x0[RG][kRowWords], x1[RG][kRowWords];
+        shift_window<C, RG>(lo0, hi0, b0, x0);
+        shift_window<C, RG>(lo1, hi1, b1, x1);
+#pragma unroll
+        for (int r = 0; r < RG; ++r)
+#pragma unroll
+            for (int k = 0; k < kRowWords; ++k) acc[r][k] |= x0[r][k] | x1[r][k];
+    }
+    if (mask) {   // the last one
+        const u32 b0 = __builtin_amdgcn_readlane(sd, first + (u32)__builtin_ctz(mask));
+        u32x4 lo0[RG], hi0[RG];
+        dense_read<C, RG>(lds, lane, b0, lo0, hi0);
+#pragma unroll
+        for (int r = 0; r < RG; ++r) {
+            keep_block(lo0[r]);
+            keep_block(hi0[r]);
+        }
+        u32 x0[RG][kRowWords];
+        shift_window<C, RG>(lo0, hi0, b0, x0);
+#pragma unroll
+        for (int r = 0; r < RG; ++r)
+#pragma unroll
+            for (int k = 0; k < kRowWords; ++k) acc[r][k] |= x0[r][k];
+    }
+}
+
+// the patterns of word offset C: slots 0..31, then 32..63 (rarely any)
+template <int C, int RG>
+__device__ __forceinline__ void dense_offset(const u32* lds, u32 lane, u32 sd, u64 mask, u32 (&acc)[RG][kRowWords]) {
+    dense_class<C, RG>(lds, lane, sd, (u32)mask, 0u, acc);
+    if (mask >> 32) dense_class<C, RG>(lds, lane, sd, (u32)(mask >> 32), 32u, acc);
+}
+
 // GROUP: the launch sieves the jobs of several shapes (tables loaded per shape);
 // RESIDENT: its jobs may carry resident phases (a group-launch variant of its own, so the
 // launches without them keep their register allocation)
 template <int RG, int NBV, bool GROUP, bool RESIDENT>
 __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
-    extern __shared__ u32 lds[];
+    extern __shared__ uint4 lds_blocks[];   // 16-byte aligned: the dense patterns are read as b128
+    u32* const lds = (u32*)lds_blocks;
     const u32 tid = threadIdx.x;
     u32* const ctl = lds + A.lds_ctl;   // scheduler words past the tables (8-byte aligned)
     const u32 lane = lane_id();
@@ -451,58 +543,44 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
 
             const u32 rows = (U.n + kRowBits - 1) / kRowBits;
             for (u32 row0 = 0; row0 < rows; row0 += RG) {
-                u32 acc[RG];
+                // lane l owns candidates row*8192 + 128*l + 32*k + b (k = 0..3, b = 0..31): per
+                // pattern and row it reads the two aligned 16-byte blocks at its 128-bit window
+                // (ds_read_b128, the 64 lanes over 1 KiB: conflict-free) and shifts the window by
+                // the pattern's word and bit offset, which are the same for the whole wave
+                u32 acc[RG][kRowWords];
     #pragma unroll
-                for (int r = 0; r < RG; ++r) acc[r] = 0;
-                // SPS patterns per step so their LDS reads (2*RG per pattern) are in flight together
-                constexpr int SPS = RG >= 8 ? 2 : 4;
-                u32 p = 0;
-                for (; p + SPS <= t_nd(); p += SPS) {
-                    u32 ba[SPS], lo[SPS][RG], hi[SPS][RG];
+                for (int r = 0; r < RG; ++r)
     #pragma unroll
-                    for (int u = 0; u < SPS; ++u) {
-                        ba[u] = __builtin_amdgcn_readlane(sd, p + (u32)u);
-                        const u32* src = lds + (ba[u] >> 5) + lane;
-                        // A.one == 1: an offset the compiler cannot fold, so the lo words of
-                        // row pairs (and the hi words) load with one stride-64 instruction each
-                        const u32* srch = src + A.one;
-    #pragma unroll
-                        for (int r = 0; r < RG; ++r) {
-                            lo[u][r] = src[64 * r];
-                            hi[u][r] = srch[64 * r];
-                        }
-                    }
-    #pragma unroll
-                    for (int u = 0; u < SPS; ++u)
-    #pragma unroll
-                        for (int r = 0; r < RG; ++r) acc[r] |= __builtin_amdgcn_alignbit(hi[u][r], lo[u][r], ba[u] & 31u);
-                }
-                for (; p < t_nd(); ++p) {
-                    const u32 ba = __builtin_amdgcn_readlane(sd, p);
-                    const u32* src = lds + (ba >> 5) + lane;
-    #pragma unroll
-                    for (int r = 0; r < RG; ++r) {
-                        const u32 lo = src[64 * r];
-                        const u32 hi = src[64 * r + 1];
-                        acc[r] |= __builtin_amdgcn_alignbit(hi, lo, ba & 31u);
-                    }
+                    for (int k = 0; k < kRowWords; ++k) acc[r][k] = 0;
+                // the patterns by word offset within their 16-byte block (lane p: pattern p),
+                // each offset's patterns in a loop of its own with no branch per pattern
+                {
+                    const u32 c = (sd >> 5) & 3u;
+                    const bool mine = lane < t_nd();
+                    dense_offset<0, RG>(lds, lane, sd, ballot(mine && c == 0u), acc);
+                    dense_offset<1, RG>(lds, lane, sd, ballot(mine && c == 1u), acc);
+                    dense_offset<2, RG>(lds, lane, sd, ballot(mine && c == 2u), acc);
+                    dense_offset<3, RG>(lds, lane, sd, ballot(mine && c == 3u), acc);
                 }
                 if (lane < t_nd()) {
                     u32 s = sd - doff * 32u + dstep;
                     if (s >= dq) s -= dq;
                     sd = doff * 32u + s;
                 }
-                u32 alive[RG];
+                u32 alive[RG][kRowWords];
     #pragma unroll
                 for (int r = 0; r < RG; ++r) {
                     const u32 rowbase = (row0 + (u32)r) * (u32)kRowBits;
-                    u32 v = ~acc[r];
-                    if (rowbase + (u32)kRowBits > U.n) {   // the unit's last, partial row (uniform)
-                        const u32 jb = rowbase + lane * 32u;
-                        if (jb >= U.n) v = 0;
-                        else if (U.n - jb < 32u) v &= (1u << (U.n - jb)) - 1u;
+    #pragma unroll
+                    for (int k = 0; k < kRowWords; ++k) {
+                        u32 v = ~acc[r][k];
+                        if (rowbase + (u32)kRowBits > U.n) {   // the unit's last, partial row (uniform)
+                            const u32 jb = rowbase + lane * 128u + 32u * (u32)k;
+                            if (jb >= U.n) v = 0;
+                            else if (U.n - jb < 32u) v &= (1u << (U.n - jb)) - 1u;
+                        }
+                        alive[r][k] = v;
                     }
-                    alive[r] = v;
                 }
                 // survivors of the dense stage in uniform control flow: the lanes with a
                 // non-empty word (ballot), then that word's bits (readlane); stage B takes them
@@ -516,14 +594,15 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                 };
                 u32 pend = 0xffffffffu;
     #pragma unroll
-                for (int r = 0; r < RG; ++r) {
-                    u64 m = ballot(alive[r] != 0u);
+                for (int rk = 0; rk < RG * kRowWords; ++rk) {
+                    const int r = rk / kRowWords, k = rk % kRowWords;
+                    u64 m = ballot(alive[r][k] != 0u);
                     while (m) {
                         const u32 l = (u32)__builtin_ctzll(m);
                         m &= m - 1u;
-                        u32 w = __builtin_amdgcn_readlane(alive[r], l);
+                        u32 w = __builtin_amdgcn_readlane(alive[r][k], l);
                         cnt_dense += (u32)__builtin_popcount(w);
-                        const u32 jb = (row0 + (u32)r) * (u32)kRowBits + l * 32u;
+                        const u32 jb = (row0 + (u32)r) * (u32)kRowBits + l * 128u + 32u * (u32)k;
                         while (w) {
                             const u32 j = jb + (u32)__builtin_ctz(w);
                             w &= w - 1u;
@@ -1148,7 +1227,7 @@ hipError_t launch_sieve(const SieveArgs& a, int rg, int nbv, int grid, int block
     CCS_SIEVE_CASE(R, 12) CCS_SIEVE_CASE(R, 16)
     // more than 8 stage-B groups: the next instance up (lanes past the last prime are padding)
     if (nbv > 8) nbv = nbv <= 12 ? 12 : 16;
-    CCS_SIEVE_ROW(1) CCS_SIEVE_ROW(2) CCS_SIEVE_ROW(4) CCS_SIEVE_ROW(8)
+    CCS_SIEVE_ROW(1) CCS_SIEVE_ROW(2) CCS_SIEVE_ROW(4)
 #undef CCS_SIEVE_ROW
 #undef CCS_SIEVE_CASE
     return hipErrorInvalidValue;

@@ -71,6 +71,8 @@ typedef struct {
     uint64_t W_hi;                  /* high 64 bits of W (family shapes)                */
     int32_t family, exp_start;      /* family shape: n = Q*D*2^exp_start - 1            */
     ccs_u128 D;                     /* family shape: immune-factor product D            */
+    uint32_t n_slots;               /* dense patterns (a pattern folds up to 3 primes)  */
+    uint32_t n_tierb_lds;           /* stage-B primes whose bitsets sit in LDS          */
 } ccs_shape_info;
 
 /* ---- fingerprint families (the v16 Q-iteration front end, v16...cu:1-30, :556-924,
@@ -158,8 +160,8 @@ typedef struct {
 typedef struct {
     uint32_t n_dense;          /* filter primes handled by the dense pattern stage (<= 128, at most 64 patterns) */
     uint32_t n_tierb;          /* next primes handled by stage B (<= 1024, all < 16384) */
-    uint32_t rows_per_group;   /* 1, 2, 4 or 8 rows of 2048 candidates per lane step */
-    uint32_t unit_j;           /* candidates per work unit, multiple of 2048, <= 2^20 */
+    uint32_t rows_per_group;   /* 1, 2 or 4 rows of 8192 candidates (64 lanes x 128) per step */
+    uint32_t unit_j;           /* candidates per work unit, multiple of 8192, <= 2^20 */
     uint32_t blocks_per_cu;    /* persistent grid = CUs * blocks_per_cu             */
     uint32_t pair_max;         /* fold up to three adjacent dense primes into one  */
                                /*   pattern while their product <= pair_max (at    */

@@ -130,9 +130,9 @@ def test_u128_carry_and_high_bits(ctx, oracle, ref, oref):
 @pytest.mark.parametrize("tuning", [
     dict(rows_per_group=1), dict(rows_per_group=4), dict(n_dense=0, n_tierb=100),
     dict(n_dense=7, n_tierb=7), dict(n_dense=43, n_tierb=96), dict(n_dense=14, n_tierb=0),
-    dict(n_dense=21, n_tierb=64), dict(n_dense=21, n_tierb=65), dict(unit_j=2048), dict(unit_j=6144),
+    dict(n_dense=21, n_tierb=64), dict(n_dense=21, n_tierb=65), dict(unit_j=8192), dict(unit_j=24576),
     dict(blocks_per_cu=1), dict(pair_max=4096), dict(pair_max=16384, n_dense=43, n_tierb=96),
-    dict(pair_max=16384, n_dense=20, rows_per_group=4), dict(pair_max=16384, n_dense=64, n_tierb=75),
+    dict(pair_max=16384, n_dense=20, rows_per_group=4), dict(pair_max=16384, n_dense=64, n_tierb=75, rows_per_group=1),
 ])
 def test_tuning_does_not_change_results(ctx, oracle, oref, tuning):
     s = WL.make_shape(ctx, "reference")
@@ -163,7 +163,7 @@ def test_deep_stage_with_survivors(ctx, oracle):
     assert len(g) > 100
 
 
-@pytest.mark.parametrize("tuning", [dict(n_tierb=1024), dict(n_tierb=700, rows_per_group=4),
+@pytest.mark.parametrize("tuning", [dict(n_tierb=1024), dict(n_tierb=700, rows_per_group=1),
                                     dict(n_dense=20, n_tierb=1000)])
 def test_wide_stage_b_vs_oracle(ctx, oracle, tuning):
     """Stage B over up to 1024 primes (16 groups of 64, primes up to ~8700, most bitsets in
