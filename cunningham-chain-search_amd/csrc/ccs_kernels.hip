@@ -150,40 +150,102 @@ __device__ __forceinline__ bool resolve(const SieveArgs& A, u64 unit, u32 lane, 
 // K1: sieve kernel
 // ---------------------------------------------------------------------------
 
-// per-lane copies of the stage-B prime constants: lane l holds prime 64*i + l in slot i
-// (group 0, probed for every dense survivor, keeps q and the offset apart; the rarely
-// probed later groups pack them into one register: offset | q << 18, q < 2^14)
+// x mod q for x < 2^21, q < 2^14 from the f32 reciprocal of the hardware (about 1 ulp): the
+// truncated quotient is within one of floor(x / q) (the product's error is below 2^-11),
+// and one unsigned min each way puts the remainder back in [0, q)
+__device__ __forceinline__ u32 mod_small(u32 x, u32 q) {
+    const int qt = (int)((float)x * __builtin_amdgcn_rcpf((float)q));
+    u32 r = x - (u32)__mul24(qt, (int)q);
+    r = min(r, r + q);   // quotient one too large: r wrapped below zero
+    r = min(r, r - q);   // quotient one too small: r >= q
+    return r;
+}
+
+// x mod q for x < 2^21 with the host's reciprocal of q, rounded up so that the truncated
+// f32 product is exactly floor(x / q)
+__device__ __forceinline__ u32 mod_exact(u32 x, int nq, u32 rcp_bits) {
+    const int qt = (int)((float)x * __uint_as_float(rcp_bits));
+    return (u32)(__mul24(qt, nq) + (int)x);
+}
+
+// Per-lane copies of the stage-B prime constants: lane l holds prime 64*i + l in group i.
+// Group 0, probed for every dense survivor, keeps its phase, -q, bitset offset and
+// reciprocal in four registers; the rarely probed later groups keep offset | q << 18
+// (q < 2^14) in one register and, up to 8 groups, the phase and the reciprocal in two more.
+// With 12 or 16 groups that made the instances spill, so there a later group's phase is a
+// 16-bit half of a register shared with the next group, the reciprocal is the hardware's
+// approximate one (mod_small), and the packed values reach each probe through an opaque
+// copy (else the compiler hoists every group's unpacked values out of the row loops and
+// the packing is undone).
 template <int NBV>
 struct TierBRegs {
-    u32 s[NBV];      // phase of the current unit
-    int nq0;         // group 0: -q
-    u32 off0;        // group 0: dword offset of the bitset (LDS)
-    u32 qo[NBV];     // groups >= 1: offset | q << 18 (LDS or device memory); [0] unused
-    u32 rcp[NBV];    // f32 bits of the rounded-up 1/q
-    __device__ __forceinline__ int nq(int i) const { return i == 0 ? nq0 : -(int)(qo[i] >> kTierBOffBits); }
+    static constexpr bool kPack = NBV > 8;
+    static constexpr int kLater = NBV > 1 ? NBV - 1 : 1;
+    static constexpr int kPhaseRegs = kPack ? NBV / 2 : kLater;
+    u32 s0;                        // group 0: phase of the current unit
+    int nq0;                       // group 0: -q
+    u32 off0;                      // group 0: dword offset of the bitset (LDS)
+    u32 rcp0;                      // group 0: f32 bits of the rounded-up 1/q
+    u32 qo[NBV];                   // groups >= 1: offset | q << 18; [0] unused
+    u32 sp[kPhaseRegs];            // groups >= 1: phases (packed: group i in half (i-1)&1 of sp[(i-1)/2])
+    u32 rc[kPack ? 1 : kLater];    // groups >= 1, unpacked form: f32 bits of the rounded-up 1/q
+
+    __device__ __forceinline__ u32 qo_at(int i) const {
+        u32 v = qo[i];
+        if constexpr (kPack) asm volatile("" : "+v"(v));
+        return v;
+    }
+    __device__ __forceinline__ u32 s(int i) const {
+        if (i == 0) return s0;
+        if constexpr (kPack) {
+            u32 w = sp[(i - 1) >> 1];
+            asm volatile("" : "+v"(w));
+            return __builtin_amdgcn_ubfe(w, 16u * (u32)((i - 1) & 1), 16u);
+        } else {
+            return sp[i - 1];
+        }
+    }
+    __device__ __forceinline__ void set_s(int i, u32 v) {
+        if (i == 0) {
+            s0 = v;
+        } else if constexpr (kPack) {
+            u32& w = sp[(i - 1) >> 1];
+            w = ((i - 1) & 1) ? ((w & 0xffffu) | (v << 16)) : ((w & 0xffff0000u) | v);
+        } else {
+            sp[i - 1] = v;
+        }
+    }
+    __device__ __forceinline__ void set_rcp(int i, float r) {
+        if (i == 0) rcp0 = __float_as_uint(r);
+        else if constexpr (!kPack) rc[i - 1] = __float_as_uint(r);
+    }
+    // x mod q_i for x < 2^21
+    __device__ __forceinline__ u32 residue(int i, u32 x) const {
+        if (i == 0) return mod_exact(x, nq0, rcp0);
+        const u32 q = qo_at(i) >> kTierBOffBits;
+        if constexpr (kPack) return mod_small(x, q);
+        else return mod_exact(x, -(int)q, rc[i - 1]);
+    }
     __device__ __forceinline__ u32 off(int i) const {
-        return i == 0 ? off0 : (qo[i] & ((1u << kTierBOffBits) - 1u));
+        return i == 0 ? off0 : (qo_at(i) & ((1u << kTierBOffBits) - 1u));
     }
 };
 
 // Stage B for one candidate j (uniform across the wave): the stage-B primes are held one
-// per lane (prime 64*i + lane in slot i), so one probe of every lane tests 64 primes at
-// once.  x = j + s < 2^21 and the host's reciprocal is rounded up so that the truncated
-// f32 product is exactly floor(x / q); the probe reads bit r of the prime's
-// forbidden-residue bitset.  Most candidates die in the first group of 64 primes, so the
-// later groups are rarely read; their bitsets sit in LDS as far as it holds them, the rest
-// in device memory (L2-resident).
+// per lane (prime 64*i + lane in group i), so one probe of every lane tests 64 primes at
+// once; x = j + s < 2^21, and the probe reads bit (x mod q) of the prime's forbidden-residue
+// bitset.  Most candidates die in the first group of 64 primes, so the later groups are
+// rarely read; their bitsets sit in LDS as far as it holds them, the rest in device memory
+// (L2-resident).
 // the probe of stage-B group i for candidate j: this lane's prime kills j
 template <int NBV>
 __device__ __forceinline__ bool stage_b_bit(const u32* __restrict__ mem, const TierBRegs<NBV>& R, int i, u32 j) {
-    const u32 x = j + R.s[i];
-    const int qt = (int)((float)x * __uint_as_float(R.rcp[i]));
-    const u32 r = (u32)(__mul24(qt, R.nq(i)) + (int)x);                       // x mod q
+    const u32 r = R.residue(i, j + R.s(i));
     const u32 w = mem[R.off(i) + __builtin_amdgcn_ubfe(r, 5u, 11u)];
     return __builtin_amdgcn_ubfe(w, r, 1u) != 0u;
 }
 
-// groups from..NBV-1 for one candidate (groups past the first live in device memory)
+// groups 1..NBV-1 for one candidate (groups past nb_lds live in device memory)
 template <int NBV>
 __device__ __forceinline__ bool stage_b_rest(const u32* __restrict__ lds, const u32* __restrict__ far,
                                              u32 nb_lds, const TierBRegs<NBV>& R, u32 j) {
@@ -352,13 +414,13 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
             tp.rcp = 1.0f;
             tp.pad = 0;
             if (p < T.nb) tp = T.tierb[p];
-            R.s[i] = 0;
+            R.set_s(i, 0);
+            R.set_rcp(i, tp.rcp);
             if (i == 0) {
                 R.nq0 = -(int)tp.q;
                 R.off0 = tp.lds_off;
             }
             R.qo[i] = tp.lds_off | (tp.q << kTierBOffBits);
-            R.rcp[i] = __float_as_uint(tp.rcp);
         }
     };
     u64 cnt_dense = 0;   // candidates alive after the dense stage (= tested in stage B)
@@ -382,9 +444,12 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
     };
 
     // Work distribution with few global atomics, in two phases.
-    //  A: the first n_a units are split into equal contiguous ranges, one per block; the
-    //     block's waves take runs of consecutive units from it through an LDS cursor
-    //     (a run is 1/(8 * waves per block) of the range's rest, 1..64 units).
+    //  A: the first n_a units are split into equal contiguous ranges, one per block, and in
+    //     a launch of several shapes each shape's share of them is split so (a unit's cost
+    //     differs between shapes several times over, so equal unit counts per block would
+    //     leave the blocks with the cheap shapes idle); the block's waves take runs of
+    //     consecutive units from a range through an LDS cursor (a run is
+    //     1/(8 * waves per block) of the range's rest, 1..64 units).
     //  B: the last n_pool units (a sixteenth, at least one per wave of the grid) form a
     //     pool shared by all blocks, drawn in block-wide rounds: thread 0 claims one unit
     //     per wave for the whole block with one global atomic (the next round's claim is in
@@ -401,7 +466,6 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
     if (n_pool < n_blocks * wpb) n_pool = n_blocks * wpb;
     if (n_pool > n_units) n_pool = n_units;
     const u32 n_a = n_units - n_pool;
-    const u32 blk_lo = (u32)((u64)n_a * blockIdx.x / n_blocks), blk_hi = (u32)((u64)n_a * (blockIdx.x + 1) / n_blocks);
     // shape of a unit, and the end of that shape's units (one shape outside group launches)
     auto shape_of = [&](u32 u) -> u32 {
         if constexpr (!GROUP) return 0;
@@ -422,18 +486,29 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
     u32 next_g = 0;   // thread 0: the claim for the next round (one unit per wave)
     if (tid == 0) next_g = (u32)atomicAdd(A.unit_counter, (u64)wpb);
     u32 cur_shape = 0xffffffffu;
-    u32 a_next = blk_lo;                       // phase A: start of the next step
+    u32 a_shape = 0;                           // phase A: the next shape's range
+    const u32 n_shapes = GROUP ? A.n_group : 1u;
     bool in_pool = false;
     u32 r_base = 0, r_next = 0, r_end = 0;     // phase B: the round, the next step's start
     u32 sd = 0;                                             // dense phase (lane < nd)
     u32 prev_owner = 0xffffffffu, prev_j0 = 0, prev_n = 0;  // the wave's previous unit
     for (;;) {
         u32 st_lo, st_hi, sh;
-        if (!in_pool && a_next < blk_hi) {
-            st_lo = a_next;
-            sh = shape_of(st_lo);
-            st_hi = shape_end(sh) < blk_hi ? shape_end(sh) : blk_hi;
-            a_next = st_hi;
+        if (!in_pool && a_shape < n_shapes) {
+            // this block's share of the shape's units below n_a (block-uniform values: a
+            // block skips an empty share as a whole)
+            sh = a_shape++;
+            u32 lo = 0, hi = n_units;
+            if constexpr (GROUP) {
+                lo = (u32)A.shape_units[sh];
+                hi = (u32)A.shape_units[sh + 1];
+            }
+            if (hi > n_a) hi = n_a;
+            if (hi <= lo) continue;
+            const u32 len = hi - lo;
+            st_lo = lo + (u32)((u64)len * blockIdx.x / n_blocks);
+            st_hi = lo + (u32)((u64)len * (blockIdx.x + 1) / n_blocks);
+            if (st_hi == st_lo) continue;
         } else {
             in_pool = true;
             if (r_next >= r_end) {
@@ -507,34 +582,40 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
             U.ph_off = bcast64(U.ph_off);
 
             // phases: dense prime p held by lane p as an LDS bit address, stage-B prime
-            // (64*i + lane) held in R.s[i].  A unit that continues the previous one (same
+            // (64*i + lane) held in group i of R.  A unit that continues the previous one (same
             // owner, next u, previous unit a whole number of row groups) keeps the dense phases
             // where the row loop left them and advances the stage-B phases by the previous
             // unit's length; otherwise the phases are computed from u_start.
             if (U.owner == prev_owner && U.j0 == prev_j0 + prev_n && prev_n % (u32)(RG * kRowBits) == 0u) {
     #pragma unroll
-                for (int i = 0; i < NBV; ++i) {   // x < 4096 + 2^20: the probe's exact quotient
-                    const u32 x = R.s[i] + prev_n;
-                    const int qt = (int)((float)x * __uint_as_float(R.rcp[i]));
-                    R.s[i] = (u32)(__mul24(qt, R.nq(i)) + (int)x);
-                }
+                for (int i = 0; i < NBV; ++i) R.set_s(i, R.residue(i, R.s(i) + prev_n));   // < 2^14 + 2^20
             } else if (RESIDENT && U.ph_slot != kNoSlot) {
                 // a served job's phases are resident: advanced from its slot by the unit's offset
                 const u32* ph = A.phases + (size_t)U.ph_slot * A.ph_stride;
                 sd = 0;
                 if (lane < t_nd()) sd = doff * 32u + advance_phase(ph[lane], U.ph_off, t_pmod()[lane]);
+                // one group at a time through opaque running pointers (16 precomputed 64-bit
+                // addresses, or 16 groups' loads in flight, would spill)
+                const u32* php = ph + t_nd() + lane;
+                const PrimeMod* pmp = t_pmod() + t_nd() + lane;
     #pragma unroll
                 for (int i = 0; i < NBV; ++i) {
                     const u32 p = (u32)i * 64u + lane;
-                    R.s[i] = p < t_nb() ? advance_phase(ph[t_nd() + p], U.ph_off, t_pmod()[t_nd() + p]) : 0u;
+                    R.set_s(i, p < t_nb() ? advance_phase(*php, U.ph_off, *pmp) : 0u);
+                    php += 64;
+                    pmp += 64;
+                    asm volatile("" : "+v"(php), "+v"(pmp) : : "memory");
                 }
             } else {
                 sd = 0;
                 if (lane < t_nd()) sd = doff * 32u + phase(t_pmod()[lane], U.a, U.a_hi, U.us_lo, U.us_hi);
+                const PrimeMod* pmp = t_pmod() + t_nd() + lane;
     #pragma unroll
                 for (int i = 0; i < NBV; ++i) {
                     const u32 p = (u32)i * 64u + lane;
-                    R.s[i] = p < t_nb() ? phase(t_pmod()[t_nd() + p], U.a, U.a_hi, U.us_lo, U.us_hi) : 0u;
+                    R.set_s(i, p < t_nb() ? phase(*pmp, U.a, U.a_hi, U.us_lo, U.us_hi) : 0u);
+                    pmp += 64;
+                    asm volatile("" : "+v"(pmp) : : "memory");
                 }
             }
             prev_owner = U.owner;

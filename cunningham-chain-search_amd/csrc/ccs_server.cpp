@@ -67,7 +67,8 @@ struct Lane {           // buffers of one in-flight slice for one launch group
     uint32_t* d_count = nullptr;
     ccs_job_survivor* h_surv = nullptr;   // pinned
     ccs_chain_result* h_res = nullptr;    // pinned
-    uint32_t* h_count = nullptr;          // pinned
+    uint32_t* h_count = nullptr;          // pinned: [0] survivors, [1] overflowed launches so far
+    uint32_t seen_overflows = 0;          // h_count[1] at the last collection
     std::vector<ccs_job> jobs;
     std::vector<uint32_t> job_shape;      // the depth profile of each job
     std::vector<uint32_t> slots;          // the arena slot of each job
@@ -131,16 +132,15 @@ static int collect_lane(ccs_server* s, int li) {
     if (!any) return CCS_OK;
     if (hipStreamSynchronize(s->streams[li]) != hipSuccess || hipStreamSynchronize(s->pstreams[li]) != hipSuccess)
         return CCS_ERR_HIP;
-    // survivors lost to a full stage-B buffer or a full slice buffer: the slice's jobs are
-    // finished incomplete (and flagged so)
+    // survivors lost to a full stage-B buffer (the set's overflow counter, copied back with
+    // the survivor count, has grown) or a full slice buffer: the slice's jobs are finished
+    // incomplete (and flagged so)
     std::vector<char> lost(s->lanes[li].size(), 0);
     for (size_t gi = 0; gi < s->lanes[li].size(); ++gi) {
         Lane& L = s->lanes[li][gi];
         if (!L.busy) continue;
-        ccs_launch_status ls;
-        const int rc = ccs_jobset_status(s->ctx, L.js, s->streams[li], &ls);
-        if (rc < 0) return rc;
-        lost[gi] = rc == CCS_OVERFLOW || *L.h_count > s->p.max_slice_survivors;
+        lost[gi] = L.h_count[1] != L.seen_overflows || L.h_count[0] > s->p.max_slice_survivors;
+        L.seen_overflows = L.h_count[1];
     }
     const double t = now_s();
     std::lock_guard<std::mutex> g(s->mu);
@@ -333,7 +333,9 @@ static void server_loop(ccs_server* s) {
                 rc = ccs_prove_jobset_launch(s->ctx, L.js, L.d_surv, L.d_count, s->p.max_slice_survivors, &pp,
                                              L.d_res, s->pstreams[li]);
             }
-            if (!rc && hipMemcpyAsync(L.h_count, L.d_count, 4, hipMemcpyDeviceToHost, s->pstreams[li]) != hipSuccess)
+            if (!rc && (hipMemcpyAsync(L.h_count, L.d_count, 4, hipMemcpyDeviceToHost, s->pstreams[li]) != hipSuccess ||
+                        hipMemcpyAsync(L.h_count + 1, ccs::jobset_status_words(L.js), 4, hipMemcpyDeviceToHost,
+                                       s->pstreams[li]) != hipSuccess))
                 rc = CCS_ERR_HIP;
             // survivors are rare: copy a bounded prefix now, the rest (if any) at collection
             const size_t pre = std::min<uint32_t>(kEagerCopy, s->p.max_slice_survivors);
