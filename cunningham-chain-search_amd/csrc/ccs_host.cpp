@@ -430,6 +430,7 @@ struct ccs_shape {
     double est_survival_b = 1.0;             // expected fraction alive after stage B
     u32 deep_masked = 0;                     // the deep stage reads the forbidden masks
     double est_survival_d = 1.0;             // expected fraction alive after the dense stage
+    double est_survival_f = 1.0;             // expected fraction alive after every filter prime
 };
 
 static bool is_prime_u32(u32 q) {
@@ -711,6 +712,8 @@ static int upload_tables(ccs_shape* s) {
     }
     if (nd == 0) s->est_survival_d = 1.0;
     s->est_survival_b = surv;
+    for (size_t i = nd + nb; i < s->fp.size(); ++i) surv *= 1.0 - (double)s->forb[i].size() / s->fp[i];
+    s->est_survival_f = surv;
     s->deep_masked = surv > kDeepMaskSurvival ? 1u : 0u;
 
     free_device_tables(s);
@@ -1820,6 +1823,7 @@ extern "C" int ccs_groupset_update(ccs_ctx* c, ccs_jobset* js, const ccs_job* h_
 
 namespace ccs {
 u32 shape_phase_words(const ccs_shape* s) { return s->nd + s->nb; }
+double shape_final_survival(const ccs_shape* s) { return s->est_survival_f; }
 const u32* jobset_status_words(const ccs_jobset* js) { return js->x.counters + kStatusWord; }
 const PrimeMod* shape_pmod(const ccs_shape* s) { return s->d_pmod; }
 int groupset_update_resident(ccs_ctx* c, ccs_jobset* js, const ccs_job* jobs, const uint32_t* job_shape,

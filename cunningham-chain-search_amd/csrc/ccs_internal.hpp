@@ -278,6 +278,8 @@ hipError_t launch_phase_init(const PhaseInit* d_jobs, u32 n_jobs, u32* phases, u
 // ---- served form hooks (ccs_host.cpp, for ccs_server.cpp) ----
 // words of resident phase state per job of a shape (dense slots + stage-B primes)
 u32 shape_phase_words(const ccs_shape* s);
+// the expected fraction of a shape's candidates that survive every filter prime
+double shape_final_survival(const ccs_shape* s);
 // the residue constants of a shape (device, dense slots then stage B)
 const PrimeMod* shape_pmod(const ccs_shape* s);
 // a group-set update whose jobs carry resident phases: job i reads slot slots[i] of the arena,

@@ -7,9 +7,12 @@
 // admitted job, runs one persistent group launch over all depth profiles (the jobs' sieve
 // phases advanced from their resident state) + one prove launch, and hands results back per
 // job.  Requests that arrive while slices are on the GPU join the next slice.  Two buffer
-// lanes alternate so the GPU runs slice k+1 while the host distributes slice k; within a
-// lane the proves run on a stream of their own, behind an event per group, so they overlap
-// the next groups' filter launches.
+// lanes alternate so the GPU runs slice k+1 while the host distributes slice k.  The filter
+// launches of every slice go to one stream and the proves (with the result copies) to a
+// second one, behind an event per launch group, so a slice's prove overlaps the next
+// slice's filter: with a filter and a prove stream per lane, the runtime put lane 1's filter
+// stream and lane 0's prove stream on one hardware queue (streams share the few hardware
+// queues round-robin), which ran every filter behind the previous slice's prove.
 //
 // The reference has no serving mode; its closest pattern is the v16 dispatch ring with
 // an asynchronous prove pool (v16...cu:2938-3279, :5594-5965), whose drained per-variant
@@ -34,7 +37,7 @@ namespace {
 
 using ccs::kNoSlot;
 
-constexpr uint32_t kEagerCopy = 1024;        // survivors copied back with every slice
+constexpr uint32_t kEagerMin = 4096;         // survivors copied back with every slice, at least
 constexpr uint64_t kMaxRequest = 1ull << 48;  // resident phases advance exactly below this offset
 
 double now_s() {
@@ -75,6 +78,8 @@ struct Lane {           // buffers of one in-flight slice for one launch group
     std::vector<uint64_t> offs;           // each chunk's u offset from the resident phases
     std::vector<Entry> entries;
     hipEvent_t filtered = nullptr;        // the group's filter launch is done (prove may start)
+    hipEvent_t done = nullptr;            // its prove and result copies are done (buffers free)
+    uint32_t eager = 0;                   // survivor entries copied back with the slice
     bool busy = false;
 };
 
@@ -85,8 +90,9 @@ struct ccs_server {
     ccs_server_params p{};
     std::vector<const ccs_shape*> shapes;   // index = depth group
     std::vector<int32_t> depths;
-    hipStream_t streams[2] = {nullptr, nullptr};    // per lane: phase inits and filter launches
-    hipStream_t pstreams[2] = {nullptr, nullptr};   // per lane: prove launches and copies
+    hipStream_t fstream = nullptr;          // phase inits and filter launches
+    hipStream_t pstream = nullptr;          // prove launches and their result copies
+    hipStream_t cstream = nullptr;          // copies of a large slice's remaining results
     std::vector<Lane> lanes[2];             // [lane][launch group]; one group when grouped
     bool grouped = false;                   // several depth profiles: one launch per slice
     bool resident = true;                   // phases from the arena (else from u0 per unit)
@@ -98,7 +104,6 @@ struct ccs_server {
     std::vector<ccs::PhaseInit> inits;      // admitted since the last slice launch
     ccs::PhaseInit* h_init[2] = {nullptr, nullptr};   // pinned staging per lane
     ccs::PhaseInit* d_init[2] = {nullptr, nullptr};
-    hipEvent_t ev_init[2] = {nullptr, nullptr};        // the lane's last phase-init launch
     std::mutex mu;
     std::condition_variable cv;
     std::deque<uint64_t> pending;
@@ -130,8 +135,8 @@ static int collect_lane(ccs_server* s, int li) {
     bool any = false;
     for (auto& L : s->lanes[li]) any |= L.busy;
     if (!any) return CCS_OK;
-    if (hipStreamSynchronize(s->streams[li]) != hipSuccess || hipStreamSynchronize(s->pstreams[li]) != hipSuccess)
-        return CCS_ERR_HIP;
+    for (auto& L : s->lanes[li])
+        if (L.busy && hipEventSynchronize(L.done) != hipSuccess) return CCS_ERR_HIP;
     // survivors lost to a full stage-B buffer (the set's overflow counter, copied back with
     // the survivor count, has grown) or a full slice buffer: the slice's jobs are finished
     // incomplete (and flagged so)
@@ -149,11 +154,15 @@ static int collect_lane(ccs_server* s, int li) {
         if (!L.busy) continue;
         L.busy = false;
         const uint32_t n = std::min(*L.h_count, s->p.max_slice_survivors);
-        if (n > kEagerCopy) {   // the rest of a large slice's survivors
-            if (hipMemcpy(L.h_surv + kEagerCopy, L.d_surv + kEagerCopy, (size_t)(n - kEagerCopy) * sizeof(ccs_job_survivor),
-                          hipMemcpyDeviceToHost) != hipSuccess ||
-                hipMemcpy(L.h_res + kEagerCopy, L.d_res + kEagerCopy, (size_t)(n - kEagerCopy) * sizeof(ccs_chain_result),
-                          hipMemcpyDeviceToHost) != hipSuccess)
+        if (n > L.eager) {   // the rest of an unexpectedly large slice's survivors (a wait that
+                             // may last until the next slice's filter launch has finished)
+            if (hipMemcpyAsync(L.h_surv + L.eager, L.d_surv + L.eager,
+                               (size_t)(n - L.eager) * sizeof(ccs_job_survivor), hipMemcpyDeviceToHost,
+                               s->cstream) != hipSuccess ||
+                hipMemcpyAsync(L.h_res + L.eager, L.d_res + L.eager,
+                               (size_t)(n - L.eager) * sizeof(ccs_chain_result), hipMemcpyDeviceToHost,
+                               s->cstream) != hipSuccess ||
+                hipStreamSynchronize(s->cstream) != hipSuccess)
                 return CCS_ERR_HIP;
         }
         for (uint32_t i = 0; i < n; ++i) {
@@ -198,16 +207,15 @@ static int collect_lane(ccs_server* s, int li) {
     return CCS_OK;
 }
 
-// the phase-init kernel for the jobs admitted since the last slice, on lane li's filter stream
-// (the other lane's next slice waits for it through ev_init[li])
+// the phase-init kernel for the jobs admitted since the last slice, on the filter stream
+// (ahead of every filter launch that reads their phases)
 static int launch_inits(ccs_server* s, int li, std::vector<ccs::PhaseInit>& inits) {
     if (inits.empty()) return CCS_OK;
     const uint32_t n = (uint32_t)inits.size();   // at most n_slots: one per free slot
     std::memcpy(s->h_init[li], inits.data(), (size_t)n * sizeof(ccs::PhaseInit));
     if (hipMemcpyAsync(s->d_init[li], s->h_init[li], (size_t)n * sizeof(ccs::PhaseInit), hipMemcpyHostToDevice,
-                       s->streams[li]) != hipSuccess ||
-        ccs::launch_phase_init(s->d_init[li], n, s->d_phases, s->stride, s->streams[li]) != hipSuccess ||
-        hipEventRecord(s->ev_init[li], s->streams[li]) != hipSuccess)
+                       s->fstream) != hipSuccess ||
+        ccs::launch_phase_init(s->d_init[li], n, s->d_phases, s->stride, s->fstream) != hipSuccess)
         return ccs::set_error(CCS_ERR_HIP, "phase init launch");
     inits.clear();
     return CCS_OK;
@@ -307,10 +315,8 @@ static void server_loop(ccs_server* s) {
                 L.entries.push_back(plan[gi][i]);
             }
         }
+        // the lane's staging buffer is free: its previous slice was collected
         int rc = launch_inits(s, li, inits);
-        // slices of the other lane may hold jobs whose phases the other lane's stream wrote
-        if (!rc && s->resident && hipStreamWaitEvent(s->streams[li], s->ev_init[li ^ 1], 0) != hipSuccess)
-            rc = ccs::set_error(CCS_ERR_HIP, "stream wait");
         if (rc) { server_fail(s, rc, "phase init"); return; }
         bool launched = false;
         for (size_t gi = 0; gi < n_groups; ++gi) {
@@ -319,32 +325,40 @@ static void server_loop(ccs_server* s) {
             const uint32_t nj = (uint32_t)L.jobs.size();
             if (s->resident)
                 rc = ccs::groupset_update_resident(s->ctx, L.js, L.jobs.data(), L.job_shape.data(), L.slots.data(),
-                                                   L.offs.data(), nj, s->d_phases, s->stride, s->streams[li]);
+                                                   L.offs.data(), nj, s->d_phases, s->stride, s->fstream);
             else
-                rc = ccs_groupset_update(s->ctx, L.js, L.jobs.data(), L.job_shape.data(), nj, s->streams[li]);
-            if (!rc) rc = ccs_jobset_launch(s->ctx, L.js, L.d_surv, s->p.max_slice_survivors, L.d_count, s->streams[li]);
-            // the prove of this group runs on the lane's prove stream, so the next group's
-            // filter launch does not wait behind the prove's latency-bound tail
-            if (!rc && (hipEventRecord(L.filtered, s->streams[li]) != hipSuccess ||
-                        hipStreamWaitEvent(s->pstreams[li], L.filtered, 0) != hipSuccess))
+                rc = ccs_groupset_update(s->ctx, L.js, L.jobs.data(), L.job_shape.data(), nj, s->fstream);
+            if (!rc) rc = ccs_jobset_launch(s->ctx, L.js, L.d_surv, s->p.max_slice_survivors, L.d_count, s->fstream);
+            // the prove runs on the prove stream, so the next filter launch does not wait
+            // behind the prove's latency-bound tail
+            if (!rc && (hipEventRecord(L.filtered, s->fstream) != hipSuccess ||
+                        hipStreamWaitEvent(s->pstream, L.filtered, 0) != hipSuccess))
                 rc = CCS_ERR_HIP;
             if (!rc) {
                 ccs_prove_params pp = s->p.prove;
                 rc = ccs_prove_jobset_launch(s->ctx, L.js, L.d_surv, L.d_count, s->p.max_slice_survivors, &pp,
-                                             L.d_res, s->pstreams[li]);
+                                             L.d_res, s->pstream);
             }
-            if (!rc && (hipMemcpyAsync(L.h_count, L.d_count, 4, hipMemcpyDeviceToHost, s->pstreams[li]) != hipSuccess ||
+            if (!rc && (hipMemcpyAsync(L.h_count, L.d_count, 4, hipMemcpyDeviceToHost, s->pstream) != hipSuccess ||
                         hipMemcpyAsync(L.h_count + 1, ccs::jobset_status_words(L.js), 4, hipMemcpyDeviceToHost,
-                                       s->pstreams[li]) != hipSuccess))
+                                       s->pstream) != hipSuccess))
                 rc = CCS_ERR_HIP;
-            // survivors are rare: copy a bounded prefix now, the rest (if any) at collection
-            const size_t pre = std::min<uint32_t>(kEagerCopy, s->p.max_slice_survivors);
+            // copy back twice the slice's expected survivors (at least kEagerMin) behind the
+            // prove: the rest, if any, is copied at collection, when a copy can wait behind
+            // the next slice's filter launch, which holds every CU
+            double expect = 0;
+            for (size_t i = 0; i < L.jobs.size(); ++i)
+                expect += (double)L.jobs[i].count *
+                          ccs::shape_final_survival(s->shapes[s->grouped ? L.job_shape[i] : gi]);
+            L.eager = (uint32_t)std::min<double>(s->p.max_slice_survivors, 2.0 * expect + kEagerMin);
+            const size_t pre = L.eager;
             if (!rc && hipMemcpyAsync(L.h_surv, L.d_surv, pre * sizeof(ccs_job_survivor),
-                                      hipMemcpyDeviceToHost, s->pstreams[li]) != hipSuccess)
+                                      hipMemcpyDeviceToHost, s->pstream) != hipSuccess)
                 rc = CCS_ERR_HIP;
             if (!rc && hipMemcpyAsync(L.h_res, L.d_res, pre * sizeof(ccs_chain_result),
-                                      hipMemcpyDeviceToHost, s->pstreams[li]) != hipSuccess)
+                                      hipMemcpyDeviceToHost, s->pstream) != hipSuccess)
                 rc = CCS_ERR_HIP;
+            if (!rc && hipEventRecord(L.done, s->pstream) != hipSuccess) rc = CCS_ERR_HIP;
             if (rc) { server_fail(s, rc, "slice launch"); return; }
             L.busy = true;
             launched = true;
@@ -364,11 +378,12 @@ static void server_loop(ccs_server* s) {
 }
 
 static void destroy(ccs_server* s) {
+    for (hipStream_t st : {s->fstream, s->pstream, s->cstream})
+        if (st) (void)hipStreamSynchronize(st);
     for (int li = 0; li < 2; ++li) {
-        if (s->streams[li]) (void)hipStreamSynchronize(s->streams[li]);
-        if (s->pstreams[li]) (void)hipStreamSynchronize(s->pstreams[li]);
         for (auto& L : s->lanes[li]) {
             if (L.filtered) (void)hipEventDestroy(L.filtered);
+            if (L.done) (void)hipEventDestroy(L.done);
             if (L.js) ccs_jobset_free(L.js);
             if (L.d_surv) ccs_dfree(s->ctx, L.d_surv);
             if (L.d_res) ccs_dfree(s->ctx, L.d_res);
@@ -379,10 +394,9 @@ static void destroy(ccs_server* s) {
         }
         if (s->h_init[li]) (void)hipHostFree(s->h_init[li]);
         if (s->d_init[li]) ccs_dfree(s->ctx, s->d_init[li]);
-        if (s->ev_init[li]) (void)hipEventDestroy(s->ev_init[li]);
-        if (s->streams[li]) (void)hipStreamDestroy(s->streams[li]);
-        if (s->pstreams[li]) (void)hipStreamDestroy(s->pstreams[li]);
     }
+    for (hipStream_t st : {s->fstream, s->pstream, s->cstream})
+        if (st) (void)hipStreamDestroy(st);
     if (s->d_phases) ccs_dfree(s->ctx, s->d_phases);
     delete s;
 }
@@ -446,11 +460,13 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
         d.count = count;
         return d;
     };
+    // created one after the other, so the runtime's round-robin gives them separate
+    // hardware queues
+    if (hipStreamCreateWithFlags(&s->fstream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->pstream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->cstream, hipStreamNonBlocking) != hipSuccess)
+        rc = ccs::set_error(CCS_ERR_HIP, "server streams");
     for (int li = 0; li < 2 && !rc; ++li) {
-        if (hipStreamCreateWithFlags(&s->streams[li], hipStreamNonBlocking) != hipSuccess ||
-            hipStreamCreateWithFlags(&s->pstreams[li], hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&s->ev_init[li], hipEventDisableTiming) != hipSuccess)
-            rc = ccs::set_error(CCS_ERR_HIP, "server streams");
         if (!rc && s->resident) {
             if (hipHostMalloc((void**)&s->h_init[li], (size_t)s->n_slots * sizeof(ccs::PhaseInit)) != hipSuccess)
                 rc = ccs::set_error(CCS_ERR_NOMEM, "phase init staging");
@@ -482,7 +498,9 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
             if (!rc && hipHostMalloc((void**)&L.h_surv, (size_t)s->p.max_slice_survivors * sizeof(ccs_job_survivor)) != hipSuccess) rc = CCS_ERR_NOMEM;
             if (!rc && hipHostMalloc((void**)&L.h_res, (size_t)s->p.max_slice_survivors * sizeof(ccs_chain_result)) != hipSuccess) rc = CCS_ERR_NOMEM;
             if (!rc && hipHostMalloc((void**)&L.h_count, 64) != hipSuccess) rc = CCS_ERR_NOMEM;
-            if (!rc && hipEventCreateWithFlags(&L.filtered, hipEventDisableTiming) != hipSuccess) rc = CCS_ERR_HIP;
+            if (!rc && (hipEventCreateWithFlags(&L.filtered, hipEventDisableTiming) != hipSuccess ||
+                        hipEventCreateWithFlags(&L.done, hipEventDisableTiming) != hipSuccess))
+                rc = CCS_ERR_HIP;
             s->fixed_bytes += (uint64_t)s->p.max_slice_survivors * (sizeof(ccs_job_survivor) + sizeof(ccs_chain_result));
         }
     }
