@@ -21,67 +21,148 @@ for p in (ROOT, PKG):
 # published campaign baseline (RTX 4090/5090, v13 3-stage filter, depth 18-20): 57-65 G/s
 PUBLISHED_GPU_CAND_PER_S = 65e9
 
-# HBM bytes per sieve_kernel launch of the config-2 step at the default tuning, from two
-# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch), each scaled by what the
-# counter reads for a 1 GiB device copy (bench.py --copy-calibration): FETCH_SIZE reads
-# 524308 KiB for 1 GiB fetched (x2.0), WRITE_SIZE 1048576 KiB for 1 GiB written (x1.0).
-SIEVE_TRAFFIC = {"c2": {"fetch_kib": 2065.25, "write_kib": 1879.53, "fetch_scale": 2.0, "write_scale": 1.0,
+# Counter profile of the config-2 step's sieve_kernel at the default tuning (rocprofv3
+# --pmc, one run per counter set, values per dispatch; the summary with its commands is
+# profiles/sieve_kernel_cc18_31primorial_pmc.md).  HBM traffic: FETCH_SIZE and WRITE_SIZE
+# (KiB per dispatch), each scaled by what the counter reads for a 1 GiB device copy
+# (bench.py --copy-calibration): FETCH_SIZE reads 524308 KiB for 1 GiB fetched (x2.0),
+# WRITE_SIZE 1048576 KiB for 1 GiB written (x1.0).  SQ counters: wave64 VALU and LDS
+# instructions and LDS bank-conflict cycles.
+SIEVE_PROFILE = {"c2": {"tuning": {"n_dense": 41, "n_tierb": 320, "rows_per_group": 4, "unit_j": 1048576},
+                        "fetch_kib": 7791.28, "write_kib": 13808.4, "fetch_scale": 2.0, "write_scale": 1.0,
+                        "sq_insts_valu": 1.77307e9, "sq_insts_lds": 3.63146e8, "sq_lds_bank_conflict": 1.38853e8,
                         "profile": "profiles/sieve_kernel_cc18_31primorial_pmc.md"}}
-# hardware utilization of the sieve kernel from the same counter profile (SQ counters over
-# 256 CUs x kernel cycles): wave64 VALU issue, LDS port bytes against 128 B/clk per CU,
-# LDS bank-conflict cycles
-SIEVE_ISSUE = {"c2": {"valu_issue_frac": 0.57, "lds_port_frac": 0.76, "lds_bank_conflict_frac": 0.068,
-                      "profile": "profiles/sieve_kernel_cc18_31primorial_pmc.md"}}
+LDS_PROBE = os.path.join(PKG, "lds_probe")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_threads():
+def host_cpu():
+    """The host CPU as this process sees it: nproc, the affinity set, the thread share a
+    GPU job is given (OMP_NUM_THREADS, set on the GPU host to the per-GPU share), and lscpu's
+    model, sockets, cores per socket and threads per core."""
+    import subprocess
     try:
-        n = len(os.sched_getaffinity(0))
+        aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        aff = os.cpu_count() or 1
+    info = {"nproc": os.cpu_count(), "affinity": aff, "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except (OSError, subprocess.TimeoutExpired):
+        pass
+    share = aff
+    if info["omp_num_threads"] and info["omp_num_threads"].isdigit():
+        share = min(aff, int(info["omp_num_threads"]))
+    info["threads_used"] = max(1, share)
+    return info
 
 
-def cpu_baseline(name, t0, seconds):
-    """The oracle's CPU restatement of the reference CPU path (the v34-style 64-candidate
-    bit-vector filter, per-candidate for the larger primes), all host threads (<= 16), on a
-    bounded sample of the same workload: consecutive tiles from the same start."""
-    from oracle import pyoracle
-    from ccs import workloads as WL
-    p = WL.shape_params(name)
-    o = pyoracle.Shape(p["crt"], p["wheel"], p["target"], p["depth"], p["filter_primes"], p["pos0_max_q"])
-    nt = cpu_threads()
-    o.filter_bv(t0, 6, cap=1 << 16, nthreads=1)        # one-time table build (startup work)
-    tiles = 6 * nt * 50
-    while True:                                         # calibrate on >= 1 s of work
+def time_port(o, t0, tiles_min, seconds, nthreads, engine, full, kind, bits):
+    """One row of the port matrix: the oracle's v33-structured per-candidate filter
+    ('v33') or v34-structured bit-vector filter ('v34') over consecutive tiles from t0,
+    sized to about `seconds` by a short calibration; full = the prove of every survivor
+    (the GMP-engine semantics, BPSW or the config's MR) included in the time."""
+    filt = o.filter if engine == "v33" else o.filter_bv
+
+    def run(tiles):
         t = time.perf_counter()
-        o.filter_bv(t0, tiles, cap=1 << 22, nthreads=nt)
-        dt = time.perf_counter() - t
-        if dt >= 1.0 or tiles >= 6 * 10 ** 8:
+        surv, c = filt(t0, tiles, cap=1 << 22, nthreads=nthreads)
+        cand = c[0] if engine == "v33" else c
+        primes = 0
+        if full and len(surv):
+            res = o.prove(t0, surv, bits=bits, kind=kind, nthreads=nthreads)
+            primes = int(((res[:, 0] & 4) != 0).sum())
+        return time.perf_counter() - t, cand, len(surv), primes
+
+    tiles = tiles_min
+    while True:
+        dt, *_ = run(tiles)
+        if dt >= 0.25 or tiles >= 6 * 10 ** 8:
             break
         tiles *= 4
-    tiles = int(min(6 * 10 ** 8, tiles * seconds / max(dt, 1e-3)))
+    tiles = max(tiles_min, int(min(6 * 10 ** 8, tiles * seconds / max(dt, 1e-3))))
     tiles -= tiles % 6
-    t = time.perf_counter()
-    surv, cand = o.filter_bv(t0, tiles, cap=1 << 22, nthreads=nt)
-    dt = time.perf_counter() - t
-    model = "cpu"
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                model = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
-    return {"value": cand / dt, "unit": "candidates/s", "cores": nt, "kind": "port",
-            "sample": f"{tiles} consecutive tiles = {cand} candidates of the same shape and start, "
-                      f"{dt:.1f} s on {nt} threads ({model}); v34-style bit-vector filter restated "
-                      f"in C, sieve only (on the reference shape it runs at 2.86 G/s on 8 Xeon "
-                      f"threads where the reference v34 binary ran at 4.07 G/s)"}
+    dt, cand, ns, primes = run(tiles)
+    return {"value": cand / dt, "candidates": cand, "seconds": dt, "tiles": tiles, "survivors": ns,
+            "primes": primes}
+
+
+def time_reference(binary, seconds, nthreads, full, bits=89, target=18):
+    """One row of the reference matrix: the reference engine itself (oracle/_ref, built from
+    the reference's C sources) on its own shape, a sequential sweep from the start of the
+    --bits range, stopped by SIGINT after `seconds` (it finishes the batch in hand and
+    prints its totals); the rate is its own wheel candidates over its own run time."""
+    import re
+    import signal
+    import subprocess
+    cmd = [binary, "--prefix", "0b1", "--sequential", "--bits", str(bits), "--target", str(target),
+           "--threads", str(nthreads), "--report", "60", "--log", str(target)]
+    if not full:
+        cmd.append("--sieve-only")
+    p = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    time.sleep(seconds)
+    p.send_signal(signal.SIGINT)
+    out, _ = p.communicate(timeout=120)
+    w = re.search(r"Wheel candidates \(L1\):\s+(\d+)", out)
+    t = re.search(r"Time:\s+([\d.]+) seconds", out)
+    pr = re.search(r"Primes found:\s+(\d+)", out)
+    if not (w and t) or float(t.group(1)) <= 0:
+        return None
+    return {"value": int(w.group(1)) / float(t.group(1)), "candidates": int(w.group(1)),
+            "seconds": float(t.group(1)), "primes": int(pr.group(1)) if pr else 0}
+
+
+def cpu_baseline(seconds=2.0, configs=("reference", "c1", "c2", "c3")):
+    """The CPU path on this host, every thread of the job's share: the reference's own v33
+    and v34 engines (oracle/_ref, when built) on the reference shape, and the oracle's
+    v33- and v34-structured restatements on each config; sieve-only and full (with the
+    prove of every survivor).  The headline is the fastest sieve-only row."""
+    from oracle import pyoracle
+    from ccs import workloads as WL
+    host = host_cpu()
+    nt = host["threads_used"]
+    rows = []
+    ref_dir = os.path.join(ROOT, "oracle", "_ref")
+    for eng, binary in (("v34", "cc_gmp_v34"), ("v33", "cc_gmp_v33")):
+        path = os.path.join(ref_dir, binary)
+        if not os.access(path, os.X_OK):
+            continue
+        for full in (False, True):
+            r = time_reference(path, seconds + 1.0, nt, full)
+            if r:
+                rows.append(dict(r, kind="reference", engine=eng, config="reference", mode="full" if full else "sieve",
+                                 threads=nt))
+    starts = {"reference": WL.reference_range(bits=89)[0], "c1": WL.c1_range()[0], "c2": WL.c2_range()[0],
+              "c3": WL.c3_range()[0]}
+    for name in configs:
+        p = WL.shape_params(name)
+        o = pyoracle.Shape(p["crt"], p["wheel"], p["target"], p["depth"], p["filter_primes"], p["pos0_max_q"])
+        o.filter_bv(starts[name], 6, cap=1 << 16, nthreads=1)      # table build (startup work)
+        kind, bits = (pyoracle.MR, 60) if name == "c1" else (pyoracle.BPSW, 0)
+        for eng in ("v34", "v33"):
+            for full in (False, True):
+                r = time_port(o, starts[name], 6 * nt, seconds, nt, eng, full, kind, bits)
+                rows.append(dict(r, kind="port", engine=eng, config=name, mode="full" if full else "sieve",
+                                 threads=nt))
+    best = max((r for r in rows if r["mode"] == "sieve"), key=lambda r: r["value"])
+    gap = None
+    pr = [r for r in rows if r["config"] == "reference" and r["mode"] == "sieve" and r["engine"] == "v34"]
+    if len(pr) == 2:
+        gap = [r for r in pr if r["kind"] == "port"][0]["value"] / [r for r in pr if r["kind"] == "reference"][0]["value"]
+    return {"value": best["value"], "unit": "candidates/s", "cores": nt, "kind": best["kind"],
+            "sample": f"fastest sieve-only row: {best['engine']} {best['kind']} on {best['config']}, "
+                      f"{best['candidates']} candidates in {best['seconds']:.1f} s on {nt} threads "
+                      f"({host.get('model', 'cpu')}); every row below ~{seconds:.0f} s",
+            "host": host, "port_over_reference_v34": gap, "matrix": rows}
 
 
 def measure_batched(ccs, WL, ctx, n_jobs=4096, tune_trials=40):
@@ -127,12 +208,15 @@ def measure_batched(ccs, WL, ctx, n_jobs=4096, tune_trials=40):
 
 
 def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_candidates=1 << 37,
-                   max_slice_survivors=1 << 18):
+                   max_slice_survivors=1 << 18, reps=3):
     """Config 5: a Poisson stream of search jobs (depth U{12..20}, width U{64,80,96}, random
-    admissible class mod 31#, length log-uniform in [2^24, 2^34]) at `load` times the server's
-    measured capacity; p50 / p99 latency from submission to completion.  Each depth's shape
-    is autotuned on its own sample jobs (the stage cutoffs that suit depth 12 and depth 20
-    differ)."""
+    admissible class mod 31#, length log-uniform in [2^24, 2^34]).  Capacity is measured on
+    the stream's own jobs: all of them submitted at once and drained by one server, against
+    the plain loop over the same jobs (one filter launch per job, one prove launch per
+    depth), `reps` times each, alternating, for the spread of the ratio.  Then the same jobs
+    arrive as a Poisson stream at `load` times the median capacity: p50 / p99 latency from
+    submission to completion.  Each depth's shape is autotuned on its own sample jobs (the
+    stage cutoffs that suit depth 12 and depth 20 differ)."""
     import numpy as np
     shapes = {d: WL.make_shape(ctx, f"c5d{d}", classes=False) for d in range(12, 21)}
     tune_pool = WL.served_requests(n_jobs=2000, seed=0x7E)
@@ -140,57 +224,68 @@ def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_c
     for d, sh in shapes.items():
         sample = [(r["a"], r["u0"], min(r["count"], 1 << 27)) for r in tune_pool if r["depth"] == d][:192]
         tuned[d], _ = sh.autotune_jobs(sample, max_trials=tune_trials)
-        if tuned[d]["rows_per_group"] != 2:   # one row layout lets the server launch all depths at once
+        if tuned[d]["rows_per_group"] != 2:   # one row layout: fewer launch groups per slice
             try:
                 sh.set_tuning(rows_per_group=2)
                 tuned[d] = sh.tuning
             except ccs.CcsError:
                 pass
-    cal = WL.served_requests(n_jobs=400, seed=0xCA1)
-    # capacity: the calibration jobs drained twice by one server, the second pass timed (the
-    # plain loop below is timed on its second pass as well)
+    reqs = WL.served_requests(n_jobs=n_jobs)
+    total = sum(r["count"] for r in reqs)
+    cal = WL.served_requests(n_jobs=400, seed=0xCA1)     # warm-up jobs (not timed)
     srv = ccs.Server(ctx, shapes, slice_candidates=slice_candidates, max_slice_survivors=max_slice_survivors)
-    for timed in (False, True):
+
+    def served_pass(jobs):
         t = time.perf_counter()
-        for r in cal:
+        for r in jobs:
             srv.submit(r["depth"], r["a"], r["u0"], r["count"])
         srv.drain()
-        if timed:
-            capacity = sum(r["count"] for r in cal) / (time.perf_counter() - t)
+        dt = time.perf_counter() - t
         while srv.poll():
             pass
-    srv.shutdown()
-    # the plain comparator: the same calibration jobs with one filter launch per job (each
-    # with the unit size it would have alone; jobs above 2^32 candidates, the most one
-    # launch takes per job, in pieces of 2^32), then one prove launch per depth
+        return dt
+
+    # the plain comparator: one filter launch per job (each with the unit size it would have
+    # alone; jobs above 2^32 candidates, the most one launch takes per job, in pieces of
+    # 2^32), then one prove launch per depth over that depth's survivors
     cap = 1 << 22
     d_out, d_cnt, d_res = ctx.dalloc(cap * 8), ctx.dalloc(64), ctx.dalloc(cap * 32)
-    by_depth = {}
-    for r in cal:
-        u0, left = r["u0"], r["count"]
-        while left > 0:
-            n = min(left, 1 << 32)
-            by_depth.setdefault(r["depth"], []).append((r["a"], u0, n))
-            u0 += n
-            left -= n
-    plain_s = 0.0
-    for d, jl in by_depth.items():
-        js = ccs.JobSet(shapes[d], jl)
-        js.launch_each(d_out, cap, d_cnt)      # warm-up (and the per-job unit sizes)
+
+    def job_sets(jobs):
+        by_depth = {}
+        for r in jobs:
+            u0, left = r["u0"], r["count"]
+            while left > 0:
+                n = min(left, 1 << 32)
+                by_depth.setdefault(r["depth"], []).append((r["a"], u0, n))
+                u0 += n
+                left -= n
+        return [ccs.JobSet(shapes[d], jl) for d, jl in sorted(by_depth.items())]
+
+    def plain_pass(sets):
         ctx.synchronize()
         t = time.perf_counter()
-        js.launch_each(d_out, cap, d_cnt)
-        js.prove_launch(d_out, d_cnt, cap, d_res)
+        for js in sets:
+            js.launch_each(d_out, cap, d_cnt)
+            js.prove_launch(d_out, d_cnt, cap, d_res)
         ctx.synchronize()
-        plain_s += time.perf_counter() - t
+        return time.perf_counter() - t
+
+    cal_sets, full_sets = job_sets(cal), job_sets(reqs)
+    served_pass(cal)
+    plain_pass(cal_sets)
+    caps, plains = [], []
+    for _ in range(reps):
+        caps.append(total / served_pass(reqs))
+        plains.append(total / plain_pass(full_sets))
+    for js in cal_sets + full_sets:
         js.close()
     for ptr in (d_out, d_cnt, d_res):
         ctx.dfree(ptr)
-    plain = sum(r["count"] for r in cal) / plain_s
-    srv = ccs.Server(ctx, shapes, slice_candidates=slice_candidates, max_slice_survivors=max_slice_survivors)
-    reqs = WL.served_requests(n_jobs=n_jobs)
-    mean = sum(r["count"] for r in reqs) / len(reqs)
-    lam = load * capacity / mean
+    ratios = sorted(c / p for c, p in zip(caps, plains))
+    capacity = float(np.median(caps))
+    # the Poisson stream at `load` times the capacity
+    lam = load * capacity / (total / len(reqs))
     gaps = np.random.default_rng(0xC5).exponential(1.0 / lam, size=len(reqs))
     t0 = time.perf_counter()
     tn = t0
@@ -211,11 +306,13 @@ def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_c
     st = srv.stats()
     srv.shutdown()
     lat = np.array([d["t_done"] - d["t_submit"] for d in done])
-    total = sum(r["count"] for r in reqs)
     return {"workload": "config 5: Poisson stream, depth U{12..20}, width U{64,80,96}, classes mod 31#, "
                         "lengths log-uniform in [2^24, 2^34]",
-            "jobs": len(done), "load": load, "arrival_rate_per_s": lam, "capacity_candidates_per_s": capacity,
-            "plain_loop_candidates_per_s": plain, "speedup_capacity_vs_plain": capacity / plain,
+            "jobs": len(done), "candidates": total, "load": load, "arrival_rate_per_s": lam,
+            "capacity_candidates_per_s": capacity, "capacity_runs": caps,
+            "plain_loop_candidates_per_s": float(np.median(plains)), "plain_loop_runs": plains,
+            "speedup_capacity_vs_plain": ratios[len(ratios) // 2],
+            "speedup_runs": {"min": ratios[0], "median": ratios[len(ratios) // 2], "max": ratios[-1], "n": reps},
             "aggregate_candidates_per_s": total / wall, "wall_s": wall,
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_ms": float(lat.mean() * 1e3), "slices": st["slices"], "launches": st["launches"],
@@ -279,12 +376,63 @@ def measure_families(ccs, ctx, depth=20, q_per_family=1 << 28, exp_start=1, bits
             "survivors": survivors}
 
 
+def lds_probe():
+    """LDS read throughput of this GPU (lds_probe: every CU reading with ds_read_b128)."""
+    import subprocess
+    try:
+        r = subprocess.run([LDS_PROBE], capture_output=True, text=True, timeout=120)
+        return json.loads(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else None
+    except (OSError, ValueError, IndexError, subprocess.TimeoutExpired):
+        return None
+
+
+def roofline(shape, name, cand, ms_sieve, e_tests, value, probe=None):
+    """The sieve kernel against the two resources it runs on: int32 VALU lane-ops (the
+    bound: its algorithmic lower limit on time is the larger one) and LDS bytes, both from
+    the stage work model per candidate and the kernel's HIP-event time; the counter profile
+    beside them (same tuning only); HBM traffic from the profile's FETCH/WRITE counters."""
+    from ccs import workloads as WL
+    info = shape.info
+    m = WL.sieve_work_model(shape.filter_primes, shape.depth, shape.pos0_max_q, info.n_dense, info.n_slots,
+                            info.n_tierb, info.n_tierb_lds)
+    t = ms_sieve / 1e3
+    valu = cand * m["valu_lane_ops"] / t / 1e12
+    lds = cand * m["lds_bytes"] / t / 1e9
+    out = {"bound": "valu", "achieved": valu, "peak": WL.INT32_PEAK_TOPS, "unit": "Tops/s (int32 lane-ops)",
+           "frac": valu / WL.INT32_PEAK_TOPS, "traffic": None, "kernel": "sieve_kernel", "kernel_ms": ms_sieve,
+           "model_per_candidate": {"valu_lane_ops": m["valu_lane_ops"], "lds_bytes": m["lds_bytes"],
+                                   "n_slots": int(info.n_slots), "n_dense": int(info.n_dense),
+                                   "n_tierb": int(info.n_tierb), "p_dense": m["p_dense"],
+                                   "stage_b_probes": m["stage_b_probes"]},
+           "lds": {"achieved": lds, "peak": WL.LDS_PEAK_GBS, "unit": "GB/s", "frac": lds / WL.LDS_PEAK_GBS,
+                   "probe": probe},
+           "equivalent_mod_ops": {"value": value * e_tests, "unit": "residue tests/s",
+                                  "e_tests_per_candidate": e_tests,
+                                  "note": "the reference predicate's per-candidate tests (4 int32 ops each); "
+                                          "a dense pattern OR tests 32 candidates at once, so this is not a "
+                                          "fraction of any peak"}}
+    prof = SIEVE_PROFILE.get(name)
+    tun = shape.tuning
+    if prof and all(tun.get(k) == v for k, v in prof["tuning"].items()):
+        out["traffic"] = (prof["fetch_kib"] * prof["fetch_scale"] + prof["write_kib"] * prof["write_scale"]) * 1024
+        cu_clk = WL.MI355X_CUS * t * WL.MI355X_CLK_GHZ * 1e9
+        out["hardware"] = {
+            "valu_issue_frac": prof["sq_insts_valu"] / cu_clk,
+            "valu_lane_ops_per_candidate": prof["sq_insts_valu"] * 64 / cand,
+            "lds_insts": prof["sq_insts_lds"],
+            "lds_insts_model": cand * (m["dense_lds_reads_b128"] + m["stage_b_lds_reads"]),
+            "lds_bank_conflict_frac": prof["sq_lds_bank_conflict"] / cu_clk,
+            "profile": prof["profile"]}
+    return out
+
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=2.0, help="seconds per CPU-baseline matrix row")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="c2", choices=["c2", "reference", "c3"])
     ap.add_argument("--tune", action="store_true",
@@ -315,6 +463,11 @@ def main():
     import ccs
     from ccs import workloads as WL
 
+    # the LDS ceiling of this box, measured in a child process before this one opens the GPU
+    probe = lds_probe()
+    if probe:
+        probe = {"b128_gbs": probe["b128_bytes_per_s"] / 1e9, "b32_pair_gbs": probe["b32_pair_bytes_per_s"] / 1e9,
+                 "clock_khz": probe["clock_khz"], "cus": probe["cus"]}
     ctx = ccs.Context(0)
     name = args.config
     shape = WL.make_shape(ctx, name)
@@ -398,13 +551,7 @@ def main():
         ctx.d2h(res, d_res[last])
     value = cand * args.steps / dt
     ms_sieve = kt["ms_sieve"] / max(kt["n_sieve"], 1)
-    # 4 int32 VALU ops per residue test, against the int32 VALU peak
-    tr = SIEVE_TRAFFIC.get(name)
-    traffic = None
-    if tr and tr["fetch_kib"] is not None:
-        traffic = (tr["fetch_kib"] * tr["fetch_scale"] + tr["write_kib"] * tr["write_scale"]) * 1024
-    ops = cand * e_tests * WL.INT32_OPS_PER_TEST
-    achieved = ops / (ms_sieve / 1e3) / 1e12
+    roof = roofline(shape, name, cand, ms_sieve, e_tests, value, probe)
     out = {
         "metric": "filtered candidates/s at CC18 depth (bit-exact)",
         "value": value,
@@ -420,11 +567,7 @@ def main():
         "data": "synthetic: seeded tile range (xorshift64), no dataset",
         "config": {"workload": workload, "tile_start": str(t0), "tile_count": tc,
                    "candidates_per_step": cand, "tuning": shape.tuning},
-        "roofline": {"bound": "valu_int32", "achieved": achieved, "peak": WL.INT32_PEAK_TOPS,
-                     "unit": "Tops/s (int32 VALU)", "frac": achieved / WL.INT32_PEAK_TOPS,
-                     "traffic": traffic, "kernel": "sieve_kernel",
-                     "kernel_ms": ms_sieve, "e_tests_per_candidate": e_tests,
-                     "ops_per_launch": ops, "hardware": SIEVE_ISSUE.get(name)},
+        "roofline": roof,
         "kernels_ms": {"sieve": ms_sieve, "deep": kt["ms_deep"] / max(kt["n_deep"], 1),
                        "prove": kt["ms_prove"] / max(kt["n_prove"], 1)},
         "survivors_per_step": int(n_surv[0]),
@@ -440,7 +583,7 @@ def main():
         out["families"] = measure_families(ccs, ctx)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(name, t0, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
     ctx.close()

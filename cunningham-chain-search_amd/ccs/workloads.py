@@ -202,14 +202,36 @@ def served_requests(n_jobs=10000, seed=0xC5, lo_exp=24, hi_exp=34):
     return out
 
 
-# analytic work per candidate (ascending-prime early exit with uniform residues): the
-# integer-VALU roofline charges 4 int32 ops per residue test
-E_TESTS = {"reference": 2.458, "c1": 4.90, "c2": 2.45, "c3": 2.66}
-INT32_OPS_PER_TEST = 4
+# One MI355X (gfx950): 256 CUs at 2.4 GHz.  A CU issues one wave64 VALU instruction per
+# clock (four SIMD16s, four clocks each), i.e. 64 int32 lane-ops per clock, and its LDS
+# returns 256 B per clock to 128-bit reads (the LDS throughput probe, lds_probe, reads
+# 118 TB/s with ds_read_b128 over the whole chip, above the 78.6 TB/s a 128 B/clk port
+# would allow).
 MI355X_CUS = 256
 MI355X_LANES_PER_CU = 64
 MI355X_CLK_GHZ = 2.4
+MI355X_LDS_BYTES_PER_CLK = 256
 INT32_PEAK_TOPS = MI355X_CUS * MI355X_LANES_PER_CU * MI355X_CLK_GHZ * 1e9 / 1e12   # 39.3
+LDS_PEAK_GBS = MI355X_CUS * MI355X_LDS_BYTES_PER_CLK * MI355X_CLK_GHZ                # 157 286
+
+# The per-candidate figure of the reference's predicate (one residue test per prime until
+# the first kill, 4 int32 ops each): kept as "equivalent mod-ops", not as a roofline.
+INT32_OPS_PER_TEST = 4
+
+# The sieve kernel's algorithmic work per candidate, by stage:
+#  dense stage, per pattern (slot) and 128 candidates of a lane: two aligned 16-byte LDS
+#    blocks (32 B), four v_alignbit (the 128-bit window) and two v_or3 (one 3-input OR
+#    folds two patterns into a word), i.e. 0.25 B and 6/128 lane-ops per candidate;
+#    the row's live words: a complement and a non-zero test per 32 candidates;
+#  stage B, per candidate alive after the dense stage and group of 64 primes probed: every
+#    lane tests its prime (phase add, f32-reciprocal quotient cvt/mul/cvt, v_mad_i32_i24
+#    remainder, word index shift/and/add, bit extract, compare = 10 lane-ops) with one
+#    4-byte bitset read (LDS for the groups held there, L2 for the rest).
+DENSE_LDS_BYTES_PER_SLOT = 32 / 128
+DENSE_VALU_PER_SLOT = 6 / 128
+ROW_VALU_PER_CANDIDATE = 2 / 32
+STAGE_B_VALU_PER_PROBE = 10 * 64
+STAGE_B_BYTES_PER_PROBE = 4 * 64
 
 
 def expected_tests(fp_sizes):
@@ -219,3 +241,27 @@ def expected_tests(fp_sizes):
         e += alive
         alive *= 1.0 - nf / q
     return e, alive
+
+
+def sieve_work_model(filter_primes, depth, pos0_max_q, n_dense, n_slots, n_tierb, n_tierb_lds):
+    """Algorithmic VALU lane-ops and LDS bytes per candidate of the sieve kernel for a
+    shape split as (n_dense primes in n_slots patterns, n_tierb stage-B primes of which the
+    first n_tierb_lds have their bitsets in LDS), with the survival fractions from the
+    forbidden-residue sets (uniform residues)."""
+    alive, p_dense, groups, lds_groups = 1.0, None, 0.0, 0.0
+    for i, q in enumerate(filter_primes[: n_dense + n_tierb]):
+        if i == n_dense:
+            p_dense = alive
+        if i >= n_dense and (i - n_dense) % 64 == 0:   # a stage-B group starts: probed if alive
+            groups += alive
+            if i - n_dense < n_tierb_lds:
+                lds_groups += alive
+        lo = 0 if q <= pos0_max_q else 1
+        alive *= 1.0 - len(forbidden(q, lo, depth)) / q
+    if p_dense is None:
+        p_dense = alive
+    valu = n_slots * DENSE_VALU_PER_SLOT + ROW_VALU_PER_CANDIDATE + groups * STAGE_B_VALU_PER_PROBE
+    lds = n_slots * DENSE_LDS_BYTES_PER_SLOT + lds_groups * STAGE_B_BYTES_PER_PROBE
+    return {"valu_lane_ops": valu, "lds_bytes": lds, "p_dense": p_dense,
+            "stage_b_probes": groups, "stage_b_lds_probes": lds_groups, "p_stage_b": alive,
+            "dense_lds_reads_b128": n_slots * 2 / 8192, "stage_b_lds_reads": lds_groups}

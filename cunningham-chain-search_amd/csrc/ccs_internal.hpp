@@ -280,6 +280,14 @@ hipError_t launch_phase_init(const PhaseInit* d_jobs, u32 n_jobs, u32* phases, u
 u32 shape_phase_words(const ccs_shape* s);
 // the expected fraction of a shape's candidates that survive every filter prime
 double shape_final_survival(const ccs_shape* s);
+// stage-B register width (groups of 64 primes) of the sieve kernel instance that runs a
+// shape with n_tierb stage-B primes: one group per 64 primes, widths above 8 rounded up
+// to 12 or 16 (the instances compiled)
+inline u32 sieve_stage_b_width(u32 n_tierb) {
+    u32 w = n_tierb ? (n_tierb + 63) / 64 : 1;
+    if (w > 8) w = w <= 12 ? 12 : 16;
+    return w;
+}
 // the residue constants of a shape (device, dense slots then stage B)
 const PrimeMod* shape_pmod(const ccs_shape* s);
 // a group-set update whose jobs carry resident phases: job i reads slot slots[i] of the arena,

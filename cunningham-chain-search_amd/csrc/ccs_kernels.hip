@@ -1307,7 +1307,7 @@ hipError_t launch_sieve(const SieveArgs& a, int rg, int nbv, int grid, int block
     CCS_SIEVE_CASE(R, 5) CCS_SIEVE_CASE(R, 6) CCS_SIEVE_CASE(R, 7) CCS_SIEVE_CASE(R, 8)      \
     CCS_SIEVE_CASE(R, 12) CCS_SIEVE_CASE(R, 16)
     // more than 8 stage-B groups: the next instance up (lanes past the last prime are padding)
-    if (nbv > 8) nbv = nbv <= 12 ? 12 : 16;
+    nbv = (int)sieve_stage_b_width((u32)nbv * 64u);
     CCS_SIEVE_ROW(1) CCS_SIEVE_ROW(2) CCS_SIEVE_ROW(4)
 #undef CCS_SIEVE_ROW
 #undef CCS_SIEVE_CASE

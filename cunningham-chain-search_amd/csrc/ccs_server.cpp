@@ -4,15 +4,21 @@
 // slot of the server's device arena is free: the arena holds each admitted job's residue
 // state (the phases of its sieve primes at the request's u0, written once by a kernel at
 // admission) and is sized by the memory budget.  Every slice takes the next chunk of every
-// admitted job, runs one persistent group launch over all depth profiles (the jobs' sieve
-// phases advanced from their resident state) + one prove launch, and hands results back per
-// job.  Requests that arrive while slices are on the GPU join the next slice.  Two buffer
-// lanes alternate so the GPU runs slice k+1 while the host distributes slice k.  The filter
-// launches of every slice go to one stream and the proves (with the result copies) to a
-// second one, behind an event per launch group, so a slice's prove overlaps the next
-// slice's filter: with a filter and a prove stream per lane, the runtime put lane 1's filter
-// stream and lane 0's prove stream on one hardware queue (streams share the few hardware
-// queues round-robin), which ran every filter behind the previous slice's prove.
+// admitted job and runs, per launch group of depth profiles, one persistent group launch
+// (the jobs' sieve phases advanced from their resident state) + one prove launch, and hands
+// results back per job.  Depth profiles share a launch group when their sieve kernel
+// instances are close (same row layout, stage-B register widths within a factor of two): a
+// launch runs at the widest instance of its group, and the shallow profiles' 16-group
+// instance slows the deep ones.  Requests that arrive while slices are on the GPU join the
+// next slice.  Three buffer lanes rotate: slice k is launched before slice k-2 is collected,
+// so the host's collection and planning never leave the GPU without a queued slice (with
+// two lanes, a slice could only be launched once the previous one's prove, which runs in
+// the tail of the next filter, was collected).  The filter launches of every slice go to
+// one stream and the proves (with the result copies) to a second one, behind an event per
+// launch group, so a slice's prove overlaps the next slice's filter: with a filter and a
+// prove stream per lane, the runtime put one lane's filter stream and another's prove
+// stream on one hardware queue (streams share the few hardware queues round-robin), which
+// ran every filter behind the previous slice's prove.
 //
 // The reference has no serving mode; its closest pattern is the v16 dispatch ring with
 // an asynchronous prove pool (v16...cu:2938-3279, :5594-5965), whose drained per-variant
@@ -38,6 +44,7 @@ namespace {
 using ccs::kNoSlot;
 
 constexpr uint32_t kEagerMin = 4096;         // survivors copied back with every slice, at least
+constexpr int kLanes = 3;                     // slices in flight: one collected, two on the GPU
 constexpr uint64_t kMaxRequest = 1ull << 48;  // resident phases advance exactly below this offset
 
 double now_s() {
@@ -93,8 +100,13 @@ struct ccs_server {
     hipStream_t fstream = nullptr;          // phase inits and filter launches
     hipStream_t pstream = nullptr;          // prove launches and their result copies
     hipStream_t cstream = nullptr;          // copies of a large slice's remaining results
-    std::vector<Lane> lanes[2];             // [lane][launch group]; one group when grouped
-    bool grouped = false;                   // several depth profiles: one launch per slice
+    std::vector<Lane> lanes[kLanes];        // [lane][launch group]
+    // launch groups: the depth profiles whose sieve launches share one kernel instance
+    // closely enough to run as one group launch (same row layout, stage-B register widths
+    // within a factor of two); lg_of[profile], pos_in_lg[profile] = its index in the group
+    std::vector<std::vector<const ccs_shape*>> lg_shapes;
+    std::vector<std::vector<uint32_t>> lg_profiles;
+    std::vector<uint32_t> lg_of, pos_in_lg;
     bool resident = true;                   // phases from the arena (else from u0 per unit)
     // the arena: per slot `stride` words of phases
     uint32_t stride = 0, n_slots = 0;
@@ -102,8 +114,8 @@ struct ccs_server {
     uint64_t job_bytes = 0, fixed_bytes = 0;
     std::vector<uint32_t> free_slots;
     std::vector<ccs::PhaseInit> inits;      // admitted since the last slice launch
-    ccs::PhaseInit* h_init[2] = {nullptr, nullptr};   // pinned staging per lane
-    ccs::PhaseInit* d_init[2] = {nullptr, nullptr};
+    ccs::PhaseInit* h_init[kLanes] = {};    // pinned staging per lane
+    ccs::PhaseInit* d_init[kLanes] = {};
     std::mutex mu;
     std::condition_variable cv;
     std::deque<uint64_t> pending;
@@ -285,9 +297,9 @@ static void server_loop(ccs_server* s) {
                 r.u0.lo = u0lo;
                 r.u0.hi = j.req.u0.hi + (u0lo < j.req.u0.lo);
                 r.count = n;
-                const size_t gi = s->grouped ? 0 : (size_t)j.group;
+                const size_t gi = s->lg_of[j.group];
                 recs[gi].push_back(r);
-                shp[gi].push_back(s->grouped ? (uint32_t)j.group : 0u);
+                shp[gi].push_back(s->pos_in_lg[j.group]);
                 slots[gi].push_back(j.slot);
                 offs[gi].push_back(j.cursor);   // the resident phases are those at u0
                 plan[gi].push_back(Entry{id, j.cursor});
@@ -349,7 +361,7 @@ static void server_loop(ccs_server* s) {
             double expect = 0;
             for (size_t i = 0; i < L.jobs.size(); ++i)
                 expect += (double)L.jobs[i].count *
-                          ccs::shape_final_survival(s->shapes[s->grouped ? L.job_shape[i] : gi]);
+                          ccs::shape_final_survival(s->lg_shapes[gi][L.job_shape[i]]);
             L.eager = (uint32_t)std::min<double>(s->p.max_slice_survivors, 2.0 * expect + kEagerMin);
             const size_t pre = L.eager;
             if (!rc && hipMemcpyAsync(L.h_surv, L.d_surv, pre * sizeof(ccs_job_survivor),
@@ -365,22 +377,23 @@ static void server_loop(ccs_server* s) {
             s->stat_launches++;
         }
         if (launched) s->stat_slices++;
-        // while lane li runs, collect the other lane
-        if ((rc = collect_lane(s, li ^ 1))) { server_fail(s, rc, "collect"); return; }
+        // while lanes li and li-1 are on the GPU, collect the oldest slice (lane li+1, whose
+        // buffers the next slice takes)
+        const int next = (li + 1) % kLanes;
+        if ((rc = collect_lane(s, next))) { server_fail(s, rc, "collect"); return; }
         {
             std::lock_guard<std::mutex> g(s->mu);
             s->busy_s += now_s() - t_busy;
         }
-        li ^= 1;
+        li = next;
     }
-    collect_lane(s, 0);
-    collect_lane(s, 1);
+    for (int k = 1; k <= kLanes; ++k) collect_lane(s, (li + k) % kLanes);
 }
 
 static void destroy(ccs_server* s) {
     for (hipStream_t st : {s->fstream, s->pstream, s->cstream})
         if (st) (void)hipStreamSynchronize(st);
-    for (int li = 0; li < 2; ++li) {
+    for (int li = 0; li < kLanes; ++li) {
         for (auto& L : s->lanes[li]) {
             if (L.filtered) (void)hipEventDestroy(L.filtered);
             if (L.done) (void)hipEventDestroy(L.done);
@@ -436,14 +449,40 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
     s->p.max_active = std::min(s->p.max_active, s->n_slots);
     for (uint32_t k = s->n_slots; k-- > 0;) s->free_slots.push_back(k);
     int rc = CCS_OK;
-    // several depth profiles with one row layout: one group launch per slice
-    s->grouped = n_shapes > 1;
+    // launch groups: profiles sorted by stage-B register width (widest first); a group
+    // takes the next profile while it has the group's row layout and at least half the
+    // group's widest stage-B width
     {
-        ccs_tuning t0, t;
-        rc = ccs_shape_get_tuning(shapes[0], &t0);
-        for (uint32_t i = 1; i < n_shapes && !rc; ++i) {
+        std::vector<uint32_t> order(n_shapes), width(n_shapes), rows(n_shapes);
+        for (uint32_t i = 0; i < n_shapes && !rc; ++i) {
+            ccs_tuning t;
             rc = ccs_shape_get_tuning(shapes[i], &t);
-            if (!rc && t.rows_per_group != t0.rows_per_group) s->grouped = false;
+            ccs_shape_info info;
+            if (!rc) rc = ccs_shape_query(shapes[i], &info);
+            order[i] = i;
+            rows[i] = t.rows_per_group;
+            width[i] = ccs::sieve_stage_b_width(info.n_tierb);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+            return rows[a] != rows[b] ? rows[a] < rows[b] : width[a] > width[b];
+        });
+        s->lg_of.assign(n_shapes, 0);
+        s->pos_in_lg.assign(n_shapes, 0);
+        for (uint32_t k = 0; k < n_shapes && !rc; ++k) {
+            const uint32_t i = order[k];
+            bool fresh = s->lg_profiles.empty();
+            if (!fresh) {
+                const uint32_t head = s->lg_profiles.back().front();
+                fresh = rows[head] != rows[i] || 2 * width[i] < width[head];
+            }
+            if (fresh) {
+                s->lg_profiles.emplace_back();
+                s->lg_shapes.emplace_back();
+            }
+            s->lg_of[i] = (uint32_t)s->lg_profiles.size() - 1;
+            s->pos_in_lg[i] = (uint32_t)s->lg_profiles.back().size();
+            s->lg_profiles.back().push_back(i);
+            s->lg_shapes.back().push_back(shapes[i]);
         }
     }
     if (s->resident && !rc)
@@ -466,17 +505,17 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
         hipStreamCreateWithFlags(&s->pstream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&s->cstream, hipStreamNonBlocking) != hipSuccess)
         rc = ccs::set_error(CCS_ERR_HIP, "server streams");
-    for (int li = 0; li < 2 && !rc; ++li) {
+    for (int li = 0; li < kLanes && !rc; ++li) {
         if (!rc && s->resident) {
             if (hipHostMalloc((void**)&s->h_init[li], (size_t)s->n_slots * sizeof(ccs::PhaseInit)) != hipSuccess)
                 rc = ccs::set_error(CCS_ERR_NOMEM, "phase init staging");
             if (!rc) rc = ccs_dmalloc(ctx, (void**)&s->d_init[li], (uint64_t)s->n_slots * sizeof(ccs::PhaseInit));
         }
-        s->lanes[li].resize(s->grouped ? 1 : n_shapes);
+        s->lanes[li].resize(s->lg_profiles.size());
         for (uint32_t gi = 0; gi < s->lanes[li].size() && !rc; ++gi) {
             Lane& L = s->lanes[li][gi];
-            // a group set over every profile (grouped), or over the one profile of this group
-            const uint32_t first = s->grouped ? 0 : gi, count = s->grouped ? n_shapes : 1;
+            // a group set over the profiles of this launch group
+            const uint32_t count = (uint32_t)s->lg_shapes[gi].size();
             std::vector<ccs_job> dummy;
             std::vector<uint32_t> dshape;
             for (uint32_t k = 0; k < count; ++k)
@@ -490,8 +529,8 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
                 dummy.push_back(dummy_job(per));
                 dshape.push_back(count - 1);
             }
-            rc = ccs_groupset_create(ctx, shapes + first, count, dummy.data(), dshape.data(), (uint32_t)dummy.size(),
-                                     &L.js);
+            rc = ccs_groupset_create(ctx, s->lg_shapes[gi].data(), count, dummy.data(), dshape.data(),
+                                     (uint32_t)dummy.size(), &L.js);
             if (!rc) rc = ccs_dmalloc(ctx, (void**)&L.d_surv, (uint64_t)s->p.max_slice_survivors * sizeof(ccs_job_survivor));
             if (!rc) rc = ccs_dmalloc(ctx, (void**)&L.d_res, (uint64_t)s->p.max_slice_survivors * sizeof(ccs_chain_result));
             if (!rc) rc = ccs_dmalloc(ctx, (void**)&L.d_count, 64);
