@@ -207,8 +207,8 @@ def measure_batched(ccs, WL, ctx, n_jobs=4096, tune_trials=40):
             "tuning": tuned}
 
 
-def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_candidates=1 << 37,
-                   max_slice_survivors=1 << 18, reps=3):
+def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_candidates=1 << 38,
+                   max_slice_survivors=1 << 19, reps=3):
     """Config 5: a Poisson stream of search jobs (depth U{12..20}, width U{64,80,96}, random
     admissible class mod 31#, length log-uniform in [2^24, 2^34]).  Capacity is measured on
     the stream's own jobs: all of them submitted at once and drained by one server, against
@@ -316,6 +316,7 @@ def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_c
             "aggregate_candidates_per_s": total / wall, "wall_s": wall,
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_ms": float(lat.mean() * 1e3), "slices": st["slices"], "launches": st["launches"],
+            "incomplete_jobs": st["incomplete"],
             "slice_candidates": slice_candidates, "tuning": {str(d): t for d, t in tuned.items()}}
 
 

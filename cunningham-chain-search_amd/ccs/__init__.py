@@ -495,14 +495,14 @@ class Server:
 
     def __init__(self, ctx, shapes_by_depth, mem_budget_bytes=256 << 20, max_active=1024,
                  max_slice_survivors=1 << 16, slice_candidates=1 << 35, min_chunk=1 << 18,
-                 max_chunk=1 << 32, bits=0, test="bpsw", bases=(2,), recompute=False):
+                 max_chunk=1 << 32, bits=0, test="bpsw", bases=(2,), recompute=False, group_width_ratio=0):
         self.ctx, self.lib = ctx, ctx.lib
         self.depths = sorted(shapes_by_depth)
         self.shapes = [shapes_by_depth[d] for d in self.depths]
         arr = (C.c_void_p * len(self.shapes))(*[sh.h.value for sh in self.shapes])
         dep = (C.c_int32 * len(self.depths))(*self.depths)
         flags = L.SERVER_RECOMPUTE_PHASES if recompute else 0
-        p = L.ServerParams(mem_budget_bytes, max_active, flags, max_slice_survivors, 0,
+        p = L.ServerParams(mem_budget_bytes, max_active, flags, max_slice_survivors, group_width_ratio,
                            slice_candidates, min_chunk, max_chunk, _prove_params(bits, test, bases))
         self.h = C.c_void_p()
         self.incomplete = False

@@ -180,7 +180,7 @@ class KernelTimes(C.Structure):
 
 class ServerParams(C.Structure):
     _fields_ = [("mem_budget_bytes", C.c_uint64), ("max_active", C.c_uint32), ("flags", C.c_uint32),
-                ("max_slice_survivors", C.c_uint32), ("pad", C.c_uint32), ("slice_candidates", C.c_uint64),
+                ("max_slice_survivors", C.c_uint32), ("group_width_ratio", C.c_uint32), ("slice_candidates", C.c_uint64),
                 ("min_chunk", C.c_uint64), ("max_chunk", C.c_uint64), ("prove", ProveParams)]
 
 

@@ -438,8 +438,8 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
     }
     s->stride = (s->stride + 63) & ~63u;   // slots on 256-byte boundaries
     // one admitted job's device state: its phases in the arena, and its rows (job record and
-    // unit prefix) in both lanes' slice tables
-    s->job_bytes = (s->resident ? (uint64_t)s->stride * 4 : 0) + 2 * (sizeof(ccs::JobRec) + 8);
+    // unit prefix) in every lane's slice table
+    s->job_bytes = (s->resident ? (uint64_t)s->stride * 4 : 0) + kLanes * (sizeof(ccs::JobRec) + 8);
     const uint64_t fit = s->p.mem_budget_bytes / s->job_bytes;
     s->n_slots = (uint32_t)std::min<uint64_t>(s->p.max_active, fit);
     if (!s->n_slots) {
@@ -450,8 +450,9 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
     for (uint32_t k = s->n_slots; k-- > 0;) s->free_slots.push_back(k);
     int rc = CCS_OK;
     // launch groups: profiles sorted by stage-B register width (widest first); a group
-    // takes the next profile while it has the group's row layout and at least half the
-    // group's widest stage-B width
+    // takes the next profile while it has the group's row layout and at least 1/ratio of
+    // the group's widest stage-B width
+    if (!s->p.group_width_ratio) s->p.group_width_ratio = 2;
     {
         std::vector<uint32_t> order(n_shapes), width(n_shapes), rows(n_shapes);
         for (uint32_t i = 0; i < n_shapes && !rc; ++i) {
@@ -473,7 +474,7 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
             bool fresh = s->lg_profiles.empty();
             if (!fresh) {
                 const uint32_t head = s->lg_profiles.back().front();
-                fresh = rows[head] != rows[i] || 2 * width[i] < width[head];
+                fresh = rows[head] != rows[i] || s->p.group_width_ratio * width[i] < width[head];
             }
             if (fresh) {
                 s->lg_profiles.emplace_back();

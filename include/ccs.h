@@ -504,7 +504,9 @@ typedef struct {
     uint32_t max_active;            /* admitted jobs at most (and slice table size)         */
     uint32_t flags;                 /* CCS_SERVER_*                                         */
     uint32_t max_slice_survivors;   /* survivor buffer per slice and depth group            */
-    uint32_t pad;
+    uint32_t group_width_ratio;     /* depth profiles share a launch per slice while their */
+                                    /*   stage-B register widths differ by at most this     */
+                                    /*   factor (0: 2; 1: one launch per kernel instance)   */
     uint64_t slice_candidates;      /* target candidates per slice over all admitted jobs   */
     uint64_t min_chunk, max_chunk;  /* bounds of one job's chunk per slice (<= 2^32)        */
     ccs_prove_params prove;         /* prove settings for every job                         */

@@ -87,8 +87,8 @@ def test_admission_by_memory_budget(ctx, shapes):
     st = probe.stats()
     probe.shutdown()
     # one job's resident state: the phases of its widest profile's sieve primes (at least the
-    # stage-B primes of depth 12) plus its rows in both lanes' slice tables
-    assert st["job_bytes"] >= 4 * int(shapes[12].info.n_tierb) + 2 * 80
+    # stage-B primes of depth 12) plus its rows in the three lanes' slice tables
+    assert st["job_bytes"] >= 4 * int(shapes[12].info.n_tierb) + 3 * 80
     # a budget for two resident jobs: the rest wait for admission, all complete
     srv = ccs.Server(ctx, shapes, mem_budget_bytes=2 * st["job_bytes"] + 100, slice_candidates=1 << 20,
                      min_chunk=1 << 14)
