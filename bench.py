@@ -325,7 +325,8 @@ def measure_families(ccs, ctx, depth=20, q_per_family=1 << 28, exp_start=1, bits
     fingerprints (known_fingerprints.txt), p_0 = Q*D*2^E - 1, each family's Q window for
     90..127-bit p_0, sieved at depth 20 by the primes 3..1009 (v16's GPU primes to 503 and
     its CPU ext sieve 509..1009).  One group launch over all families against one launch
-    per family (v16's per-variant dispatch); the unit is (Q, V) pairs per second."""
+    per family (v16's per-variant dispatch), each form at the best tuning a coordinate
+    descent finds for it on the launch itself; the unit is (Q, V) pairs per second."""
     import random
     pool = ccs.pool_load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "pools",
                                       "known_fingerprints.txt"))
@@ -346,7 +347,7 @@ def measure_families(ccs, ctx, depth=20, q_per_family=1 << 28, exp_start=1, bits
     cap = 1 << 22
     d_out, d_cnt = ctx.dalloc(cap * 8), ctx.dalloc(64)
 
-    def best_of(fn, reps=3):
+    def best_of(fn, reps=2):
         fn()
         ctx.synchronize()
         best = 1e30
@@ -357,13 +358,63 @@ def measure_families(ccs, ctx, depth=20, q_per_family=1 << 28, exp_start=1, bits
             best = min(best, time.perf_counter() - t)
         return best
 
+    defaults = [sh.tuning for sh in shapes]
+
+    def apply(tun):
+        """one tuning for every family (None: each family's own default); CcsError when it
+        does not fit some family's tables"""
+        for sh, own in zip(shapes, defaults):
+            if tun is None:
+                sh.set_tuning(**own)
+                continue
+            nf = int(sh.info.n_filter)   # a family's immune primes are not among its filter primes
+            nd = min(tun["n_dense"], nf)
+            sh.set_tuning(**dict(tun, n_dense=nd, n_tierb=min(tun["n_tierb"], nf - nd)))
+
+    def group_s(tun):
+        apply(tun)
+        gs = ccs.GroupSet(shapes, jobs)
+        t = best_of(lambda: gs.launch(d_out, cap, d_cnt))
+        gs.close()
+        return t
+
+    def loop_s(tun):
+        apply(tun)
+        sets = [ccs.JobSet(shapes[k], [j for kk, j in jobs if kk == k]) for k in range(len(shapes))]
+        t = best_of(lambda: [js.launch(d_out, cap, d_cnt) for js in sets])
+        for js in sets:
+            js.close()
+        return t
+
+    def tune(timer):
+        """coordinate descent on the launch form itself: a group launch reloads each family's
+        LDS image as its blocks reach that family, so the single-shape autotuner (which never
+        pays that) misjudges pattern sizes there"""
+        best, tb = None, timer(None)
+        start = defaults[0]
+        knobs = (("n_dense", [sum(1 for q in primes if q < c) for c in (64, 100, 128, 200, 256)]),
+                 ("pair_max", [0, 4096, 16384, 65536]), ("unit_j", [65536, 262144, 1048576]),
+                 ("n_tierb", [64, 128, 192]))
+        for knob, values in knobs:
+            for v in values:
+                base = best or start
+                if v == base[knob]:
+                    continue
+                t = dict(base, **{knob: v})
+                try:
+                    tt = timer(t)
+                except ccs.CcsError:
+                    continue
+                if tt < tb:
+                    best, tb = t, tt
+        return best, tb
+
+    tuned_g, tg = tune(group_s)
+    tuned_l, tl = tune(loop_s)
+    apply(tuned_g)
     gs = ccs.GroupSet(shapes, jobs)
-    tg = best_of(lambda: gs.launch(d_out, cap, d_cnt))
+    gs.launch(d_out, cap, d_cnt)
     survivors = gs.stats(d_cnt)["survivors"]
-    sets = [ccs.JobSet(shapes[k], [j for kk, j in jobs if kk == k]) for k in range(len(shapes))]
-    tl = best_of(lambda: [js.launch(d_out, cap, d_cnt) for js in sets])
-    for js in sets:
-        js.close()
     gs.close()
     ctx.dfree(d_out)
     ctx.dfree(d_cnt)
@@ -374,7 +425,7 @@ def measure_families(ccs, ctx, depth=20, q_per_family=1 << 28, exp_start=1, bits
                         f"2^{q_per_family.bit_length() - 1} Q per family",
             "families": len(pool), "pairs": total, "group_s": tg, "per_family_loop_s": tl,
             "group_pairs_per_s": total / tg, "per_family_loop_pairs_per_s": total / tl,
-            "survivors": survivors}
+            "survivors": survivors, "tuning_group": tuned_g, "tuning_per_family_loop": tuned_l}
 
 
 def lds_probe():
@@ -532,6 +583,27 @@ def main():
         t_tune = time.perf_counter()
         tuned, ms = shape.autotune(t0, tc, max_trials=args.tune_trials)
         log(f"[bench] autotune {time.perf_counter() - t_tune:.1f}s -> {tuned} ({ms:.2f} ms on the sample)")
+    if not args.tuning:
+        # rows per lane step chosen on whole steps (setup work, not timed): with 4 rows the
+        # sieve takes a CU's whole register file, so a prove-heavy shape (the v13 campaign
+        # shape proves ~8·10^5 survivors a step) can no longer run its prove beside it
+        best_rg, best_t = None, 1e30
+        for rg in (4, 2):
+            try:
+                shape.set_tuning(rows_per_group=rg)
+            except ccs.CcsError:
+                continue
+            for _ in range(3):
+                step()
+            ctx.synchronize()
+            t = time.perf_counter()
+            for _ in range(8):
+                step()
+            ctx.synchronize()
+            if time.perf_counter() - t < best_t:
+                best_rg, best_t = rg, time.perf_counter() - t
+        if best_rg is not None:
+            shape.set_tuning(rows_per_group=best_rg)
     log(f"[bench] {name}: tiles={tc} candidates/step={cand} tuning={shape.tuning}")
     # ctx.synchronize() is a device-wide hipDeviceSynchronize: both streams are drained
     for _ in range(args.warmup):
