@@ -207,6 +207,60 @@ def measure_batched(ccs, WL, ctx, n_jobs=4096, tune_trials=40):
             "tuning": tuned}
 
 
+def tune_rows_fixed(ccs, ctx, sh, sample, rows=2):
+    """A shape's tuning with its row layout fixed (the server launches the depth profiles of a
+    launch group together, so they share one): coordinate descent from the autotuner's
+    pick, timed on the job-set launch over the sample jobs.  The dense cutoff that suits 4
+    rows does not suit 2 (the patterns' LDS extension halves)."""
+    cap = 1 << 22
+    d_out, d_cnt = ctx.dalloc(cap * 8), ctx.dalloc(64)
+
+    def timed(t):
+        sh.set_tuning(**t)
+        js = ccs.JobSet(sh, sample)
+        js.launch(d_out, cap, d_cnt)
+        ctx.synchronize()
+        best = 1e30
+        for _ in range(2):
+            t0 = time.perf_counter()
+            js.launch(d_out, cap, d_cnt)
+            ctx.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        js.close()
+        return best
+
+    best = dict(sh.tuning, rows_per_group=rows)
+    try:
+        tb = timed(best)
+    except ccs.CcsError:   # the autotuner's pick does not fit at this row layout
+        best = dict(best, pair_max=16384)
+        tb = timed(best)
+    fp = sh.filter_primes
+    knobs = (("n_dense", [sum(1 for q in fp if q < c) for c in (128, 200, 256, 320, 400, 512)]),
+             ("pair_max", [0, 4096, 16384, 65536]), ("n_dense", None),
+             ("n_tierb", [256, 512, 768, 1024]), ("unit_j", [65536, 131072, 262144, 1048576]))
+    last_dense = None
+    for knob, values in knobs:
+        if values is None:            # the dense cutoff again, after the pattern period
+            knob, values = last_dense
+        if knob == "n_dense":
+            last_dense = (knob, values)
+        for v in values:
+            if v == best[knob]:
+                continue
+            t = dict(best, **{knob: v})
+            try:
+                tt = timed(t)
+            except ccs.CcsError:
+                continue
+            if tt < tb:
+                best, tb = t, tt
+    sh.set_tuning(**best)
+    ctx.dfree(d_out)
+    ctx.dfree(d_cnt)
+    return sh.tuning
+
+
 def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_candidates=1 << 38,
                    max_slice_survivors=1 << 19, reps=3):
     """Config 5: a Poisson stream of search jobs (depth U{12..20}, width U{64,80,96}, random
@@ -223,13 +277,8 @@ def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_c
     tuned = {}
     for d, sh in shapes.items():
         sample = [(r["a"], r["u0"], min(r["count"], 1 << 27)) for r in tune_pool if r["depth"] == d][:192]
-        tuned[d], _ = sh.autotune_jobs(sample, max_trials=tune_trials)
-        if tuned[d]["rows_per_group"] != 2:   # one row layout: fewer launch groups per slice
-            try:
-                sh.set_tuning(rows_per_group=2)
-                tuned[d] = sh.tuning
-            except ccs.CcsError:
-                pass
+        sh.autotune_jobs(sample, max_trials=tune_trials)
+        tuned[d] = tune_rows_fixed(ccs, ctx, sh, sample, rows=2)
     reqs = WL.served_requests(n_jobs=n_jobs)
     total = sum(r["count"] for r in reqs)
     cal = WL.served_requests(n_jobs=400, seed=0xCA1)     # warm-up jobs (not timed)
