@@ -648,24 +648,24 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                     if (s >= dq) s -= dq;
                     sd = doff * 32u + s;
                 }
-                u32 alive[RG][kRowWords];
+                // acc holds the killed candidates; in the unit's last, partial row (uniform) the
+                // candidates past its end count as killed.  A word is tested as is (!= ~0) and
+                // complemented only when read out, in scalar code: no VALU complement per word.
     #pragma unroll
                 for (int r = 0; r < RG; ++r) {
                     const u32 rowbase = (row0 + (u32)r) * (u32)kRowBits;
+                    if (rowbase + (u32)kRowBits > U.n) {
     #pragma unroll
-                    for (int k = 0; k < kRowWords; ++k) {
-                        u32 v = ~acc[r][k];
-                        if (rowbase + (u32)kRowBits > U.n) {   // the unit's last, partial row (uniform)
+                        for (int k = 0; k < kRowWords; ++k) {
                             const u32 jb = rowbase + lane * 128u + 32u * (u32)k;
-                            if (jb >= U.n) v = 0;
-                            else if (U.n - jb < 32u) v &= (1u << (U.n - jb)) - 1u;
+                            if (jb >= U.n) acc[r][k] = ~0u;
+                            else if (U.n - jb < 32u) acc[r][k] |= ~((1u << (U.n - jb)) - 1u);
                         }
-                        alive[r][k] = v;
                     }
                 }
                 // survivors of the dense stage in uniform control flow: the lanes with a
-                // non-empty word (ballot), then that word's bits (readlane); stage B takes them
-                // two at a time
+                // word not all killed (ballot), then that word's live bits (readlane);
+                // stage B takes them two at a time
                 auto emit = [&](u32 j) {
                     if (lane == q_n) {
                         q_owner = U.owner;
@@ -677,11 +677,11 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
     #pragma unroll
                 for (int rk = 0; rk < RG * kRowWords; ++rk) {
                     const int r = rk / kRowWords, k = rk % kRowWords;
-                    u64 m = ballot(alive[r][k] != 0u);
+                    u64 m = ballot(acc[r][k] != ~0u);
                     while (m) {
                         const u32 l = (u32)__builtin_ctzll(m);
                         m &= m - 1u;
-                        u32 w = __builtin_amdgcn_readlane(alive[r][k], l);
+                        u32 w = ~(u32)__builtin_amdgcn_readlane(acc[r][k], l);
                         cnt_dense += (u32)__builtin_popcount(w);
                         const u32 jb = (row0 + (u32)r) * (u32)kRowBits + l * 128u + 32u * (u32)k;
                         while (w) {
