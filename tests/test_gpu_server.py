@@ -124,6 +124,23 @@ def test_resident_state_equals_recomputed(ctx, shapes):
             assert np.array_equal(res_r[r["id"]][1][f], res_c[r["id"]][1][f]), f
 
 
+def test_launch_groups_do_not_change_results(ctx, shapes):
+    """the depth profiles in one launch per slice, in one launch per kernel instance, and
+    in the default grouping (widths within a factor of two) give the same per-job results"""
+    reqs = [r for r in WL.served_requests(n_jobs=60, seed=0x6D, lo_exp=18, hi_exp=22) if r["depth"] in shapes][:12]
+    say(f"[server] launch groups, {len(reqs)} jobs")
+    base, st = run_server(ctx, shapes, reqs, slice_candidates=1 << 23, min_chunk=1 << 16)
+    for ratio in (1, 64):
+        other, st2 = run_server(ctx, shapes, reqs, slice_candidates=1 << 23, min_chunk=1 << 16,
+                                group_width_ratio=ratio)
+        for r in reqs:
+            assert base[r["id"]][0] == other[r["id"]][0]
+            for f in ("chain_len", "back_steps", "flags", "root_lo", "root_hi"):
+                assert np.array_equal(base[r["id"]][1][f], other[r["id"]][1][f]), f
+        if ratio == 64:   # one launch group: one launch per slice
+            assert st2["launches"] == st2["slices"]
+
+
 def test_survivor_offsets_beyond_2_32(ctx, oracle, shapes):
     """a request longer than 2^32 candidates: survivors past offset 2^32 are reported at
     their full 64-bit offset, equal to the oracle's class-form filter there"""
