@@ -674,6 +674,7 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                     if (++q_n == 64u) flush();
                 };
                 u32 pend = 0xffffffffu;
+                u32 cnt_rows = 0;   // this row group's dense survivors (a 32-bit scalar sum)
     #pragma unroll
                 for (int rk = 0; rk < RG * kRowWords; ++rk) {
                     const int r = rk / kRowWords, k = rk % kRowWords;
@@ -682,7 +683,7 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                         const u32 l = (u32)__builtin_ctzll(m);
                         m &= m - 1u;
                         u32 w = ~(u32)__builtin_amdgcn_readlane(acc[r][k], l);
-                        cnt_dense += (u32)__builtin_popcount(w);
+                        cnt_rows += (u32)__builtin_popcount(w);
                         const u32 jb = (row0 + (u32)r) * (u32)kRowBits + l * 128u + 32u * (u32)k;
                         while (w) {
                             const u32 j = jb + (u32)__builtin_ctz(w);
@@ -700,6 +701,7 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                     }
                 }
                 if (pend != 0xffffffffu && stage_b_pass<NBV>(lds, t_far(), t_nb_lds(), R, pend)) emit(pend);
+                cnt_dense += cnt_rows;
             }
         }
     }
