@@ -516,8 +516,10 @@ static u32 tierb_in_lds(const ccs_shape* s, const ccs_tuning* t) {
     u32 words = 0;
     for (auto sl : dense_slots(s, t)) words += dense_ext_words(slot_period(s, sl), t->rows_per_group);
     const u32 bpc = t->blocks_per_cu ? t->blocks_per_cu : 1;
-    const u32 cap = s->max_lds_words - kLdsCtlWords - 2;   // the zero word and scheduler words follow
-    const u32 budget = std::min(cap, std::max(s->max_lds_words / bpc, std::min(s->max_lds_words, 16384u)) - kLdsCtlWords - 2);
+    // the zero word, the alignment of the control words and the control words follow
+    const u32 ctl = lds_ctl_words((int)t->rows_per_group) + 4;
+    const u32 cap = s->max_lds_words - ctl;
+    const u32 budget = std::min(cap, std::max(s->max_lds_words / bpc, std::min(s->max_lds_words, 16384u)) - ctl);
     u32 n = 0;
     while (n < t->n_tierb) {
         const u32 end = std::min(n + (u32)kTierBGroup, t->n_tierb);
@@ -1057,7 +1059,7 @@ static void fill_sieve_common(const ccs_shape* s, const Scratch& x, SieveArgs& a
     a.tab = shape_tab(s);
     a.group = nullptr;
     a.n_group = 1;
-    a.lds_ctl = (s->lds_words + 1) & ~1u;
+    a.lds_ctl = (s->lds_words + 3) & ~3u;
     a.one = 1;
     a.unit_j = s->tune.unit_j;
     a.phases = nullptr;
@@ -1089,7 +1091,7 @@ static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, 
     const int nbv = nbm ? (int)((nbm + 63) / 64) : 1;
     const u32 wpb = s->tune.waves_per_block;
     const int block = 64 * (int)wpb;
-    const size_t lds = (size_t)(sa.lds_ctl + kLdsCtlWords) * 4;
+    const size_t lds = (size_t)(sa.lds_ctl + lds_ctl_words((int)s->tune.rows_per_group)) * 4;
     u64 want_grid = (u64)c->n_cu * s->tune.blocks_per_cu;
     const u64 blocks_needed = (n_units + wpb - 1) / wpb;
     if (want_grid > blocks_needed) want_grid = blocks_needed ? blocks_needed : 1;
@@ -1700,7 +1702,7 @@ extern "C" int ccs_groupset_create(ccs_ctx* c, const ccs_shape* const* shapes, u
     js->job_shape.assign(job_shape, job_shape + n_jobs);
     double surv = 0;
     for (u32 k = 0; k < n_shapes; ++k) {
-        js->lds_ctl = std::max(js->lds_ctl, (shapes[k]->lds_words + 1) & ~1u);
+        js->lds_ctl = std::max(js->lds_ctl, (shapes[k]->lds_words + 3) & ~3u);
         js->nb_max = std::max(js->nb_max, shapes[k]->nb);
     }
     group_units_per_job(c, js, jobs, n_jobs, jr.data());

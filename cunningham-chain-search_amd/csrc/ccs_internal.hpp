@@ -25,7 +25,22 @@ constexpr int kTierBGroup = 64;     // stage-B primes tested per wave step (one 
 constexpr int kMaxTierB = 1024;     // stage-B primes: at most 16 groups of 64
 constexpr unsigned kTierBMaxQ = 16384;   // stage-B primes are below this (14 bits)
 constexpr unsigned kTierBOffBits = 18;   // bitset word offsets of stage-B primes (< 2^18)
-constexpr unsigned kLdsCtlWords = 8;   // sieve scheduler words after the LDS tables
+// After the LDS tables (at a 16-byte boundary): the sieve's scheduler words, then, in the
+// instances that queue their dense survivors (one and two rows per lane step: the shallow,
+// survivor-rich shapes), one candidate queue per wave.  A queue is tested once it holds
+// kQueueTest candidates and a word row adds at most 64 at a time, so it holds at most
+// kQueueTest + 63; then up to three padding entries and a discard slot.  The four-row
+// instances (the deep, VALU-bound shapes) take their survivors one at a time in scalar code.
+constexpr unsigned kSchedWords = 8;
+constexpr unsigned kQueueTest = 16;
+constexpr unsigned kQueueDiscard = kQueueTest + 63 + 3;
+constexpr unsigned kQueueWords = 84;    // > kQueueDiscard, a multiple of four
+constexpr unsigned kMaxWaves = 16;      // waves per block
+static_assert(kQueueDiscard < kQueueWords && kQueueWords % 4 == 0 && kSchedWords % 4 == 0, "queue layout");
+__host__ __device__ constexpr bool survivor_queue(int rows_per_group) { return rows_per_group < 4; }
+__host__ __device__ constexpr unsigned lds_ctl_words(int rows_per_group) {
+    return kSchedWords + (survivor_queue(rows_per_group) ? kMaxWaves * kQueueWords : 0u);
+}
 // Launch counters of a scratch (16 dwords): words [0, kLaunchWords) are cleared before every
 // filter launch ([0] stage-B count, [2..3] unit cursor); words from kStatusWord on are the
 // sticky overflow status, cleared only when the caller reads it: [+0] launches whose stage-B
@@ -164,7 +179,7 @@ struct SieveArgs {
     ShapeTab tab;             // the tables of a one-shape launch
     const ShapeTab* group;    // [n_group] tables of a group launch (null: one shape)
     u32 n_group;
-    u32 lds_ctl;              // word offset of the scheduler words (kLdsCtlWords) in LDS
+    u32 lds_ctl;              // word offset of the scheduler words (lds_ctl_words) in LDS
     const u64* shape_units;   // [n_group + 1] first unit of each shape's jobs (group launch)
     u32 unit_j;               // candidates per work unit (job launches: 0 = each job's own)
     u32 one;                  // 1 (see the dense loop)

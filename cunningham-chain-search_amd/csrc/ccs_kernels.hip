@@ -275,6 +275,28 @@ __device__ __forceinline__ bool stage_b_pass(const u32* __restrict__ lds, const 
 
 typedef u32 u32x4 __attribute__((ext_vector_type(4)));   // one 16-byte LDS block
 
+// Stage-B group 0 for the n queued candidates at q (a wave's LDS queue, 16-byte aligned; n a
+// multiple of four, at most 64): the probes run back to back, four candidates per 16-byte
+// read of the queue (all lanes read the same block), with no branch per candidate.  Returns
+// the mask of the candidates that no group-0 prime kills.
+template <int NBV>
+__device__ __forceinline__ u64 stage_b_group0(const u32* __restrict__ lds, const TierBRegs<NBV>& R, const u32* q,
+                                              u32 n) {
+    u64 pass = 0;
+#pragma unroll 1
+    for (u32 t = 0; t < n; t += 4) {
+        const u32x4 j = *(const u32x4*)(q + t);
+        const bool b0 = stage_b_bit<NBV>(lds, R, 0, j.x);
+        const bool b1 = stage_b_bit<NBV>(lds, R, 0, j.y);
+        const bool b2 = stage_b_bit<NBV>(lds, R, 0, j.z);
+        const bool b3 = stage_b_bit<NBV>(lds, R, 0, j.w);
+        const u64 live = (u64)(ballot(b0) == 0) | (u64)(ballot(b1) == 0) << 1 | (u64)(ballot(b2) == 0) << 2 |
+                         (u64)(ballot(b3) == 0) << 3;
+        pass |= live << t;
+    }
+    return pass;
+}
+
 // Marks the four words of a 16-byte LDS block as used together at this point (an empty asm
 // that reads and writes them: no instruction).  Without it the compiler may split a block
 // read whose words are not all used into 32-bit reads, which run at the narrow reads' LDS
@@ -369,10 +391,11 @@ __device__ __forceinline__ void dense_offset(const u32* lds, u32 lane, u32 sd, u
 // launches without them keep their register allocation)
 template <int RG, int NBV, bool GROUP, bool RESIDENT>
 __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
+    constexpr bool kQueue = survivor_queue(RG);
     extern __shared__ uint4 lds_blocks[];   // 16-byte aligned: the dense patterns are read as b128
     u32* const lds = (u32*)lds_blocks;
     const u32 tid = threadIdx.x;
-    u32* const ctl = lds + A.lds_ctl;   // scheduler words past the tables (8-byte aligned)
+    u32* const ctl = lds + A.lds_ctl;   // scheduler words and queues past the tables (16-byte aligned)
     const u32 lane = lane_id();
 
     // the tables of the shape whose patterns are in LDS (a one-shape launch reads them from
@@ -460,6 +483,7 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
     // block-uniform values (the ranges, the claims read back from LDS, the shape bounds),
     // so every wave passes the same barriers, and all leave in the same round.
     const u32 wpb = blockDim.x >> 6, wib = tid >> 6;
+    u32* const cqb = ctl + kSchedWords + wib * kQueueWords;   // this wave's candidate queue
     const u32 n_blocks = gridDim.x;
     const u32 n_units = (u32)A.n_units;   // < 2^32: the launchers reject larger launches
     u32 n_pool = n_units / 16;
@@ -622,6 +646,38 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
             prev_j0 = U.j0;
             prev_n = U.n;
 
+            auto emit = [&](u32 j) {
+                if (lane == q_n) {
+                    q_owner = U.owner;
+                    q_j = U.j0 + j;
+                }
+                if (++q_n == 64u) flush();
+            };
+            // The unit's dense survivors wait in the wave's candidate queue (LDS, offsets in
+            // the unit) and meet stage B kQueueTest or more at a time: group 0 for all of them
+            // back to back, the later groups for the few that pass it.  The queue is emptied
+            // at the end of the unit, whose stage-B phases it was tested against.
+            u32 cq_n = 0;
+            auto test_queue = [&]() {
+                const u32 n4 = (cq_n + 3u) & ~3u;
+                if (lane < n4 - cq_n) cqb[cq_n + lane] = 0u;   // padding probes read candidate 0
+                wave_sync();                                    // the queue writes before its reads
+                for (u32 base = 0; base < n4; base += 64u) {
+                    const u32 n = n4 - base < 64u ? n4 - base : 64u;
+                    u64 pass = stage_b_group0<NBV>(lds, R, cqb + base, n);
+                    const u32 real = cq_n - base;
+                    if (real < 64u) pass &= (1ull << real) - 1u;
+                    while (pass) {
+                        const u32 t = (u32)__builtin_ctzll(pass);
+                        pass &= pass - 1u;
+                        const u32 j = cqb[base + t];
+                        if (stage_b_rest<NBV>(lds, t_far(), t_nb_lds(), R, j)) emit(j);
+                    }
+                }
+                wave_sync();   // the reads before the next writes
+                cq_n = 0;
+            };
+
             const u32 rows = (U.n + kRowBits - 1) / kRowBits;
             for (u32 row0 = 0; row0 < rows; row0 += RG) {
                 // lane l owns candidates row*8192 + 128*l + 32*k + b (k = 0..3, b = 0..31): per
@@ -649,8 +705,8 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                     sd = doff * 32u + s;
                 }
                 // acc holds the killed candidates; in the unit's last, partial row (uniform) the
-                // candidates past its end count as killed.  A word is tested as is (!= ~0) and
-                // complemented only when read out, in scalar code: no VALU complement per word.
+                // candidates past its end count as killed.  A word row is tested as is (!= ~0)
+                // and complemented only when some lane has a survivor in it.
     #pragma unroll
                 for (int r = 0; r < RG; ++r) {
                     const u32 rowbase = (row0 + (u32)r) * (u32)kRowBits;
@@ -663,46 +719,65 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                         }
                     }
                 }
-                // survivors of the dense stage in uniform control flow: the lanes with a
-                // word not all killed (ballot), then that word's live bits (readlane);
-                // stage B takes them two at a time
-                auto emit = [&](u32 j) {
-                    if (lane == q_n) {
-                        q_owner = U.owner;
-                        q_j = U.j0 + j;
-                    }
-                    if (++q_n == 64u) flush();
-                };
-                u32 pend = 0xffffffffu;
                 u32 cnt_rows = 0;   // this row group's dense survivors (a 32-bit scalar sum)
-    #pragma unroll
-                for (int rk = 0; rk < RG * kRowWords; ++rk) {
-                    const int r = rk / kRowWords, k = rk % kRowWords;
-                    u64 m = ballot(acc[r][k] != ~0u);
-                    while (m) {
-                        const u32 l = (u32)__builtin_ctzll(m);
-                        m &= m - 1u;
-                        u32 w = ~(u32)__builtin_amdgcn_readlane(acc[r][k], l);
-                        cnt_rows += (u32)__builtin_popcount(w);
-                        const u32 jb = (row0 + (u32)r) * (u32)kRowBits + l * 128u + 32u * (u32)k;
-                        while (w) {
-                            const u32 j = jb + (u32)__builtin_ctz(w);
-                            w &= w - 1u;
-                            if (pend == 0xffffffffu) {
-                                pend = j;
-                                continue;
+                if constexpr (kQueue) {
+                    // survivors of the dense stage, per word row (32 candidates a lane): every
+                    // lane with a live bit writes its lowest one to the queue at its rank among
+                    // those lanes (the others to the discard slot), until no lane has one left
+        #pragma unroll
+                    for (int rk = 0; rk < RG * kRowWords; ++rk) {
+                        const int r = rk / kRowWords, k = rk % kRowWords;
+                        u64 m = ballot(acc[r][k] != ~0u);
+                        if (!m) continue;
+                        u32 live = ~acc[r][k];
+                        const u32 jb = lane * 128u + ((row0 + (u32)r) * (u32)kRowBits + 32u * (u32)k);
+                        do {
+                            const u32 rank = __builtin_amdgcn_mbcnt_hi((u32)(m >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo((u32)m, cq_n));
+                            cqb[live != 0u ? rank : kQueueDiscard] = jb + (u32)__builtin_ctzg(live, 0);
+                            const u32 c = (u32)__builtin_popcountll(m);
+                            cq_n += c;
+                            cnt_rows += c;
+                            if (cq_n >= kQueueTest) test_queue();
+                            live &= live - 1u;
+                            m = ballot(live != 0u);
+                        } while (m);
+                    }
+                } else {
+                    // survivors of the dense stage in uniform control flow: the lanes with a
+                    // word not all killed (ballot), then that word's live bits (readlane);
+                    // stage B takes them two at a time
+                    u32 pend = 0xffffffffu;
+        #pragma unroll
+                    for (int rk = 0; rk < RG * kRowWords; ++rk) {
+                        const int r = rk / kRowWords, k = rk % kRowWords;
+                        u64 m = ballot(acc[r][k] != ~0u);
+                        while (m) {
+                            const u32 l = (u32)__builtin_ctzll(m);
+                            m &= m - 1u;
+                            u32 w = ~(u32)__builtin_amdgcn_readlane(acc[r][k], l);
+                            cnt_rows += (u32)__builtin_popcount(w);
+                            const u32 jb = (row0 + (u32)r) * (u32)kRowBits + l * 128u + 32u * (u32)k;
+                            while (w) {
+                                const u32 j = jb + (u32)__builtin_ctz(w);
+                                w &= w - 1u;
+                                if (pend == 0xffffffffu) {
+                                    pend = j;
+                                    continue;
+                                }
+                                bool pa, pb;
+                                stage_b_pair<NBV>(lds, t_far(), t_nb_lds(), R, pend, j, pa, pb);
+                                if (pa) emit(pend);
+                                if (pb) emit(j);
+                                pend = 0xffffffffu;
                             }
-                            bool pa, pb;
-                            stage_b_pair<NBV>(lds, t_far(), t_nb_lds(), R, pend, j, pa, pb);
-                            if (pa) emit(pend);
-                            if (pb) emit(j);
-                            pend = 0xffffffffu;
                         }
                     }
+                    if (pend != 0xffffffffu && stage_b_pass<NBV>(lds, t_far(), t_nb_lds(), R, pend)) emit(pend);
                 }
-                if (pend != 0xffffffffu && stage_b_pass<NBV>(lds, t_far(), t_nb_lds(), R, pend)) emit(pend);
                 cnt_dense += cnt_rows;
             }
+            if (kQueue && cq_n) test_queue();
         }
     }
     flush();
