@@ -86,6 +86,10 @@ class Context:
         """test knob: launches use at most `entries` of their stage-B buffer (0 = all)"""
         L.check(self.lib.ccs_debug_limit_stage_b(self.h, entries), "ccs_debug_limit_stage_b")
 
+    def deep_batch(self, per_wave):
+        """test knob: deep-stage records per wave batch (1: one at a time, 0: by count)"""
+        L.check(self.lib.ccs_debug_deep_batch(self.h, per_wave), "ccs_debug_deep_batch")
+
     def profile_start(self):
         L.check(self.lib.ccs_profile_start(self.h), "ccs_profile_start")
 

@@ -40,6 +40,7 @@ EXPORTS = [
     "ccs_pool_parse", "ccs_pool_load", "ccs_dedup_u128", "ccs_family_top_gate_launch",
     "ccs_groupset_create", "ccs_groupset_update", "ccs_stream_status", "ccs_jobset_status",
     "ccs_debug_limit_stage_b", "ccs_chain_roots_launch", "ccs_filter_launch_noalloc",
+    "ccs_debug_deep_batch",
 ]
 
 
@@ -275,6 +276,7 @@ def load():
     L.ccs_chain_roots_launch.argtypes = [vp, vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp, vp, vp]
     L.ccs_filter_launch_noalloc.argtypes = [vp, vp, U128, C.c_uint32, vp, C.c_uint32, vp, vp]
     L.ccs_debug_limit_stage_b.argtypes = [vp, C.c_uint32]
+    L.ccs_debug_deep_batch.argtypes = [vp, C.c_uint32]
     L.ccs_prove_launch.argtypes = [vp, vp, U128, vp, vp, C.c_uint32, C.POINTER(ProveParams), vp, vp]
     L.ccs_prove.argtypes = [vp, vp, U128, vp, C.c_uint32, C.POINTER(ProveParams), vp]
     L.ccs_confirm_values.argtypes = [vp, C.c_uint32, C.POINTER(ProveParams), C.c_uint32, vp,

@@ -87,6 +87,7 @@ struct ccs_ctx {
     size_t ev_used = 0;
     std::vector<std::pair<int, size_t>> marks;   // (kind, index of the begin event)
     u32 stage_b_limit = 0;              // test knob: stage-B buffer entries used (0 = all)
+    u32 deep_per_wave = 0;              // test knob: deep-stage records per wave (0 = by count)
     unsigned char* noalloc = nullptr;   // this device's copy of the no-allocation scratch
 };
 
@@ -684,8 +685,8 @@ static int upload_tables(ccs_shape* s) {
     if (far_bits.size() > (1u << kTierBOffBits) || image.size() > (1u << kTierBOffBits))
         return fail(CCS_ERR_INVALID, "stage-B bitsets exceed 2^18 words: fewer stage-B primes");
     std::vector<DeepPrime> deep(nc ? nc : 1);
-    // nc hashes (one empty one when nc = 0), then the nc masks
-    const size_t hash_words = (size_t)(nc ? nc : 1) * kMaxHashSlots;
+    // nc bucket tables (one empty one when nc = 0), then the nc masks
+    const size_t hash_words = (size_t)(nc ? nc : 1) * kDeepTableWords;
     std::vector<u32> hash(hash_words + (size_t)nc * 4, 0xffffffffu);
     std::fill(hash.begin() + hash_words, hash.end(), 0u);
     for (u32 i = 0; i < nc; ++i) {
@@ -696,14 +697,30 @@ static int upload_tables(ccs_shape* s) {
         d.c32 = pm.c32;
         d.c64 = pm.c64;
         d.c96 = pm.c96;
-        deep[i] = d;
-        u32* mask = &hash[(size_t)nc * kMaxHashSlots + (size_t)i * 4];
-        for (u32 f : s->forb[idx]) {
-            u32 slot = f & (kMaxHashSlots - 1);
-            while (hash[(size_t)i * kMaxHashSlots + slot] != 0xffffffffu) slot = (slot + 1) & (kMaxHashSlots - 1);
-            hash[(size_t)i * kMaxHashSlots + slot] = f;
+        d.rq = 1.0 / (double)pm.q;
+        d._pad = 0;
+        // the first odd multiplier of a fixed sequence that leaves no bucket over-full
+        const std::vector<u32>& F = s->forb[idx];
+        if (F.size() > (size_t)kDeepTableWords / 2) return fail(CCS_ERR_INVALID, "too many forbidden residues");
+        d.mult = 0;
+        for (u32 k = 0; k < 65536 && !d.mult; ++k) {
+            const u32 m = (k * 0x9E3779B9u + 0x7F4A7C15u) | 1u;
+            u32 fill[kDeepBuckets] = {};
+            bool ok = true;
+            for (u32 f : F) ok = ok && ++fill[(f * m) >> 28] <= (u32)kDeepBucketSlots;
+            if (ok) d.mult = m;
+        }
+        if (!d.mult) return fail(CCS_ERR_INVALID, "no bucket multiplier for a deep prime");
+        u32* tab = &hash[(size_t)i * kDeepTableWords];
+        u32* mask = &hash[(size_t)nc * kDeepTableWords + (size_t)i * 4];
+        for (u32 f : F) {
+            u32* b = tab + ((f * d.mult) >> 28) * kDeepBucketSlots;
+            u32 k = 0;
+            while (b[k] != 0xffffffffu) ++k;   // < kDeepBucketSlots: the multiplier fits them
+            b[k] = f;
             mask[(f >> 5) & 3] |= 1u << (f & 31);
         }
+        deep[i] = d;
     }
 
     // expected survival after the GPU's stage B (sizes the stage-B survivor buffer)
@@ -1111,6 +1128,7 @@ static int run_filter_kernels(ccs_ctx* c, const ccs_shape* s, const Scratch& x, 
     da.out_count = d_count;
     da.stats = x.stats;
     da.status = x.counters + kStatusWord;
+    da.per_wave = c->deep_per_wave;
     prof_begin(c, 1, st);
     e = launch_deep(da, x.mid_cap, st);
     c->profiling = prof;
@@ -1280,6 +1298,13 @@ extern "C" int ccs_stream_status(ccs_ctx* c, void* stream, ccs_launch_status* ou
 extern "C" int ccs_debug_limit_stage_b(ccs_ctx* c, uint32_t entries) {
     if (!c) return fail(CCS_ERR_INVALID, "null context");
     c->stage_b_limit = entries;
+    return CCS_OK;
+}
+
+extern "C" int ccs_debug_deep_batch(ccs_ctx* c, uint32_t per_wave) {
+    if (!c) return fail(CCS_ERR_INVALID, "null context");
+    if (per_wave > kDeepBatch) return fail(CCS_ERR_INVALID, "deep batch above its maximum");
+    c->deep_per_wave = per_wave;
     return CCS_OK;
 }
 

@@ -20,7 +20,10 @@ typedef uint8_t u8;
 constexpr int kRowBits = 8192;      // one row = 64 lanes x 128 candidates
 constexpr int kRowWords = 4;        // kill words (32 candidates each) per lane and row
 constexpr int kMaxDense = 64;       // dense primes held one per lane
-constexpr int kMaxHashSlots = 64;   // forbidden-residue hash per deep prime
+constexpr int kDeepBuckets = 16;    // forbidden-residue table of a deep prime: 16 buckets
+constexpr int kDeepBucketSlots = 4; // of four residues (one 16-byte load)
+constexpr int kDeepTableWords = kDeepBuckets * kDeepBucketSlots;
+static_assert(kDeepBuckets == 16, "the bucket of r is the top four bits of r * mult");
 constexpr int kTierBGroup = 64;     // stage-B primes tested per wave step (one per lane)
 constexpr int kMaxTierB = 1024;     // stage-B primes: at most 16 groups of 64
 constexpr unsigned kTierBMaxQ = 16384;   // stage-B primes are below this (14 bits)
@@ -87,20 +90,27 @@ struct TierBPrime {
     u32 pad;
 };
 
-// Deep stage (K1c): forbidden residues of n mod q in a 64-slot open-addressing hash,
-// key r, slot (r & 63), linear probing, empty slot = 0xFFFFFFFF.
-// Deep-stage prime: 16 bytes, one dwordx4 load per lane and step; 1/q is rebuilt in the
-// kernel and prime i's hash is at i * kMaxHashSlots.  After the nc hashes come nc 128-bit
-// masks (4 words each): bit (r & 127) is set for every forbidden residue r, so a clear
-// bit proves r allowed without the hash probe (a scattered L2 line per lane).  Shapes
+// Deep stage (K1c): the forbidden residues of n mod q of deep prime i in a table of
+// kDeepBuckets buckets of kDeepBucketSlots residues at i * kDeepTableWords: residue r is in
+// bucket (r * mult) >> 28 (32-bit product), unused slots hold 0xFFFFFFFF, and the host picks
+// the prime's odd multiplier so that no bucket overflows (a prime has at most 32 forbidden
+// residues), so one 16-byte load and four compares decide r.  After the nc tables come nc
+// 128-bit masks (4 words each): bit (r & 127) is set for every forbidden residue r, so a
+// clear bit proves r allowed without the table load (a scattered L2 line per lane).  Shapes
 // whose stage-B survivors are many (expected survival above kDeepMaskSurvival) read the
-// masks and probe only where the bit is set; the others probe directly, as the extra
-// load costs more than it saves when the survivors die within a few steps.
+// masks and load the table only where the bit is set; the others load it directly, as the
+// extra load costs more than it saves when the survivors die within a few steps.
 struct DeepPrime {
     u32 q;
     u32 c32, c64, c96;   // 2^(32k) mod q
+    double rq;           // 1/q, correctly rounded
+    u32 mult;            // bucket multiplier of the forbidden-residue table
+    u32 _pad;
 };
 constexpr double kDeepMaskSurvival = 1e-6;
+constexpr unsigned kDeepBatch = 32;   // deep-stage records per wave batch (survivor-rich launches)
+constexpr unsigned kDeepBatchMin = 12;       // smaller batches run slower than one record per wave
+constexpr unsigned kDeepFillWaves = 8192;    // resident waves of the deep kernel (256 CUs x 32)
 
 struct ClassRec {   // plain lattice class (base_idx, k, rho) and its residue a mod W
     u64 a;
@@ -226,6 +236,7 @@ struct DeepArgs {
     u32* out_count;
     unsigned long long* stats; // [2] += stage-B survivors of this launch (may be null)
     u32* status;               // sticky overflow status (kStatusWord layout, may be null)
+    u32 per_wave;              // records per wave batch (0: from the launch's count)
 };
 
 // Prove: GMP confirmation engine semantics (v13.cu:1794-1924) per survivor.

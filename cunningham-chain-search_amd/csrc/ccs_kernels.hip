@@ -830,28 +830,54 @@ __device__ __forceinline__ Limbs limbs_of(U2 n) {
     return L;
 }
 
-// n mod q for a deep prime (q < 2^18), all in f64: the limb fold y = l0 + l1*c32 + l2*c64 +
-// l3*c96 (c_k = 2^(32k) mod q) is an integer below 2^52, so every FMA of it is exact;
-// 1/q is rebuilt from the f32 estimate by two Newton steps (relative error ~2^-52), so the
-// truncated quotient is floor(y/q) or one off, the FMA remainder y - qt*q is exact, and it
-// lies in [-q, 2q) before the two corrections (checked on the host for every prime below
-// 2^18, f32 estimates up to 4 ulp off, and y at the ends of its range and next to multiples
-// of q).
-__device__ __forceinline__ u32 deep_mod(const Limbs& L, const DeepPrime& d) {
-    const double q = (double)d.q;
-    const double y = __fma_rn(L.l3, (double)d.c96, __fma_rn(L.l2, (double)d.c64, __fma_rn(L.l1, (double)d.c32, L.l0)));
-    double r = (double)__builtin_amdgcn_rcpf((float)d.q);
-    r = __fma_rn(r, __fma_rn(-q, r, 1.0), r);
-    r = __fma_rn(r, __fma_rn(-q, r, 1.0), r);
-    const double qt = __builtin_trunc(y * r);
-    int rem = (int)__fma_rn(-qt, q, y);
-    rem += (rem >> 31) & (int)d.q;
-    const u32 ur = (u32)rem;
-    return ur >= d.q ? ur - d.q : ur;
+// A deep prime's constants as doubles: q, the limb factors c_k = 2^(32k) mod q, 1/q
+// (correctly rounded, from the table), and its bucket multiplier.
+struct DeepConst {
+    double q, rq, c32, c64, c96;
+    u32 qi, mult;
+};
+
+__device__ __forceinline__ DeepConst deep_const(const DeepPrime& d) {
+    DeepConst C;
+    C.qi = d.q;
+    C.mult = d.mult;
+    C.q = (double)d.q;
+    C.c32 = (double)d.c32;
+    C.c64 = (double)d.c64;
+    C.c96 = (double)d.c96;
+    C.rq = d.rq;
+    return C;
 }
 
-// one wave per stage-B survivor; the 64 lanes test 64 deep primes at a time, so the rare
-// survivors are not serialised over thousands of dependent per-prime steps
+// n mod q for a deep prime (q < 2^18), all in f64: the limb fold y = l0 + l1*c32 + l2*c64 +
+// l3*c96 is an integer below 2^52, so every FMA of it is exact; with 1/q correctly rounded
+// the truncated quotient is floor(y/q) or one off, the FMA remainder y - qt*q is exact, and
+// it lies in [-q, 2q) before the two corrections (checked in exact arithmetic for primes
+// below 2^18 and y at the ends of its range and next to multiples of q).
+__device__ __forceinline__ u32 deep_mod(double l0, double l1, double l2, double l3, const DeepConst& C) {
+    const double y = __fma_rn(l3, C.c96, __fma_rn(l2, C.c64, __fma_rn(l1, C.c32, l0)));
+    const double qt = __builtin_trunc(y * C.rq);
+    int rem = (int)__fma_rn(-qt, C.q, y);
+    rem += (rem >> 31) & (int)C.qi;
+    const u32 ur = (u32)rem;
+    return ur >= C.qi ? ur - C.qi : ur;
+}
+
+// bit r mod 128 of a deep prime's 128-bit residue mask (masked tables: only the residues
+// whose bit is set can be forbidden)
+__device__ __forceinline__ bool mask_bit(const uint4& m, u32 r) {
+    const u32 w = (r >> 5) & 3;
+    const u32 mw = w == 0 ? m.x : w == 1 ? m.y : w == 2 ? m.z : m.w;
+    return (mw >> (r & 31)) & 1u;
+}
+
+// whether r is one of a deep prime's forbidden residues: its bucket of the prime's table
+// (one 16-byte load), four compares
+__device__ __forceinline__ bool deep_hit(const u32* __restrict__ tab, u32 r, u32 mult) {
+    const uint4 v = ((const uint4*)tab)[(r * mult) >> 28];
+    return (v.x == r) | (v.y == r) | (v.z == r) | (v.w == r);
+}
+
 // a stage-B buffer that overflowed: the sieve dropped the records past its capacity, so the
 // launch's survivors are incomplete; the sticky status tells the caller
 __device__ __forceinline__ void note_stage_b(const DeepArgs& A, u32 all) {
@@ -864,6 +890,9 @@ __device__ __forceinline__ void note_stage_b(const DeepArgs& A, u32 all) {
     }
 }
 
+// The deep stage: every stage-B record against the shape's remaining filter primes, the 64
+// lanes of a wave testing 64 primes at a time (lane p: prime p0 + p, its residue of n by the
+// f64 limb fold, then its forbidden-residue mask and bucket table).
 __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
     const u32 lane = lane_id();
     const u32 all = *A.mid_count;
@@ -897,14 +926,13 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
         }
         q_n = 0;
     };
-    for (u32 e = wid; e < total; e += waves) {
+    // a record's n (as four f64 limbs) and its shape's deep tables
+    auto load_rec = [&](u32 e, u32& owner, u32& j, u32& shape, Limbs& L) {
         const MidRec rec = A.mid[e];
-        const u32 owner = rec.owner, j = rec.j;
+        owner = rec.owner;
+        j = rec.j;
+        shape = 0;
         u64 ulo, uhi, a, a_hi = 0, W = A.W, W_hi = A.W_hi;
-        const DeepPrime* deep = A.deep;
-        const u32* hash = A.hash;
-        u32 nc = A.nc;
-        u32 masked = A.masked;
         if (A.mode == 0) {
             const ClassRec cr = A.plain.cls[owner];
             a = cr.a;
@@ -918,55 +946,145 @@ __global__ __launch_bounds__(256) void deep_kernel(DeepArgs A) {
             uhi = jr.u0_hi;
             W = jr.W;
             W_hi = jr.W_hi;
-            if (A.group) {
-                const DeepTab t = A.group[jr.shape];
-                deep = t.deep;
-                hash = t.hash;
-                nc = t.nc;
-                masked = t.masked;
-            }
+            shape = jr.shape;
         }
         add128(ulo, uhi, j);
-        const Limbs n = limbs_of(cand_n(W, W_hi, a, a_hi, ulo, uhi));
-        bool dead = false;
-        for (u32 p0 = 0; p0 < nc; p0 += 64) {
-            const u32 p = p0 + lane;
-            bool kill = false;
-            if (p < nc) {
-                const DeepPrime dp = deep[p];
-                uint4 m = make_uint4(0u, 0u, 0u, 0u);
-                if (masked) m = ((const uint4*)(hash + (size_t)nc * kMaxHashSlots))[p];   // wave-uniform
-                const u32 r = deep_mod(n, dp);
-                bool probe = true;
-                if (masked) {
-                    const u32 w = (r >> 5) & 3;
-                    const u32 mw = w == 0 ? m.x : w == 1 ? m.y : w == 2 ? m.z : m.w;
-                    probe = (mw >> (r & 31)) & 1u;
+        L = limbs_of(cand_n(W, W_hi, a, a_hi, ulo, uhi));
+    };
+    auto table = [&](u32 shape) -> DeepTab {
+        if (A.group) return A.group[shape];
+        DeepTab T;
+        T.deep = A.deep;
+        T.hash = A.hash;
+        T.nc = A.nc;
+        T.masked = A.masked;
+        return T;
+    };
+
+    // records per wave batch: as many as keep kDeepFillWaves waves busy, up to kDeepBatch; a
+    // launch with fewer than kDeepBatchMin per wave runs one record per wave at a time
+    u32 per = A.per_wave;
+    if (!per) {
+        per = total / kDeepFillWaves;
+        if (per < kDeepBatchMin) per = 1;
+    }
+    if (per > kDeepBatch) per = kDeepBatch;
+    if (per < 2) {
+        // one record per wave at a time (the deep shapes' few records spread over the chip)
+        for (u32 e = wid; e < total; e += waves) {
+            u32 owner, j, shape;
+            Limbs n;
+            load_rec(e, owner, j, shape, n);
+            const DeepTab T = table(shape);
+            bool dead = false;
+            for (u32 p0 = 0; p0 < T.nc; p0 += 64) {
+                const u32 p = p0 + lane;
+                bool kill = false;
+                if (p < T.nc) {
+                    const DeepPrime dp = T.deep[p];
+                    uint4 m = make_uint4(0u, 0u, 0u, 0u);
+                    if (T.masked) m = ((const uint4*)(T.hash + (size_t)T.nc * kDeepTableWords))[p];
+                    const DeepConst C = deep_const(dp);
+                    const u32 r = deep_mod(n.l0, n.l1, n.l2, n.l3, C);
+                    if (!T.masked || mask_bit(m, r)) kill = deep_hit(T.hash + (size_t)p * kDeepTableWords, r, C.mult);
                 }
-                if (probe) {
-                    const u32* h = hash + (size_t)p * kMaxHashSlots;
-                    u32 slot = r & (kMaxHashSlots - 1);
-                    for (int probe = 0; probe < kMaxHashSlots; ++probe) {
-                        const u32 v = h[slot];
-                        if (v == r) { kill = true; break; }
-                        if (v == 0xffffffffu) break;
-                        slot = (slot + 1) & (kMaxHashSlots - 1);
-                    }
+                if (ballot(kill)) {
+                    dead = true;
+                    break;
                 }
             }
-            if (ballot(kill)) { dead = true; break; }
+            if (dead) continue;
+            // survivors wait in registers (lane k holds the k-th) and go out 64 per atomic
+            if (lane == q_n) {
+                q_owner = owner;
+                q_j = j;
+            }
+            if (++q_n == 64u) flush();
         }
-        if (dead) continue;
-        // survivors wait in registers (lane k holds the k-th) and go out 64 per atomic:
-        // shallow profiles keep 10^5..10^6 survivors per launch, and one atomic each on the
-        // shared count would serialise at ~12 ns apiece
-        if (lane == q_n) {
-            q_owner = owner;
-            q_j = j;
-        }
-        if (++q_n == 64u) flush();
+        flush();
+        return;
     }
-    flush();
+
+    // Batches (the shallow shapes, 10^5..10^7 records per launch): a
+    // wave takes a batch of up to kDeepBatch consecutive records (lane k computes record k's
+    // limbs and leaves them in the wave's LDS rows).  Per group of 64 deep primes (lane p:
+    // prime p0 + p) the constants, the 1/q rebuild and the masks are done once for the whole
+    // batch, and each record still alive is tested against them, two at a time (their limbs
+    // read back by every lane, both table loads in flight together), one ballot each.
+    // A batch of a group launch may hold records of several shapes; they run shape by shape.
+    // The records left alive are final survivors, written with one atomic per wave.
+    __shared__ double2 rows[4][kDeepBatch][2];   // per wave of the 256-thread block: limbs of its records
+    double2(*const my)[2] = rows[threadIdx.x >> 6];
+    for (u32 e0 = wid * per; e0 < total; e0 += waves * per) {
+        const bool have = lane < per && e0 + lane < total;
+        u32 owner = 0, j = 0, shape = 0;
+        if (have) {
+            Limbs L;
+            load_rec(e0 + lane, owner, j, shape, L);
+            my[lane][0] = make_double2(L.l0, L.l1);
+            my[lane][1] = make_double2(L.l2, L.l3);
+        }
+        wave_sync();   // the rows are written before any lane reads them
+        u64 pending = ballot(have), alive = 0;
+        while (pending) {
+            const u32 sh = (u32)__builtin_amdgcn_readlane((int)shape, (int)__builtin_ctzll(pending));
+            const u64 batch = A.group ? pending & ballot(shape == sh) : pending;
+            const DeepTab T = table(sh);
+            pending &= ~batch;
+            u64 live = batch;
+            for (u32 p0 = 0; p0 < T.nc && live; p0 += 64) {
+                const u32 p = p0 + lane;
+                const bool mine = p < T.nc;
+                DeepPrime dp = {1u, 0u, 0u, 0u, 1.0, 1u, 0u};
+                uint4 m = make_uint4(0u, 0u, 0u, 0u);
+                if (mine) {
+                    dp = T.deep[p];
+                    if (T.masked) m = ((const uint4*)(T.hash + (size_t)T.nc * kDeepTableWords))[p];
+                }
+                const DeepConst C = deep_const(dp);
+                const u32* h = T.hash + (size_t)(mine ? p : 0u) * kDeepTableWords;
+                for (u64 rem = live; rem;) {
+                    const u32 k0 = (u32)__builtin_ctzll(rem);
+                    rem &= rem - 1u;
+                    const u32 k1 = rem ? (u32)__builtin_ctzll(rem) : k0;   // (a lone record twice)
+                    rem &= rem - 1u;
+                    const double2 a0 = my[k0][0], a1 = my[k0][1], b0 = my[k1][0], b1 = my[k1][1];
+                    const u32 ra = deep_mod(a0.x, a0.y, a1.x, a1.y, C);
+                    const u32 rb = deep_mod(b0.x, b0.y, b1.x, b1.y, C);
+                    const bool pa = mine && (!T.masked || mask_bit(m, ra));
+                    const bool pb = mine && (!T.masked || mask_bit(m, rb));
+                    bool ka = false, kb = false;
+                    if (pa) ka = deep_hit(h, ra, C.mult);
+                    if (pb) kb = deep_hit(h, rb, C.mult);
+                    if (ballot(ka)) live &= ~(1ull << k0);
+                    if (ballot(kb)) live &= ~(1ull << k1);
+                }
+            }
+            alive |= live;
+        }
+        wave_sync();   // every lane is past its row reads before the next batch writes them
+        if (!alive) continue;
+        u32 base = 0;
+        if (lane == 0) base = atomicAdd(A.out_count, (u32)__builtin_popcountll(alive));
+        base = bcast32(base);
+        const u32 slot = base + lanes_below(alive);
+        if (((alive >> lane) & 1ull) && slot < A.out_cap) {
+            if (A.mode == 0) {
+                const ClassRec cr = A.plain.cls[owner];
+                ccs_survivor sv;
+                sv.base_idx = cr.base_idx;
+                sv.k_offset = cr.k;
+                sv.tile_idx = 6u * j + cr.rho - A.plain.off;
+                sv._pad = 0;
+                ((ccs_survivor*)A.out)[slot] = sv;
+            } else {
+                ccs_job_survivor sv;
+                sv.job_idx = owner;
+                sv.u_off = j;
+                ((ccs_job_survivor*)A.out)[slot] = sv;
+            }
+        }
+    }
 }
 
 // A shape without deep primes: its stage-B survivors are final, so one lane per entry
