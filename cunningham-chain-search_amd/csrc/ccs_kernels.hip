@@ -245,6 +245,24 @@ __device__ __forceinline__ bool stage_b_bit(const u32* __restrict__ mem, const T
     return __builtin_amdgcn_ubfe(w, r, 1u) != 0u;
 }
 
+// Group-0 probes of several candidates with their bitset reads all in flight: the words
+// are read first, and a scheduling barrier keeps every read ahead of the first use (left
+// to itself the scheduler reuses the first word's register for the next residue, which
+// waits out each read in turn).
+template <int NBV, int N>
+__device__ __forceinline__ void stage_b_bits0(const u32* __restrict__ lds, const TierBRegs<NBV>& R, const u32 (&j)[N],
+                                              bool (&kill)[N]) {
+    u32 r[N], w[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        r[k] = R.residue(0, j[k] + R.s(0));
+        w[k] = lds[R.off(0) + __builtin_amdgcn_ubfe(r[k], 5u, 11u)];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k = 0; k < N; ++k) kill[k] = __builtin_amdgcn_ubfe(w[k], r[k], 1u) != 0u;
+}
+
 // groups 1..NBV-1 for one candidate (groups past nb_lds live in device memory)
 template <int NBV>
 __device__ __forceinline__ bool stage_b_rest(const u32* __restrict__ lds, const u32* __restrict__ far,
@@ -261,10 +279,11 @@ template <int NBV>
 __device__ __forceinline__ void stage_b_pair(const u32* __restrict__ lds, const u32* __restrict__ far,
                                              u32 nb_lds, const TierBRegs<NBV>& R, u32 ja, u32 jb, bool& pa,
                                              bool& pb) {
-    const bool ba = stage_b_bit<NBV>(lds, R, 0, ja);
-    const bool bb = stage_b_bit<NBV>(lds, R, 0, jb);
-    pa = !ballot(ba) && stage_b_rest<NBV>(lds, far, nb_lds, R, ja);
-    pb = !ballot(bb) && stage_b_rest<NBV>(lds, far, nb_lds, R, jb);
+    const u32 j[2] = {ja, jb};
+    bool b[2];
+    stage_b_bits0<NBV, 2>(lds, R, j, b);
+    pa = !ballot(b[0]) && stage_b_rest<NBV>(lds, far, nb_lds, R, ja);
+    pb = !ballot(b[1]) && stage_b_rest<NBV>(lds, far, nb_lds, R, jb);
 }
 
 template <int NBV>
@@ -285,13 +304,12 @@ __device__ __forceinline__ u64 stage_b_group0(const u32* __restrict__ lds, const
     u64 pass = 0;
 #pragma unroll 1
     for (u32 t = 0; t < n; t += 4) {
-        const u32x4 j = *(const u32x4*)(q + t);
-        const bool b0 = stage_b_bit<NBV>(lds, R, 0, j.x);
-        const bool b1 = stage_b_bit<NBV>(lds, R, 0, j.y);
-        const bool b2 = stage_b_bit<NBV>(lds, R, 0, j.z);
-        const bool b3 = stage_b_bit<NBV>(lds, R, 0, j.w);
-        const u64 live = (u64)(ballot(b0) == 0) | (u64)(ballot(b1) == 0) << 1 | (u64)(ballot(b2) == 0) << 2 |
-                         (u64)(ballot(b3) == 0) << 3;
+        const u32x4 jq = *(const u32x4*)(q + t);
+        const u32 j[4] = {jq.x, jq.y, jq.z, jq.w};
+        bool b[4];
+        stage_b_bits0<NBV, 4>(lds, R, j, b);
+        const u64 live = (u64)(ballot(b[0]) == 0) | (u64)(ballot(b[1]) == 0) << 1 | (u64)(ballot(b[2]) == 0) << 2 |
+                         (u64)(ballot(b[3]) == 0) << 3;
         pass |= live << t;
     }
     return pass;
