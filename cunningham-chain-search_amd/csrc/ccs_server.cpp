@@ -281,31 +281,35 @@ static void server_loop(ccs_server* s) {
             if (s->stop) break;
             t_busy = now_s();
             inits.swap(s->inits);
-            // per-job quota: an even share of the slice target, at least one unit
+            // per-job quota: an even share of the slice target, at least min_chunk; a share
+            // above max_chunk goes into the launch as several pieces (rows of the job table,
+            // each with its own offset), so the slices stay full when few jobs are left
             uint64_t n_work = 0;
             for (uint64_t id : s->active) n_work += s->jobs[id].cursor < s->jobs[id].req.count;
             uint64_t quota = n_work ? s->p.slice_candidates / n_work : 0;
             quota = std::max<uint64_t>(quota, s->p.min_chunk);
-            quota = std::min<uint64_t>(quota, s->p.max_chunk);
             for (uint64_t id : s->active) {
                 Job& j = s->jobs[id];
                 if (j.cursor >= j.req.count) continue;
-                const uint64_t n = std::min<uint64_t>(quota, j.req.count - j.cursor);
-                const uint64_t u0lo = j.req.u0.lo + j.cursor;
-                ccs_job r{};
-                r.a = j.req.a;
-                r.u0.lo = u0lo;
-                r.u0.hi = j.req.u0.hi + (u0lo < j.req.u0.lo);
-                r.count = n;
-                const size_t gi = s->lg_of[j.group];
-                recs[gi].push_back(r);
-                shp[gi].push_back(s->pos_in_lg[j.group]);
-                slots[gi].push_back(j.slot);
-                offs[gi].push_back(j.cursor);   // the resident phases are those at u0
-                plan[gi].push_back(Entry{id, j.cursor});
-                j.cursor += n;
-                j.inflight++;
-                s->stat_candidates += n;
+                for (uint64_t share = std::min<uint64_t>(quota, j.req.count - j.cursor); share > 0;) {
+                    const uint64_t n = std::min<uint64_t>(share, s->p.max_chunk);
+                    const uint64_t u0lo = j.req.u0.lo + j.cursor;
+                    ccs_job r{};
+                    r.a = j.req.a;
+                    r.u0.lo = u0lo;
+                    r.u0.hi = j.req.u0.hi + (u0lo < j.req.u0.lo);
+                    r.count = n;
+                    const size_t gi = s->lg_of[j.group];
+                    recs[gi].push_back(r);
+                    shp[gi].push_back(s->pos_in_lg[j.group]);
+                    slots[gi].push_back(j.slot);
+                    offs[gi].push_back(j.cursor);   // the resident phases are those at u0
+                    plan[gi].push_back(Entry{id, j.cursor});
+                    j.cursor += n;
+                    j.inflight++;
+                    s->stat_candidates += n;
+                    share -= n;
+                }
             }
         }
         // group sets want their jobs grouped by shape: order each launch group's jobs by profile
@@ -429,6 +433,7 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
     if (!s->p.max_chunk) s->p.max_chunk = 1ull << 32;
     if (!s->p.mem_budget_bytes) s->p.mem_budget_bytes = 256ull << 20;
     s->p.max_chunk = std::min<uint64_t>(s->p.max_chunk, 1ull << 32);
+    s->p.min_chunk = std::min<uint64_t>(s->p.min_chunk, s->p.max_chunk);   // one piece per job at least
     s->resident = !(s->p.flags & CCS_SERVER_RECOMPUTE_PHASES);
     for (uint32_t i = 0; i < n_shapes; ++i) {
         if (!shapes[i]) { delete s; return ccs::set_error(CCS_ERR_INVALID, "null shape"); }
@@ -526,7 +531,9 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
                     dshape.push_back(k);
                     left -= c;
                 }
-            while (dummy.size() < s->p.max_active) {
+            // every active job a piece, plus the extra pieces of shares above max_chunk (at most
+            // slice / max_chunk + 1 over the whole slice)
+            while (dummy.size() < s->p.max_active + slice / s->p.max_chunk + 1) {
                 dummy.push_back(dummy_job(per));
                 dshape.push_back(count - 1);
             }

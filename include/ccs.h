@@ -513,7 +513,9 @@ typedef struct {
                                     /*   stage-B register widths differ by at most this     */
                                     /*   factor (0: 2; 1: one launch per kernel instance)   */
     uint64_t slice_candidates;      /* target candidates per slice over all admitted jobs   */
-    uint64_t min_chunk, max_chunk;  /* bounds of one job's chunk per slice (<= 2^32)        */
+    uint64_t min_chunk;             /* least candidates of a job per slice                  */
+    uint64_t max_chunk;             /* largest piece of a job in one launch (<= 2^32); a    */
+                                    /* job's share of a slice may span several pieces       */
     ccs_prove_params prove;         /* prove settings for every job                         */
 } ccs_server_params;
 

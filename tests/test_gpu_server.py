@@ -151,6 +151,24 @@ def test_launch_groups_do_not_change_results(ctx, shapes):
     assert 1 <= st["launches"] / st["slices"] <= len(instances)
 
 
+def test_shares_in_several_pieces_do_not_change_results(ctx, shapes):
+    """a job's share of a slice above max_chunk goes into the launch as several pieces (rows
+    of the job table with their own offsets): the per-job results equal those with one
+    piece per job and slice"""
+    reqs = [r for r in WL.served_requests(n_jobs=60, seed=0x6E, lo_exp=20, hi_exp=23) if r["depth"] in shapes][:10]
+    say(f"[server] shares in pieces, {len(reqs)} jobs")
+    base, st = run_server(ctx, shapes, reqs, slice_candidates=1 << 24, min_chunk=1 << 16)
+    pieces, st_p = run_server(ctx, shapes, reqs, slice_candidates=1 << 24, min_chunk=1 << 16, max_chunk=1 << 17)
+    assert st_p["candidates"] == st["candidates"] == sum(r["count"] for r in reqs)
+    # with 2^24-candidate slices over at most 10 jobs a share spans several 2^17 pieces: the
+    # largest job is done in fewer slices than it has pieces
+    assert st_p["slices"] * (1 << 17) < max(r["count"] for r in reqs)
+    for r in reqs:
+        assert base[r["id"]][0] == pieces[r["id"]][0]
+        for f in ("chain_len", "back_steps", "flags", "root_lo", "root_hi"):
+            assert np.array_equal(base[r["id"]][1][f], pieces[r["id"]][1][f]), f
+
+
 def test_survivor_offsets_beyond_2_32(ctx, oracle, shapes):
     """a request longer than 2^32 candidates: survivors past offset 2^32 are reported at
     their full 64-bit offset, equal to the oracle's class-form filter there"""
