@@ -29,8 +29,8 @@ PUBLISHED_GPU_CAND_PER_S = 65e9
 # WRITE_SIZE 1048576 KiB for 1 GiB written (x1.0).  SQ counters: wave64 VALU and LDS
 # instructions and LDS bank-conflict cycles.
 SIEVE_PROFILE = {"c2": {"tuning": {"n_dense": 41, "n_tierb": 320, "rows_per_group": 4, "unit_j": 1048576},
-                        "fetch_kib": 7790.52, "write_kib": 13835.3, "fetch_scale": 2.0, "write_scale": 1.0,
-                        "sq_insts_valu": 1.73786e9, "sq_insts_lds": 3.63146e8, "sq_lds_bank_conflict": 1.38853e8,
+                        "fetch_kib": 7776.08, "write_kib": 13803.1, "fetch_scale": 2.0, "write_scale": 1.0,
+                        "sq_insts_valu": 1.71303e9, "sq_insts_lds": 3.63146e8, "sq_lds_bank_conflict": 1.38853e8,
                         "profile": "profiles/sieve_kernel_cc18_31primorial_pmc.md"}}
 LDS_PROBE = os.path.join(PKG, "lds_probe")
 
