@@ -329,6 +329,22 @@ def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_c
         plains.append(total / plain_pass(full_sets))
     for js in cal_sets + full_sets:
         js.close()
+    # one launch per depth over long jobs (64 jobs of 2^30 candidates each, the same tuned
+    # shapes): a rate the served form cannot pass, so its ratio to the plain loop caps the
+    # served speed-up for this job mix
+    whole_ms, whole_cand = {}, 0
+    for d, sh in shapes.items():
+        js = ccs.JobSet(sh, [(r["a"], r["u0"], 1 << 30) for r in tune_pool if r["depth"] == d][:64])
+        js.launch(d_out, cap, d_cnt)
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(3):
+            js.launch(d_out, cap, d_cnt)
+        ctx.synchronize()
+        whole_ms[str(d)] = (time.perf_counter() - t) / 3 * 1e3
+        whole_cand += js.candidates
+        js.close()
+    whole = whole_cand / (sum(whole_ms.values()) / 1e3)
     for ptr in (d_out, d_cnt, d_res):
         ctx.dfree(ptr)
     ratios = sorted(c / p for c, p in zip(caps, plains))
@@ -362,6 +378,8 @@ def measure_served(ccs, WL, ctx, n_jobs=10000, load=0.8, tune_trials=16, slice_c
             "plain_loop_candidates_per_s": float(np.median(plains)), "plain_loop_runs": plains,
             "speedup_capacity_vs_plain": ratios[len(ratios) // 2],
             "speedup_runs": {"min": ratios[0], "median": ratios[len(ratios) // 2], "max": ratios[-1], "n": reps},
+            "whole_set_candidates_per_s": whole, "whole_set_ms_per_depth": whole_ms,
+            "speedup_cap_whole_set_vs_plain": whole / float(np.median(plains)),
             "aggregate_candidates_per_s": total / wall, "wall_s": wall,
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_ms": float(lat.mean() * 1e3), "slices": st["slices"], "launches": st["launches"],
