@@ -266,10 +266,12 @@ __device__ __forceinline__ void stage_b_bits0(const u32* __restrict__ lds, const
 // groups 1..NBV-1 for one candidate (groups past nb_lds live in device memory)
 template <int NBV>
 __device__ __forceinline__ bool stage_b_rest(const u32* __restrict__ lds, const u32* __restrict__ far,
-                                             u32 nb_lds, const TierBRegs<NBV>& R, u32 j) {
+                                             u32 nb_lds, u32 nb, const TierBRegs<NBV>& R, u32 j) {
 #pragma unroll
-    for (int i = 1; i < NBV; ++i)
+    for (int i = 1; i < NBV; ++i) {
+        if ((u32)i * 64u >= nb) break;   // a shape with fewer groups than its instance holds
         if (ballot(stage_b_bit<NBV>((u32)i * 64u < nb_lds ? lds : far, R, i, j))) return false;
+    }
     return true;
 }
 
@@ -277,19 +279,19 @@ __device__ __forceinline__ bool stage_b_rest(const u32* __restrict__ lds, const 
 // in flight together; the rare candidates that pass the first group continue alone.
 template <int NBV>
 __device__ __forceinline__ void stage_b_pair(const u32* __restrict__ lds, const u32* __restrict__ far,
-                                             u32 nb_lds, const TierBRegs<NBV>& R, u32 ja, u32 jb, bool& pa,
-                                             bool& pb) {
+                                             u32 nb_lds, u32 nb, const TierBRegs<NBV>& R, u32 ja, u32 jb,
+                                             bool& pa, bool& pb) {
     const u32 j[2] = {ja, jb};
     bool b[2];
     stage_b_bits0<NBV, 2>(lds, R, j, b);
-    pa = !ballot(b[0]) && stage_b_rest<NBV>(lds, far, nb_lds, R, ja);
-    pb = !ballot(b[1]) && stage_b_rest<NBV>(lds, far, nb_lds, R, jb);
+    pa = !ballot(b[0]) && stage_b_rest<NBV>(lds, far, nb_lds, nb, R, ja);
+    pb = !ballot(b[1]) && stage_b_rest<NBV>(lds, far, nb_lds, nb, R, jb);
 }
 
 template <int NBV>
 __device__ __forceinline__ bool stage_b_pass(const u32* __restrict__ lds, const u32* __restrict__ far,
-                                             u32 nb_lds, const TierBRegs<NBV>& R, u32 j) {
-    return !ballot(stage_b_bit<NBV>(lds, R, 0, j)) && stage_b_rest<NBV>(lds, far, nb_lds, R, j);
+                                             u32 nb_lds, u32 nb, const TierBRegs<NBV>& R, u32 j) {
+    return !ballot(stage_b_bit<NBV>(lds, R, 0, j)) && stage_b_rest<NBV>(lds, far, nb_lds, nb, R, j);
 }
 
 typedef u32 u32x4 __attribute__((ext_vector_type(4)));   // one 16-byte LDS block
@@ -689,7 +691,7 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                         const u32 t = (u32)__builtin_ctzll(pass);
                         pass &= pass - 1u;
                         const u32 j = cqb[base + t];
-                        if (stage_b_rest<NBV>(lds, t_far(), t_nb_lds(), R, j)) emit(j);
+                        if (stage_b_rest<NBV>(lds, t_far(), t_nb_lds(), t_nb(), R, j)) emit(j);
                     }
                 }
                 wave_sync();   // the reads before the next writes
@@ -764,7 +766,11 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                 } else {
                     // survivors of the dense stage in uniform control flow: the lanes with a
                     // word not all killed (ballot), then that word's live bits (readlane);
-                    // stage B takes them two at a time
+                    // stage B takes them two at a time, over every group of the instance
+                    // (stopping at the shape's last group, as the queue form does, cost this
+                    // instance 2 % at config 2; in a launch of narrower shapes the groups past
+                    // a shape's own hold q = 1 lanes on a zero word, which never kill)
+                    constexpr u32 kAllGroups = (u32)NBV * 64u;
                     u32 pend = 0xffffffffu;
         #pragma unroll
                     for (int rk = 0; rk < RG * kRowWords; ++rk) {
@@ -784,14 +790,14 @@ __global__ __launch_bounds__(1024) void sieve_kernel(SieveArgs A) {
                                     continue;
                                 }
                                 bool pa, pb;
-                                stage_b_pair<NBV>(lds, t_far(), t_nb_lds(), R, pend, j, pa, pb);
+                                stage_b_pair<NBV>(lds, t_far(), t_nb_lds(), kAllGroups, R, pend, j, pa, pb);
                                 if (pa) emit(pend);
                                 if (pb) emit(j);
                                 pend = 0xffffffffu;
                             }
                         }
                     }
-                    if (pend != 0xffffffffu && stage_b_pass<NBV>(lds, t_far(), t_nb_lds(), R, pend)) emit(pend);
+                    if (pend != 0xffffffffu && stage_b_pass<NBV>(lds, t_far(), t_nb_lds(), kAllGroups, R, pend)) emit(pend);
                 }
                 cnt_dense += cnt_rows;
             }

@@ -455,9 +455,11 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
     for (uint32_t k = s->n_slots; k-- > 0;) s->free_slots.push_back(k);
     int rc = CCS_OK;
     // launch groups: profiles sorted by stage-B register width (widest first); a group
-    // takes the next profile while it has the group's row layout and at least 1/ratio of
-    // the group's widest stage-B width
-    if (!s->p.group_width_ratio) s->p.group_width_ratio = 2;
+    // takes the next profile while it has the group's row layout and (with a ratio set) at
+    // least 1/ratio of the group's widest stage-B width.  By default (ratio 0) every profile
+    // of a row layout shares one launch per slice: one persistent launch at the widest
+    // profile's kernel instance runs the served mix faster than a launch per width class
+    // (fewer launch tails per slice)
     {
         std::vector<uint32_t> order(n_shapes), width(n_shapes), rows(n_shapes);
         for (uint32_t i = 0; i < n_shapes && !rc; ++i) {
@@ -479,7 +481,8 @@ extern "C" int ccs_server_create(ccs_ctx* ctx, const ccs_shape* const* shapes, c
             bool fresh = s->lg_profiles.empty();
             if (!fresh) {
                 const uint32_t head = s->lg_profiles.back().front();
-                fresh = rows[head] != rows[i] || s->p.group_width_ratio * width[i] < width[head];
+                fresh = rows[head] != rows[i] ||
+                        (s->p.group_width_ratio && s->p.group_width_ratio * width[i] < width[head]);
             }
             if (fresh) {
                 s->lg_profiles.emplace_back();

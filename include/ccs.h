@@ -511,7 +511,8 @@ typedef struct {
     uint32_t max_slice_survivors;   /* survivor buffer per slice and depth group            */
     uint32_t group_width_ratio;     /* depth profiles share a launch per slice while their */
                                     /*   stage-B register widths differ by at most this     */
-                                    /*   factor (0: 2; 1: one launch per kernel instance)   */
+                                    /*   factor (0: no limit, one launch per row layout;    */
+                                    /*   1: one launch per kernel instance)                 */
     uint64_t slice_candidates;      /* target candidates per slice over all admitted jobs   */
     uint64_t min_chunk;             /* least candidates of a job per slice                  */
     uint64_t max_chunk;             /* largest piece of a job in one launch (<= 2^32); a    */
