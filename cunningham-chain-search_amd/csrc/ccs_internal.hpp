@@ -108,7 +108,7 @@ struct DeepPrime {
     u32 _pad;
 };
 constexpr double kDeepMaskSurvival = 1e-6;
-constexpr unsigned kDeepBatch = 32;   // deep-stage records per wave batch (survivor-rich launches)
+constexpr unsigned kDeepBatch = 64;   // deep-stage records per wave batch (survivor-rich launches)
 constexpr unsigned kDeepBatchMin = 12;       // smaller batches run slower than one record per wave
 constexpr unsigned kDeepFillWaves = 8192;    // resident waves of the deep kernel (256 CUs x 32)
 

@@ -370,7 +370,7 @@ int ccs_stream_status(ccs_ctx* ctx, void* stream, ccs_launch_status* out);
  * their stage-B buffer (0 = all of it), so tests can force the overflow path. */
 int ccs_debug_limit_stage_b(ccs_ctx* ctx, uint32_t entries);
 /* Test knob (no reference counterpart): the deep stage of launches on the context takes
- * `per_wave` stage-B records per wave batch (1: one at a time; 2..32: batches of that
+ * `per_wave` stage-B records per wave batch (1: one at a time; 2..64: batches of that
  * many; 0 = chosen from the launch's count, the default), so tests can run either form
  * on any input.  Results do not depend on it. */
 int ccs_debug_deep_batch(ccs_ctx* ctx, uint32_t per_wave);

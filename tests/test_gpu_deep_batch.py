@@ -49,7 +49,7 @@ def oracle_shape(oracle, depth):
 
 def test_deep_batch_knob_bounds(ctx):
     with pytest.raises(ccs.CcsError):
-        ctx.deep_batch(33)
+        ctx.deep_batch(65)
     ctx.deep_batch(0)
 
 
@@ -63,7 +63,7 @@ def test_each_form_equals_oracle(ctx, oracle):
     say(f"[deep batch] depth 12, {len(reqs)} jobs, {js.candidates} candidates, {len(want)} survivors")
     assert len(want) > 10
     base = None
-    for per in (1, 2, 3, 17, 32):
+    for per in (1, 2, 3, 17, 32, 64):
         got, st = launch(ctx, js, per)
         say(f"[deep batch] per_wave {per}: {len(got)} survivors, {st['alive_tierb']} stage-B records")
         assert got == want
