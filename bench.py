@@ -546,6 +546,41 @@ def roofline(shape, name, cand, ms_sieve, e_tests, value, probe=None):
 
 
 
+def served_depth(args):
+    """One served depth profile on its own: 64 of the stream's jobs of that depth, 2^30
+    candidates each (2^36 in all), as one job-set launch (filter: sieve + deep stage) per
+    step, at the profile's default tuning or --tuning; prints one JSON line (a profiling
+    target for the served kernels, not the headline)."""
+    import ccs
+    from ccs import workloads as WL
+    ctx = ccs.Context(0)
+    sh = WL.make_shape(ctx, f"c5d{args.served_depth}", classes=False)
+    if args.tuning:
+        sh.set_tuning(**json.loads(args.tuning))
+    reqs = [r for r in WL.served_requests(n_jobs=2000, seed=0xC5) if r["depth"] == args.served_depth][:64]
+    js = ccs.JobSet(sh, [(r["a"], r["u0"], 1 << 30) for r in reqs])
+    cap = 1 << 22
+    d_out, d_cnt = ctx.dalloc(cap * 8), ctx.dalloc(64)
+    for _ in range(args.warmup):
+        js.launch(d_out, cap, d_cnt)
+    ctx.synchronize()
+    ctx.profile_start()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        js.launch(d_out, cap, d_cnt)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t) / args.steps
+    kt = ctx.profile_stop()
+    st = js.stats(d_cnt)
+    print(json.dumps({"workload": f"served depth {args.served_depth}: 64 jobs of 2^30 candidates, classes mod 31#",
+                      "candidates_per_step": js.candidates, "ms_per_step": dt * 1e3,
+                      "candidates_per_s": js.candidates / dt, "tuning": sh.tuning,
+                      "kernels_ms": {"sieve": kt["ms_sieve"] / max(kt["n_sieve"], 1),
+                                     "deep": kt["ms_deep"] / max(kt["n_deep"], 1)},
+                      "alive_dense": st["alive_dense"], "alive_tierb": st["alive_tierb"],
+                      "survivors": st["survivors"]}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -563,6 +598,8 @@ def main():
     ap.add_argument("--tuning", default=None, help="JSON tuning to use instead of the autotuner")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (scales rocprofv3's FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--served-depth", type=int, default=0,
+                    help="only time one served depth profile's job-set launch (64 jobs of 2^30 candidates)")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -572,6 +609,9 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": a.numel() * 4}), flush=True)
+        return
+    if args.served_depth:
+        served_depth(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this pipeline shards by disjoint tile ranges with no "
